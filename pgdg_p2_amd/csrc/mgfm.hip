@@ -1,0 +1,64 @@
+// mgfm.hip - batched MGFM interface Riemann solve (replaces MPM::MGFM,
+// S3:mpm.cpp:882-926 / T2:mpm.cpp:1373-1416, once per face pair).
+//
+// Layout: SoA FP64 columns so every load and store of a wavefront is one
+// contiguous 512-byte segment; one face per lane, grid-stride loop.
+#include "hip_util.hpp"
+#include "riemann.hpp"
+
+namespace pgdg {
+
+__global__ void __launch_bounds__(256) mgfm_batch_kernel(
+    const double* __restrict__ in, double* __restrict__ out, int32_t* __restrict__ iters,
+    int64_t n, Eos ml, Eos mr, int variant, int tol_mode, double rtol, int max_iter,
+    const double* __restrict__ pstar0) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double ul = in[i];
+    double ur = in[n + i];
+    double pl = in[2 * n + i];
+    double pr = in[3 * n + i];
+    double rl = in[4 * n + i];
+    double rr = in[5 * n + i];
+    double g = pstar0 ? pstar0[i] : -1.0;
+    MgfmOut o = mgfm_solve(ul, ur, pl, pr, rl, rr, ml, mr, variant, tol_mode, rtol, max_iter, g);
+    out[i] = o.rho_l;
+    out[n + i] = o.pstar;
+    out[2 * n + i] = o.u_l;
+    if (iters) iters[i] = o.iters;
+  }
+}
+
+}  // namespace pgdg
+
+extern "C" int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, int64_t n,
+                               const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts,
+                               const double* pstar0, void* stream) {
+  using namespace pgdg;
+  if (n < 0 || !eos || !opts) return PGDG_ERR_ARG;
+  if (n == 0) return PGDG_OK;
+  if (!in || !out) return PGDG_ERR_ARG;
+  if (opts->variant != MGFM_S3 && opts->variant != MGFM_T2) return PGDG_ERR_ARG;
+  if (opts->tol_mode != TOL_REF && opts->tol_mode != TOL_REL) return PGDG_ERR_ARG;
+  if (opts->max_iter < 1) return PGDG_ERR_ARG;
+  Eos ml{eos->gamma_l, eos->pinf_l};
+  Eos mr{eos->gamma_r, eos->pinf_r};
+  const int block = 256;
+  unsigned grid = grid_for(n, block, 256 * 64);
+  hipLaunchKernelGGL(mgfm_batch_kernel, dim3(grid), dim3(block), 0, as_stream(stream), in, out,
+                     iters, n, ml, mr, opts->variant, opts->tol_mode, opts->rtol, opts->max_iter,
+                     pstar0);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+extern "C" const char* pgdg_version(void) { return "pgdg-mi355x 0.1 gfx950"; }
+
+extern "C" int pgdg_device_count(int* count) {
+  if (!count) return PGDG_ERR_ARG;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return PGDG_OK;
+}

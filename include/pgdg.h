@@ -90,6 +90,92 @@ int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, int64_t n,
                     const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts,
                     const double* pstar0, void* stream);
 
+/* ======================================================================== *
+ *  PG-DG time step: case description and host state layout                *
+ * ======================================================================== */
+
+/* Boundary-face tags, as in the reference Cell_dg::bound
+ * (S3:mpm_ini.cpp:447-549): 0 interior, 1 wall, -1 inlet, -2 outlet,
+ * -3 symmetry.                                                           */
+#define PGDG_BC_INTERIOR  0
+#define PGDG_BC_WALL      1
+#define PGDG_BC_INLET    -1
+#define PGDG_BC_OUTLET   -2
+#define PGDG_BC_SYMMETRY -3
+
+/* Side order of pgdg_case.bc, the reference face order of a cell:
+ * yL, xR, yR, xL, zL, zR (S3:mpm_ini.cpp:447-549; T2:mpm_ini.cpp:329-375). */
+enum { PGDG_SIDE_YL = 0, PGDG_SIDE_XR, PGDG_SIDE_YR, PGDG_SIDE_XL, PGDG_SIDE_ZL, PGDG_SIDE_ZR };
+
+/* Semantics of the reference source the case follows.
+ *   PGDG_SEM_T2 : 2-D triple point (T2/T3): every boundary face is a wall in
+ *                 the DG flux, T2 MGFM, per-phase v in the cell acceleration,
+ *                 no inlet/outlet and no mark_in;
+ *   PGDG_SEM_S3 : 3-D generalized code (S3): inlet/outlet/symmetry faces,
+ *                 S3 MGFM, inlet particle copies and outlet removal.      */
+#define PGDG_SEM_T2 2
+#define PGDG_SEM_S3 3
+
+#define PGDG_MAX_PHASE 3
+
+typedef struct pgdg_case {
+  int    dim;            /* D: 2 or 3                                      */
+  int    nphase;         /* F: 2 or 3 (reference FldNum)                   */
+  int    order;          /* 1: P1 basis (1,x,y[,z]) of the reference        */
+  int    rk_stages;      /* 2: the reference's two-stage Heun update        */
+  int    semantics;      /* PGDG_SEM_T2 or PGDG_SEM_S3                     */
+  int    n[3];           /* cells per axis (n[2] = 1 when dim == 2)         */
+  int    bc[6];          /* tag of each domain side, PGDG_SIDE_* order      */
+  double box_l[3], box_r[3];
+  double p_sz;           /* particle spacing (reference p_sz)               */
+  double p_g;            /* grid spacing used for kernel widths (p_g)       */
+  double gamma[PGDG_MAX_PHASE], pinf[PGDG_MAX_PHASE], ini_rho[PGDG_MAX_PHASE];
+  double inlet_rho, inlet_u[3], inlet_p;   /* inlet ghost state (S3)       */
+  double p2g_cut;        /* P2G particle cut-off distance; the reference
+                            uses 1.5*p_sz (S3:mpm.cpp:307), which covers a
+                            whole cell only at one particle per cell       */
+  double output_dt;      /* dt is truncated to land on multiples of this    */
+  double max_v0;         /* initial max_v (reference 1e-7)                  */
+} pgdg_case;
+
+/* Host SoA views.  Arrays are sized by capacity `cap` (particles) or by the
+ * cell count nc = n0*n1*n2 (cells, index (i*n1 + j)*n2 + k); vector fields
+ * are stored component-major ([3][cap]); coef is [U*M][cap] with entry
+ * u*M + m.  U = dim+2, M = dim+1 for order 1.                              */
+typedef struct pgdg_particles {
+  int64_t n, cap;
+  double *x, *v, *tv, *rhov;                 /* [3][cap]                 */
+  double *vol, *mass, *h, *P, *rho, *E, *c;  /* [cap]                    */
+  double *coef;                              /* [U*M][cap]               */
+  int32_t *phase, *mark_in;                  /* [cap]                    */
+} pgdg_particles;
+
+typedef struct pgdg_walls {                  /* boundary particles (phase 10) */
+  int64_t n;
+  double *x, *norm;                          /* [3][n]                   */
+  double *vol;                               /* [n]                      */
+} pgdg_walls;
+
+typedef struct pgdg_cells {
+  double *coef, *coef0, *dcoef;   /* [F][U][M][nc]                         */
+  double *lim;                    /* [F][U][nc]                            */
+  double *alph, *rho, *E, *c, *ac;/* [F][nc]                               */
+  double *rhov;                   /* [F][3][nc]                            */
+  int32_t *ph_state, *ph_state_b; /* [F][nc]                               */
+  double *v, *a;                  /* [3][nc]  mixture velocity, acceleration */
+  double *P;                      /* [nc]                                  */
+  double *G_v;                    /* [9][nc]  velocity gradient, row-major  */
+} pgdg_cells;
+
+typedef struct pgdg_scalars {
+  double  dt;             /* glbl_timestep                                 */
+  double  max_v;          /* max |v| of the last Set_timestep              */
+  double  integral_time;  /* time since the last output instant            */
+  double  run_time;
+  int64_t iterate_num;    /* steps taken                                   */
+  int64_t n_split, n_inlet, n_outlet, n_merge;  /* event counts, last step */
+} pgdg_scalars;
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,63 @@
+"""ctypes binding of the oracle's step API (TEST INFRASTRUCTURE ONLY)."""
+from __future__ import annotations
+
+import ctypes as C
+
+from pgdg_p2_amd.state import Case, Cells, HostState, Particles, Scalars, Walls
+
+FN = {"set_timestep": 0, "used_state_update": 1, "bin": 2, "split": 3, "merge": 4,
+      "membership": 5, "neighbors": 6, "p2g": 7, "limiter": 8, "rhs": 9, "update_grid": 10,
+      "update_particle": 11, "run_simulation": 12}
+
+
+def declare(L: C.CDLL) -> None:
+    L.oracle_create.restype = C.c_void_p
+    L.oracle_create.argtypes = [C.POINTER(Case)]
+    L.oracle_destroy.argtypes = [C.c_void_p]
+    L.oracle_load.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(Walls),
+                              C.POINTER(Cells), C.POINTER(Scalars)]
+    L.oracle_save.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(Cells),
+                              C.POINTER(Scalars)]
+    L.oracle_step.argtypes = [C.c_void_p, C.c_int]
+    L.oracle_call.argtypes = [C.c_void_p, C.c_int, C.c_int]
+
+
+class OracleSim:
+    """The oracle's restatement of the reference step, driven from Python."""
+
+    def __init__(self, case: Case, st: HostState):
+        from oracle.pyoracle import lib
+        self.L = lib()
+        self.case = case
+        self.h = self.L.oracle_create(C.byref(case))
+        if not self.h:
+            raise RuntimeError("oracle_create failed")
+        self.load(st)
+
+    def load(self, st: HostState):
+        P, W, Cc = st.particles(), st.walls(), st.cells()
+        rc = self.L.oracle_load(self.h, C.byref(P), C.byref(W), C.byref(Cc), C.byref(st.s))
+        assert rc == 0, rc
+
+    def save(self, st: HostState):
+        P, Cc = st.particles(), st.cells()
+        rc = self.L.oracle_save(self.h, C.byref(P), C.byref(Cc), C.byref(st.s))
+        assert rc == 0, rc
+        st.n = P.n
+        return st
+
+    def step(self, n=1):
+        rc = self.L.oracle_step(self.h, n)
+        assert rc == 0, rc
+
+    def call(self, fn: str, arg: int = 0):
+        rc = self.L.oracle_call(self.h, FN[fn], arg)
+        assert rc == 0, rc
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.L.oracle_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

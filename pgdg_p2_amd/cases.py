@@ -1,0 +1,228 @@
+"""Case set-up: geometry, boundary tags, EOS tables and initial particle/cell
+states (the reference's MPM::Initialize_case, S3:mpm_ini.cpp:33-673 and
+T2:mpm_ini.cpp:33-482, generalised to several particles per cell with a
+seeded jitter).
+
+Particles are placed on a lattice of spacing p_sz = p_g / ppc_axis at
+(i + 1/2) p_sz (S3:mpm_ini.cpp:177-184), optionally jittered by
+U(-jitter, jitter) * p_sz per axis (numpy PCG64, fixed seed).  Wall particles
+sit on the boundary lattice with the reference's outward normals
+(S3:mpm_ini.cpp:555-620, T2:mpm_ini.cpp:385-430).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .state import (BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_WALL, Case, HostState, SEM_S3,
+                    SEM_T2)
+
+CELL_RATIO, CUT_OFF = 3.2, 2.0
+
+
+def _base_case(dim, nphase, n, box_l, box_r, semantics, bc, p_g, ppc_axis):
+    c = Case()
+    c.dim, c.nphase, c.order, c.rk_stages, c.semantics = dim, nphase, 1, 2, semantics
+    for a in range(3):
+        c.n[a] = n[a] if a < dim else 1
+        c.box_l[a] = box_l[a] if a < dim else 0.0
+        c.box_r[a] = box_r[a] if a < dim else 1.0
+    for k in range(6):
+        c.bc[k] = bc[k]
+    # reference: p_sz = (px py [pz])^(1/D) / round(p_g / p_sz)
+    ps = [(c.box_r[a] - c.box_l[a]) / c.n[a] for a in range(dim)]
+    prod = ps[0] * ps[1] if dim == 2 else ps[0] * ps[1] * ps[2]
+    c.p_g = p_g
+    c.p_sz = (math.sqrt(prod) if dim == 2 else prod ** (1.0 / 3.0)) / round(ppc_axis)
+    # P2G cut-off: the reference's 1.5*p_sz at one particle per cell; with
+    # several particles per cell 1.5*p_g (equal to it at one per cell) so
+    # that every particle of a cell reaches the cell centre.
+    c.p2g_cut = 1.5 * (c.p_sz if round(ppc_axis) == 1 else c.p_g)
+    c.output_dt = 1e30
+    c.max_v0 = 1e-7
+    return c
+
+
+def _lattice(case: Case, jitter: float, seed: int):
+    D = case.dim
+    counts = [round((case.box_r[a] - case.box_l[a]) / case.p_sz) for a in range(D)]
+    axes = [(np.arange(counts[a]) + 0.5) * case.p_sz + case.box_l[a] for a in range(D)]
+    grids = np.meshgrid(*axes, indexing="ij")  # i, j, k order (S3:mpm_ini.cpp:177-179)
+    pos = np.stack([g.reshape(-1) for g in grids])
+    if jitter > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = pos + rng.uniform(-jitter, jitter, pos.shape) * case.p_sz
+    return pos
+
+
+def _fill_particles(st: HostState, case: Case, pos, phase, rho, p, vel):
+    D, U, M = case.dim, case.U, case.M
+    n = pos.shape[1]
+    st.n = n
+    P = st.p
+    P["x"][:D, :n] = pos
+    g = np.array(case.gamma[: case.nphase])[phase - 1]
+    pinf = np.array(case.pinf[: case.nphase])[phase - 1]
+    vol = case.p_sz ** D if D == 2 else (case.p_sz * case.p_sz) * case.p_sz
+    P["vol"][:n] = vol
+    P["mass"][:n] = vol * rho
+    P["h"][:n] = CELL_RATIO / CUT_OFF * case.p_sz
+    P["phase"][:n] = phase
+    P["rho"][:n] = rho
+    P["P"][:n] = p
+    P["v"][:D, :n] = vel
+    P["rhov"][:D, :n] = vel * rho
+    sq = (vel * vel).sum(axis=0) if vel.size else 0.0
+    P["E"][:n] = (p + g * pinf) / (g - 1) + 0.5 * rho * sq
+    P["c"][:n] = np.sqrt(g * (p + pinf) / rho)
+    P["tv"][:, :n] = 0.0
+    P["coef"][:, :n] = 0.0
+    P["mark_in"][:n] = 0
+
+
+def _init_cells(st: HostState, case: Case):
+    F = case.nphase
+    c = st.c
+    for s in range(F):
+        c["rho"][s] = case.ini_rho[s]
+        c["lim"][s] = 1.0
+        c["c"][s] = math.sqrt(case.gamma[s] * (case.pinf[s] + 0) / case.ini_rho[s])
+        c["ph_state"][s] = 1
+        c["ph_state_b"][s] = 0
+    st.s.max_v = case.max_v0
+
+
+def _walls_nodes(case: Case, spacing, vol):
+    """Boundary nodes of a lattice with `spacing` per axis, in sx, sy[, sz] order."""
+    D = case.dim
+    cnt = [round((case.box_r[a] - case.box_l[a]) / spacing[a]) for a in range(D)]
+    xs, ns = [], []
+    rng = [range(cnt[a] + 1) for a in range(D)]
+    import itertools
+    for idx in itertools.product(*rng):
+        on = [idx[a] == 0 or idx[a] == cnt[a] for a in range(D)]
+        if not any(on):
+            continue
+        x = [case.box_l[a] + spacing[a] * (idx[a] + 0.0) for a in range(D)]
+        nrm = [0.0, 0.0, 0.0]
+        # first matching condition: sx==0, sx==N, sy==0, sy==N, sz==0, sz==N
+        for a in range(D):
+            if idx[a] == 0:
+                nrm[a] = -1.0
+                break
+            if idx[a] == cnt[a]:
+                nrm[a] = 1.0
+                break
+        xs.append(x + [0.0] * (3 - D))
+        ns.append(nrm)
+    x = np.array(xs).T.copy()
+    nrm = np.array(ns).T.copy()
+    return x, nrm, np.full(x.shape[1], vol)
+
+
+def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nphase=2,
+                    cap_factor=3.0, gammas=None):
+    """2-D triple point on [0,7]x[0,3] with T2 semantics (all sides walls).
+
+    Two phases (default): phase 1 gamma=1.5 holds the left block x<=1
+    (rho=1, p=1) and the bottom-right block (rho=1, p=0.1); phase 2
+    gamma=1.4 the top-right block (rho=0.125, p=0.1).  Region tests follow
+    T2:mpm_ini.cpp:159,200 (x > 1, y < 1.5).
+    Three phases: phase 1 left (gamma 1.5), phase 2 bottom-right (gamma 1.4),
+    phase 3 top-right (gamma 1.67).
+    """
+    box_l, box_r = (0.0, 0.0, 0.0), (7.0, 3.0, 1.0)
+    p_g = 3.0 / ny
+    case = _base_case(2, nphase, (nx, ny, 1), box_l, box_r, SEM_T2, [BC_WALL] * 6, p_g, ppc_axis)
+    if nphase == 2:
+        g = gammas or (1.5, 1.4)
+        rho0 = (1.0, 0.125)
+    else:
+        g = gammas or (1.5, 1.4, 1.67)
+        rho0 = (1.0, 1.0, 0.125)
+    for s in range(nphase):
+        case.gamma[s] = g[s]
+        case.pinf[s] = 0.0
+        case.ini_rho[s] = rho0[s]
+    pos = _lattice(case, jitter, seed)
+    x, y = pos[0], pos[1]
+    right = x > 1
+    bottom = right & (y < 1.5)
+    top = right & ~bottom
+    n = pos.shape[1]
+    phase = np.ones(n, dtype=np.int32)
+    rho = np.ones(n)
+    p = np.ones(n)
+    rho[top] = 0.125
+    p[right] = 0.1
+    if nphase == 2:
+        phase[top] = 2
+    else:
+        phase[bottom] = 2
+        phase[top] = 3
+    vel = np.zeros((2, n))
+    # wall particles on the particle lattice, vol = p_sz (T2:mpm_ini.cpp:385-409)
+    pg = [round((box_r[a] - box_l[a]) / case.p_sz) for a in range(2)]
+    spacing = [(box_r[a] - box_l[a]) / pg[a] for a in range(2)]
+    wx, wn, wv = _walls_nodes(case, spacing, case.p_sz)
+    st = HostState(case, int(n * cap_factor) + 64, wx.shape[1])
+    _fill_particles(st, case, pos, phase, rho, p, vel)
+    st.w["x"][:] = wx
+    st.w["norm"][:] = wn
+    st.w["vol"][:] = wv
+    _init_cells(st, case)
+    return case, st
+
+
+def rankine_hugoniot(mach, gamma, rho1, p1):
+    """Post-shock (rho2, u2, p2) behind a shock of Mach `mach` entering gas at rest."""
+    g = gamma
+    p2 = p1 * (1 + 2 * g / (g + 1) * (mach * mach - 1))
+    rho2 = rho1 * ((g + 1) * mach * mach) / ((g - 1) * mach * mach + 2)
+    c1 = math.sqrt(g * p1 / rho1)
+    us = mach * c1
+    u2 = us * (1 - rho1 / rho2)
+    return rho2, u2, p2
+
+
+def shock_bubble_3d(nx=256, ny=128, nz=128, ppc_axis=2, jitter=0.25, seed=0x5EED0002,
+                    x_shock=0.2, cap_factor=2.5, length=2.0):
+    """3-D shock-bubble channel [0,L]x[0,1]x[0,1] with S3 semantics.
+
+    Air (gamma 1.4, rho 1, p 1) holds a helium sphere (gamma 1.648,
+    rho 0.138, p 1) of radius 0.25 centred at (0.5, 0.5, 0.5); a Mach 1.22
+    shock starts at x = x_shock and the inlet (x = 0) feeds the post-shock
+    state (Rankine-Hugoniot).  Side tags follow S3:mpm_ini.cpp:447-549:
+    yL/zL symmetry, yR/zR wall, xL inlet, xR outlet.
+    """
+    box_l, box_r = (0.0, 0.0, 0.0), (length, 1.0, 1.0)
+    p_g = 1.0 / ny
+    bc = [BC_SYMMETRY, BC_OUTLET, BC_WALL, BC_INLET, BC_SYMMETRY, BC_WALL]
+    case = _base_case(3, 2, (nx, ny, nz), box_l, box_r, SEM_S3, bc, p_g, ppc_axis)
+    case.gamma[0], case.pinf[0], case.ini_rho[0] = 1.4, 0.0, 1.0
+    case.gamma[1], case.pinf[1], case.ini_rho[1] = 1.648, 0.0, 0.138
+    rho2, u2, p2 = rankine_hugoniot(1.22, 1.4, 1.0, 1.0)
+    case.inlet_rho, case.inlet_p = rho2, p2
+    case.inlet_u[0], case.inlet_u[1], case.inlet_u[2] = u2, 0.0, 0.0
+    pos = _lattice(case, jitter, seed)
+    n = pos.shape[1]
+    phase = np.ones(n, dtype=np.int32)
+    rho = np.ones(n)
+    p = np.ones(n)
+    vel = np.zeros((3, n))
+    post = pos[0] < x_shock
+    rho[post], p[post], vel[0, post] = rho2, p2, u2
+    d2 = (pos[0] - 0.5) ** 2 + (pos[1] - 0.5) ** 2 + (pos[2] - 0.5) ** 2
+    he = (~post) & (d2 <= 0.25 ** 2)
+    phase[he], rho[he] = 2, 0.138
+    # walls on the grid nodes, vol = p_sz^(D-1) (S3:mpm_ini.cpp:555-620)
+    spacing = [(box_r[a] - box_l[a]) / case.n[a] for a in range(3)]
+    wx, wn, wv = _walls_nodes(case, spacing, case.p_sz * case.p_sz)
+    st = HostState(case, int(n * cap_factor) + 64, wx.shape[1])
+    _fill_particles(st, case, pos, phase, rho, p, vel)
+    st.w["x"][:] = wx
+    st.w["norm"][:] = wn
+    st.w["vol"][:] = wv
+    _init_cells(st, case)
+    return case, st

@@ -1,7 +1,30 @@
 """ctypes signatures of the step-level C ABI (include/pgdg.h)."""
 import ctypes as C
 
+from .state import Case, Cells, Particles, Scalars, Walls
+
+_CTX = C.c_void_p
+_STAGES = ["pgdg_set_timestep", "pgdg_used_state_update", "pgdg_map_particles_to_cells",
+           "pgdg_split_particle_fast", "pgdg_merge_particle_fast", "pgdg_refresh_cell_membership",
+           "pgdg_mls_p2g", "pgdg_calculate_limiter", "pgdg_grid_acceleration"]
+
 
 def declare(L: C.CDLL) -> None:
-    """Attach argtypes/restype for every step-level entry point present."""
-    return None
+    L.pgdg_create.argtypes = [C.POINTER(Case), C.c_int64, C.c_int64, C.POINTER(_CTX)]
+    L.pgdg_destroy.argtypes = [_CTX]
+    L.pgdg_upload_state.argtypes = [_CTX, C.POINTER(Particles), C.POINTER(Walls),
+                                    C.POINTER(Cells), C.POINTER(Scalars)]
+    L.pgdg_download_state.argtypes = [_CTX, C.POINTER(Particles), C.POINTER(Cells),
+                                      C.POINTER(Scalars)]
+    L.pgdg_step.argtypes = [_CTX, C.c_int]
+    L.pgdg_stage.argtypes = [_CTX, C.c_int, C.c_int]
+    L.pgdg_sync.argtypes = [_CTX]
+    L.pgdg_get_stream.argtypes = [_CTX, C.POINTER(C.c_void_p)]
+    L.pgdg_particle_count.argtypes = [_CTX, C.POINTER(C.c_int64)]
+    L.pgdg_memory_bytes.argtypes = [_CTX, C.POINTER(C.c_int64)]
+    for name in _STAGES:
+        getattr(L, name).argtypes = [_CTX]
+    L.pgdg_refresh_neighbor_info.argtypes = [_CTX, C.c_int]
+    L.pgdg_update_grid.argtypes = [_CTX, C.c_int]
+    L.pgdg_update_particle.argtypes = [_CTX, C.c_int]
+    L.pgdg_run_simulation.argtypes = [_CTX, C.c_int64]

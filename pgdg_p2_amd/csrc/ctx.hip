@@ -1,0 +1,531 @@
+// ctx.hip - the step context (device state of one case on one GPU) and the
+// C ABI of the time step.
+//
+// pgdg_step follows Solver::MPM_Solver's loop body (S3:solver.cpp:33-51):
+// Set_timestep, the output-time truncation of dt, then
+// MPM::MPM_run_simulation (S3:mpm_ini.cpp:682-766):
+//   Used_state_update (first step), binning, Split_particle_fast,
+//   Merge_particle_fast x4, cell membership, and two RK stages of
+//   [neighbour lists, MLS_P2G (stage 0), Calculate_limiter,
+//    Grid_acceleration, Update_grid, Update_particle].
+// All work is enqueued on the context's stream; dt and the loop scalars stay
+// on the device.  Split and merge read a few counts back to size their
+// appended/removed particle sets.
+#include <string.h>
+
+#include <cmath>
+#include <new>
+
+#include "dev_common.hpp"
+#include "hip_util.hpp"
+#include "step_kernels.hpp"
+#include "step_topo.hpp"
+
+namespace pgdg {
+
+__global__ void k_used_clear(int32_t* used, int nc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nc) used[c] = 0;
+}
+
+// Used_state_update (S3:mpm.cpp:184-263): cells within +-5 of a cell whose
+// phase 0 or 1 is present.
+__global__ void k_used_mark(Geo g, const int32_t* ph_state, int32_t* used, int D) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  if (!(ph_state[c] == 1 || ph_state[g.nc + c] == 1)) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  const int rk = D == 3 ? 5 : 0;
+  for (int t = max(i - 5, 0); t <= min(i + 5, g.N[0] - 1); t++)
+    for (int s = max(j - 5, 0); s <= min(j + 5, g.N[1] - 1); s++)
+      for (int q = max(k - rk, 0); q <= min(k + rk, g.N[2] - 1); q++) used[(t * g.N[1] + s) * g.N[2] + q] = 1;
+}
+
+struct Ctx {
+  pgdg_case cs;
+  Geo g;
+  int D, U, M, F, nc;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  ParticleArrays PA;
+  int64_t np = 0;
+  WallDev W{};
+  double* w_slab = nullptr;
+  CellDev C{};
+  double* c_slab = nullptr;
+  int32_t* ci_slab = nullptr;
+  int32_t* used = nullptr;
+  StageBufs S{};
+  double* s_slab = nullptr;
+  Topo T;
+  DevScalars* sc = nullptr;
+  int64_t iterate_num = 0;
+  int64_t n_split = 0, n_inlet = 0, n_outlet = 0, n_merge = 0;
+  bool loaded = false;
+  size_t bytes = 0;
+
+  int init(const pgdg_case* c, int64_t cap, int64_t nwall);
+  void release();
+  int upload(const pgdg_particles* p, const pgdg_walls* w, const pgdg_cells* c,
+             const pgdg_scalars* s);
+  int download(pgdg_particles* p, pgdg_cells* c, pgdg_scalars* s);
+  template <int D_>
+  int set_timestep();
+  template <int D_>
+  int run_simulation(int64_t n);
+  template <int D_>
+  int stage(int fn, int arg);
+  int step(int nsteps);
+};
+
+static void fill_geo(const pgdg_case& cs, Geo& g) {
+  memset(&g, 0, sizeof(g));
+  const int D = cs.dim;
+  for (int a = 0; a < 3; a++) {
+    g.N[a] = a < D ? cs.n[a] : 1;
+    g.ps[a] = a < D ? (cs.box_r[a] - cs.box_l[a]) / double(g.N[a]) : 1.0;
+    g.box_l[a] = cs.box_l[a];
+    g.box_r[a] = cs.box_r[a];
+  }
+  g.nc = g.N[0] * g.N[1] * g.N[2];
+  g.F = cs.nphase;
+  g.sem = cs.semantics;
+  for (int f = 0; f < 6; f++) g.bc[f] = cs.bc[f];
+  g.p_sz = cs.p_sz;
+  g.p_g = cs.p_g;
+  g.p2g_cut = cs.p2g_cut;
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    g.gamma[s] = cs.gamma[s];
+    g.pinf[s] = cs.pinf[s];
+    g.ini_rho[s] = cs.ini_rho[s];
+  }
+  g.inlet_rho = cs.inlet_rho;
+  for (int a = 0; a < 3; a++) g.inlet_u[a] = cs.inlet_u[a];
+  g.inlet_p = cs.inlet_p;
+  g.radius = (int)std::ceil(kCellRatio / (cs.p_g / cs.p_sz));  // S3:level_infor.cpp:155
+  g.Gp[0] = -1. / std::sqrt(3.);  // S3:mpm_ini.cpp:37-41
+  g.Gp[1] = 1. / std::sqrt(3.);
+  g.Gw[0] = g.Gw[1] = 0.5;
+  for (int jj = 0; jj < 4; jj++) {  // limiter probe angles, S3:mpm.cpp:697-698
+    double theta = (jj + 0.5) * kPiRef * 0.5;
+    g.probe_c[jj] = std::cos(theta);
+    g.probe_s[jj] = std::sin(theta);
+  }
+  g.sqrt2 = std::sqrt(2.);
+  double pd = D == 3 ? (cs.p_sz * cs.p_sz) * cs.p_sz : cs.p_sz * cs.p_sz;
+  g.vol_max = pd * 1.6;          // S3:mpm.cpp:1068
+  g.vol_min = pd * 2. / 3.;      // S3:mpm.cpp:1540
+  g.dss0 = D == 3 ? root_d<3>(g.vol_min) : root_d<2>(g.vol_min);
+}
+
+int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
+  cs = *c;
+  D = cs.dim;
+  U = D + 2;
+  M = D + 1;
+  F = cs.nphase;
+  fill_geo(cs, g);
+  nc = g.nc;
+  PGDG_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  own_stream = true;
+  PGDG_TRY(PA.init(U, M, cap));
+  // walls
+  W.n = nwall;
+  PGDG_HIP_TRY(hipMalloc((void**)&w_slab, sizeof(double) * 7 * (nwall > 0 ? nwall : 1)));
+  W.x = w_slab;
+  W.norm = w_slab + 3 * nwall;
+  W.vol = w_slab + 6 * nwall;
+  // cells
+  size_t fum = (size_t)F * U * M * nc, fu = (size_t)F * U * nc, fn = (size_t)F * nc;
+  size_t nd = 3 * fum + fu + 5 * fn + 3 * fn + 3 * nc + 3 * nc + nc + 9 * (size_t)nc;
+  PGDG_HIP_TRY(hipMalloc((void**)&c_slab, sizeof(double) * nd));
+  PGDG_HIP_TRY(hipMemset(c_slab, 0, sizeof(double) * nd));
+  double* b = c_slab;
+  C.coef = b; b += fum;
+  C.coef0 = b; b += fum;
+  C.dcoef = b; b += fum;
+  C.lim = b; b += fu;
+  C.alph = b; b += fn;
+  C.rho = b; b += fn;
+  C.E = b; b += fn;
+  C.c = b; b += fn;
+  C.ac = b; b += fn;
+  C.rhov = b; b += 3 * fn;
+  C.v = b; b += 3 * (size_t)nc;
+  C.a = b; b += 3 * (size_t)nc;
+  C.P = b; b += nc;
+  C.G = b; b += 9 * (size_t)nc;
+  PGDG_HIP_TRY(hipMalloc((void**)&ci_slab, sizeof(int32_t) * 2 * fn));
+  C.ph_state = ci_slab;
+  C.ph_state_b = ci_slab + fn;
+  PGDG_HIP_TRY(hipMalloc((void**)&used, sizeof(int32_t) * nc));
+  // stage buffers: xb[3], hb, x0[3], vol0
+  PGDG_HIP_TRY(hipMalloc((void**)&s_slab, sizeof(double) * 8 * cap));
+  PGDG_HIP_TRY(hipMemset(s_slab, 0, sizeof(double) * 8 * cap));
+  S.xb = s_slab;
+  S.hb = s_slab + 3 * cap;
+  S.x0 = s_slab + 4 * cap;
+  S.vol0 = s_slab + 7 * cap;
+  PGDG_HIP_TRY(hipMalloc((void**)&S.in_ele, sizeof(int32_t) * cap));
+  PGDG_HIP_TRY(hipMalloc((void**)&sc, sizeof(DevScalars)));
+  PGDG_HIP_TRY(hipMemset(sc, 0, sizeof(DevScalars)));
+  PGDG_TRY(T.init(cap, nwall, nc, st));
+  PGDG_HIP_TRY(hipStreamSynchronize(st));
+  bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall) + T.bytes_total;
+  return PGDG_OK;
+}
+
+void Ctx::release() {
+  if (st) (void)hipStreamSynchronize(st);
+  PA.release();
+  T.release();
+  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (own_stream && st) (void)hipStreamDestroy(st);
+}
+
+int Ctx::upload(const pgdg_particles* p, const pgdg_walls* w, const pgdg_cells* c,
+                const pgdg_scalars* s) {
+  if (!p || !w || !c || !s) return PGDG_ERR_ARG;
+  if (p->n > PA.cap || p->n < 0) return PGDG_ERR_CAPACITY;
+  if (w->n != W.n) return PGDG_ERR_ARG;
+  np = p->n;
+  PtlDev P = PA.view();
+  const int64_t hc = p->cap;
+  auto up = [&](double* dst, const double* src, int rows) -> int {
+    for (int r = 0; r < rows; r++)
+      PGDG_HIP_TRY(hipMemcpyAsync(dst + (size_t)r * PA.cap, src + (size_t)r * hc,
+                                  sizeof(double) * np, hipMemcpyHostToDevice, st));
+    return PGDG_OK;
+  };
+  if (np > 0) {
+    PGDG_TRY(up(P.x, p->x, 3));
+    PGDG_TRY(up(P.v, p->v, 3));
+    PGDG_TRY(up(P.tv, p->tv, 3));
+    PGDG_TRY(up(P.rhov, p->rhov, 3));
+    PGDG_TRY(up(P.vol, p->vol, 1));
+    PGDG_TRY(up(P.mass, p->mass, 1));
+    PGDG_TRY(up(P.h, p->h, 1));
+    PGDG_TRY(up(P.P, p->P, 1));
+    PGDG_TRY(up(P.rho, p->rho, 1));
+    PGDG_TRY(up(P.E, p->E, 1));
+    PGDG_TRY(up(P.c, p->c, 1));
+    PGDG_TRY(up(P.coef, p->coef, U * M));
+    PGDG_HIP_TRY(hipMemcpyAsync(P.phase, p->phase, sizeof(int32_t) * np, hipMemcpyHostToDevice, st));
+    PGDG_HIP_TRY(hipMemcpyAsync(P.mark_in, p->mark_in, sizeof(int32_t) * np, hipMemcpyHostToDevice, st));
+  }
+  if (W.n > 0) {
+    PGDG_HIP_TRY(hipMemcpyAsync(W.x, w->x, sizeof(double) * 3 * W.n, hipMemcpyHostToDevice, st));
+    PGDG_HIP_TRY(hipMemcpyAsync(W.norm, w->norm, sizeof(double) * 3 * W.n, hipMemcpyHostToDevice, st));
+    PGDG_HIP_TRY(hipMemcpyAsync(W.vol, w->vol, sizeof(double) * W.n, hipMemcpyHostToDevice, st));
+  }
+  size_t fum = (size_t)F * U * M * nc, fu = (size_t)F * U * nc, fn = (size_t)F * nc;
+  auto upc = [&](double* dst, const double* src, size_t cnt) -> int {
+    PGDG_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, st));
+    return PGDG_OK;
+  };
+  PGDG_TRY(upc(C.coef, c->coef, fum));
+  PGDG_TRY(upc(C.coef0, c->coef0, fum));
+  PGDG_TRY(upc(C.dcoef, c->dcoef, fum));
+  PGDG_TRY(upc(C.lim, c->lim, fu));
+  PGDG_TRY(upc(C.alph, c->alph, fn));
+  PGDG_TRY(upc(C.rho, c->rho, fn));
+  PGDG_TRY(upc(C.E, c->E, fn));
+  PGDG_TRY(upc(C.c, c->c, fn));
+  PGDG_TRY(upc(C.ac, c->ac, fn));
+  PGDG_TRY(upc(C.rhov, c->rhov, 3 * fn));
+  PGDG_TRY(upc(C.v, c->v, 3 * (size_t)nc));
+  PGDG_TRY(upc(C.a, c->a, 3 * (size_t)nc));
+  PGDG_TRY(upc(C.P, c->P, nc));
+  PGDG_TRY(upc(C.G, c->G_v, 9 * (size_t)nc));
+  PGDG_HIP_TRY(hipMemcpyAsync(C.ph_state, c->ph_state, sizeof(int32_t) * fn, hipMemcpyHostToDevice, st));
+  PGDG_HIP_TRY(hipMemcpyAsync(C.ph_state_b, c->ph_state_b, sizeof(int32_t) * fn, hipMemcpyHostToDevice, st));
+  DevScalars h;
+  memset(&h, 0, sizeof(h));
+  h.dt = s->dt;
+  h.max_v = s->max_v;
+  h.integral_time = s->integral_time;
+  h.run_time = s->run_time;
+  h.iterate_num = s->iterate_num;
+  PGDG_HIP_TRY(hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  iterate_num = s->iterate_num;
+  // no neighbour lists exist at upload time (reference: empty lists)
+  hipLaunchKernelGGL(k_used_clear, dim3((nc + 255) / 256), dim3(256), 0, st, used, nc);
+  PGDG_HIP_TRY(hipMemsetAsync(T.lcnt, 0, sizeof(int) * PA.cap, st));
+  T.pool_used = 0;
+  PGDG_HIP_TRY(hipStreamSynchronize(st));
+  loaded = true;
+  return PGDG_OK;
+}
+
+int Ctx::download(pgdg_particles* p, pgdg_cells* c, pgdg_scalars* s) {
+  if (!loaded) return PGDG_ERR_STATE;
+  PGDG_HIP_TRY(hipStreamSynchronize(st));
+  if (p) {
+    if (np > p->cap) return PGDG_ERR_CAPACITY;
+    p->n = np;
+    PtlDev P = PA.view();
+    const int64_t hc = p->cap;
+    auto dn = [&](double* dst, const double* src, int rows) -> int {
+      for (int r = 0; r < rows; r++)
+        PGDG_HIP_TRY(hipMemcpy(dst + (size_t)r * hc, src + (size_t)r * PA.cap, sizeof(double) * np,
+                               hipMemcpyDeviceToHost));
+      return PGDG_OK;
+    };
+    if (np > 0) {
+      PGDG_TRY(dn(p->x, P.x, 3));
+      PGDG_TRY(dn(p->v, P.v, 3));
+      PGDG_TRY(dn(p->tv, P.tv, 3));
+      PGDG_TRY(dn(p->rhov, P.rhov, 3));
+      PGDG_TRY(dn(p->vol, P.vol, 1));
+      PGDG_TRY(dn(p->mass, P.mass, 1));
+      PGDG_TRY(dn(p->h, P.h, 1));
+      PGDG_TRY(dn(p->P, P.P, 1));
+      PGDG_TRY(dn(p->rho, P.rho, 1));
+      PGDG_TRY(dn(p->E, P.E, 1));
+      PGDG_TRY(dn(p->c, P.c, 1));
+      PGDG_TRY(dn(p->coef, P.coef, U * M));
+      PGDG_HIP_TRY(hipMemcpy(p->phase, P.phase, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+      PGDG_HIP_TRY(hipMemcpy(p->mark_in, P.mark_in, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    }
+  }
+  if (c) {
+    size_t fum = (size_t)F * U * M * nc, fu = (size_t)F * U * nc, fn = (size_t)F * nc;
+    auto dc = [&](double* dst, const double* src, size_t cnt) -> int {
+      PGDG_HIP_TRY(hipMemcpy(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+      return PGDG_OK;
+    };
+    PGDG_TRY(dc(c->coef, C.coef, fum));
+    PGDG_TRY(dc(c->coef0, C.coef0, fum));
+    PGDG_TRY(dc(c->dcoef, C.dcoef, fum));
+    PGDG_TRY(dc(c->lim, C.lim, fu));
+    PGDG_TRY(dc(c->alph, C.alph, fn));
+    PGDG_TRY(dc(c->rho, C.rho, fn));
+    PGDG_TRY(dc(c->E, C.E, fn));
+    PGDG_TRY(dc(c->c, C.c, fn));
+    PGDG_TRY(dc(c->ac, C.ac, fn));
+    PGDG_TRY(dc(c->rhov, C.rhov, 3 * fn));
+    PGDG_TRY(dc(c->v, C.v, 3 * (size_t)nc));
+    PGDG_TRY(dc(c->a, C.a, 3 * (size_t)nc));
+    PGDG_TRY(dc(c->P, C.P, nc));
+    PGDG_TRY(dc(c->G_v, C.G, 9 * (size_t)nc));
+    PGDG_HIP_TRY(hipMemcpy(c->ph_state, C.ph_state, sizeof(int32_t) * fn, hipMemcpyDeviceToHost));
+    PGDG_HIP_TRY(hipMemcpy(c->ph_state_b, C.ph_state_b, sizeof(int32_t) * fn, hipMemcpyDeviceToHost));
+  }
+  if (s) {
+    DevScalars h;
+    PGDG_HIP_TRY(hipMemcpy(&h, sc, sizeof(h), hipMemcpyDeviceToHost));
+    s->dt = h.dt;
+    s->max_v = h.max_v;
+    s->integral_time = h.integral_time;
+    s->run_time = h.run_time;
+    s->iterate_num = h.iterate_num;
+    s->n_split = n_split;
+    s->n_inlet = n_inlet;
+    s->n_outlet = n_outlet;
+    s->n_merge = n_merge;
+  }
+  return PGDG_OK;
+}
+
+template <int D_>
+int Ctx::set_timestep() {
+  return launch_timestep<D_>(g, PA.view(), C, used, np, iterate_num > 1, sc, cs.output_dt, st);
+}
+
+// Stage functions numbered as the oracle's (oracle/step_ref.cpp `call`).
+enum {
+  FN_SET_TIMESTEP = 0, FN_USED = 1, FN_BIN = 2, FN_SPLIT = 3, FN_MERGE = 4, FN_MEMBERSHIP = 5,
+  FN_NEIGHBORS = 6, FN_P2G = 7, FN_LIMITER = 8, FN_RHS = 9, FN_UPDATE_GRID = 10,
+  FN_UPDATE_PARTICLE = 11, FN_RUN = 12
+};
+
+template <int D_>
+int Ctx::stage(int fn, int arg) {
+  PtlDev P = PA.view();
+  switch (fn) {
+    case FN_SET_TIMESTEP:
+      return set_timestep<D_>();
+    case FN_USED:
+      hipLaunchKernelGGL(k_used_clear, dim3((nc + 255) / 256), dim3(256), 0, st, used, nc);
+      hipLaunchKernelGGL(k_used_mark, dim3((nc + 255) / 256), dim3(256), 0, st, g, C.ph_state, used, D_);
+      PGDG_HIP_TRY(hipGetLastError());
+      return PGDG_OK;
+    case FN_BIN:
+      return T.build_bins<D_>(g, P, W, np);
+    case FN_SPLIT: {
+      int64_t cnt3[3];
+      PGDG_TRY(T.split<D_>(g, PA, W, &np, cnt3));
+      n_split = cnt3[0];
+      n_inlet = cnt3[1];
+      n_outlet = cnt3[2];
+      return PGDG_OK;
+    }
+    case FN_MERGE: {
+      int64_t m = 0;
+      PGDG_TRY(T.merge<D_>(g, PA, W, &np, &m));
+      n_merge += m;
+      return PGDG_OK;
+    }
+    case FN_MEMBERSHIP:
+      return launch_in_ele<D_>(g, P, S, np, st);
+    case FN_NEIGHBORS:
+      return launch_stage_save<D_>(g, P, S, np, arg, st);
+    case FN_P2G:
+      return launch_cells<D_>(CELL_P2G, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
+    case FN_LIMITER:
+      return launch_cells<D_>(CELL_LIMITER, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
+    case FN_RHS:
+      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
+    case FN_UPDATE_GRID:
+      return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, arg, st);
+    case FN_UPDATE_PARTICLE:
+      PGDG_TRY(launch_g2p<D_>(g, P, C, S, np, sc, arg, st));
+      return launch_pst<D_>(g, P, W, S, np, T.bin_off, T.bin_ent, sc, st);
+    case FN_RUN:
+      return run_simulation<D_>(arg);
+    default:
+      return PGDG_ERR_ARG;
+  }
+}
+
+template <int D_>
+int Ctx::run_simulation(int64_t n) {
+  n_merge = 0;
+  if (n == 1) PGDG_TRY(stage<D_>(FN_USED, 0));
+  PGDG_TRY(stage<D_>(FN_BIN, 0));
+  PGDG_TRY(stage<D_>(FN_SPLIT, 0));
+  for (int it = 0; it < 4; it++) PGDG_TRY(stage<D_>(FN_MERGE, 0));
+  PGDG_TRY(stage<D_>(FN_MEMBERSHIP, 0));
+  for (int md = 0; md < 2; md++) {
+    PGDG_TRY(stage<D_>(FN_NEIGHBORS, md));
+    if (md == 0) PGDG_TRY(stage<D_>(FN_P2G, 0));
+    PGDG_TRY(stage<D_>(FN_LIMITER, 0));
+    PGDG_TRY(stage<D_>(FN_RHS, 0));
+    PGDG_TRY(stage<D_>(FN_UPDATE_GRID, md));
+    PGDG_TRY(stage<D_>(FN_UPDATE_PARTICLE, md));
+  }
+  // lists the next step's split/merge passes read (built from the last
+  // stage's neighbour-list inputs)
+  return T.end_of_step_lists<D_>(g, PA.view(), S.xb, S.hb, W, np);
+}
+
+int Ctx::step(int nsteps) {
+  for (int s = 0; s < nsteps; s++) {
+    if (D == 2) {
+      PGDG_TRY(set_timestep<2>());
+      iterate_num++;
+      PGDG_TRY(run_simulation<2>(iterate_num));
+    } else {
+      PGDG_TRY(set_timestep<3>());
+      iterate_num++;
+      PGDG_TRY(run_simulation<3>(iterate_num));
+    }
+  }
+  return PGDG_OK;
+}
+
+}  // namespace pgdg
+
+struct pgdg_ctx {
+  pgdg::Ctx c;
+};
+
+using pgdg::Ctx;
+
+extern "C" {
+
+int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_ctx** out) {
+  if (!c || !out || particle_cap <= 0 || n_wall < 0) return PGDG_ERR_ARG;
+  if (c->dim != 2 && c->dim != 3) return PGDG_ERR_ARG;
+  if (c->nphase < 2 || c->nphase > PGDG_MAX_PHASE) return PGDG_ERR_ARG;
+  if (c->order != 1 || c->rk_stages != 2) return PGDG_ERR_ARG;
+  if (c->semantics != PGDG_SEM_T2 && c->semantics != PGDG_SEM_S3) return PGDG_ERR_ARG;
+  if (particle_cap > (int64_t)0x7ffffff0) return PGDG_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PGDG_ERR_NODEVICE;
+  pgdg_ctx* x = new (std::nothrow) pgdg_ctx();
+  if (!x) return PGDG_ERR_CAPACITY;
+  int r = x->c.init(c, particle_cap, n_wall);
+  if (r != PGDG_OK) {
+    x->c.release();
+    delete x;
+    return r;
+  }
+  *out = x;
+  return PGDG_OK;
+}
+
+int pgdg_destroy(pgdg_ctx* x) {
+  if (!x) return PGDG_ERR_ARG;
+  x->c.release();
+  delete x;
+  return PGDG_OK;
+}
+
+int pgdg_upload_state(pgdg_ctx* x, const pgdg_particles* p, const pgdg_walls* w,
+                      const pgdg_cells* c, const pgdg_scalars* s) {
+  if (!x) return PGDG_ERR_ARG;
+  return x->c.upload(p, w, c, s);
+}
+
+int pgdg_download_state(pgdg_ctx* x, pgdg_particles* p, pgdg_cells* c, pgdg_scalars* s) {
+  if (!x) return PGDG_ERR_ARG;
+  return x->c.download(p, c, s);
+}
+
+int pgdg_step(pgdg_ctx* x, int nsteps) {
+  if (!x || nsteps < 0) return PGDG_ERR_ARG;
+  if (!x->c.loaded) return PGDG_ERR_STATE;
+  return x->c.step(nsteps);
+}
+
+int pgdg_stage(pgdg_ctx* x, int fn, int arg) {
+  if (!x) return PGDG_ERR_ARG;
+  if (!x->c.loaded) return PGDG_ERR_STATE;
+  return x->c.D == 2 ? x->c.stage<2>(fn, arg) : x->c.stage<3>(fn, arg);
+}
+
+int pgdg_sync(pgdg_ctx* x) {
+  if (!x) return PGDG_ERR_ARG;
+  PGDG_HIP_TRY(hipStreamSynchronize(x->c.st));
+  return PGDG_OK;
+}
+
+int pgdg_get_stream(pgdg_ctx* x, void** stream) {
+  if (!x || !stream) return PGDG_ERR_ARG;
+  *stream = (void*)x->c.st;
+  return PGDG_OK;
+}
+
+int pgdg_particle_count(pgdg_ctx* x, int64_t* n) {
+  if (!x || !n) return PGDG_ERR_ARG;
+  *n = x->c.np;
+  return PGDG_OK;
+}
+
+int pgdg_memory_bytes(pgdg_ctx* x, int64_t* bytes) {
+  if (!x || !bytes) return PGDG_ERR_ARG;
+  *bytes = (int64_t)x->c.bytes;
+  return PGDG_OK;
+}
+
+// Reference-named stage entry points (each replaces the MPM member function
+// of the same name; see include/pgdg.h).
+int pgdg_set_timestep(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_SET_TIMESTEP, 0); }
+int pgdg_used_state_update(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_USED, 0); }
+int pgdg_map_particles_to_cells(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_BIN, 0); }
+int pgdg_split_particle_fast(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_SPLIT, 0); }
+int pgdg_merge_particle_fast(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_MERGE, 0); }
+int pgdg_refresh_cell_membership(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_MEMBERSHIP, 0); }
+int pgdg_refresh_neighbor_info(pgdg_ctx* x, int stage) { return pgdg_stage(x, pgdg::FN_NEIGHBORS, stage); }
+int pgdg_mls_p2g(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_P2G, 0); }
+int pgdg_calculate_limiter(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_LIMITER, 0); }
+int pgdg_grid_acceleration(pgdg_ctx* x) { return pgdg_stage(x, pgdg::FN_RHS, 0); }
+int pgdg_update_grid(pgdg_ctx* x, int md) { return pgdg_stage(x, pgdg::FN_UPDATE_GRID, md); }
+int pgdg_update_particle(pgdg_ctx* x, int md) { return pgdg_stage(x, pgdg::FN_UPDATE_PARTICLE, md); }
+int pgdg_run_simulation(pgdg_ctx* x, int64_t n) { return pgdg_stage(x, pgdg::FN_RUN, (int)n); }
+
+}  // extern "C"

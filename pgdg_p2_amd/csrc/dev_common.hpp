@@ -1,0 +1,171 @@
+// dev_common.hpp - device-side views, geometry constants and scalar math of
+// the PG-DG step kernels.
+//
+// Data layout in HBM (all FP64 unless noted, SoA so a wavefront touching 64
+// consecutive particles or cells reads 512 contiguous bytes per field):
+//   particles  x[3][cap], v[3][cap], tv[3][cap], rhov[3][cap],
+//              vol, mass, h, P, rho, E, c [cap], coef[U*M][cap],
+//              phase, mark_in (int32) [cap]
+//   cells      coef/coef0/dcoef[F][U][M][nc], lim[F][U][nc],
+//              alph/rho/E/c/ac[F][nc], rhov[F][3][nc], ph_state(_b)[F][nc],
+//              v/a[3][nc], P[nc], G[9][nc]; cell index (i*n1 + j)*n2 + k
+//   bins       CSR over cells: bin_off[nc+1], bin_ent[] (particle refs:
+//              >= 0 fluid particle, <= -2 wall -(w+2), -1 removed)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "pgdg.h"
+#include "riemann.hpp"
+
+namespace pgdg {
+
+constexpr double kCellRatio = 3.2;
+constexpr double kCutOff = 2.;
+constexpr double kPiRef = 3.1415926;  // S3:glbparam.h:36
+
+__host__ __device__ inline bool ref_is_wall(int r) { return r <= -2; }
+__host__ __device__ inline int ref_wall(int r) { return -r - 2; }
+
+// Geometry and constants of one case, passed by value to every kernel.
+struct Geo {
+  int N[3];
+  int nc;
+  int F;
+  int sem;
+  int bc[6];
+  int radius;  // status-2 neighbour search radius in cells
+  double ps[3];
+  double box_l[3], box_r[3];
+  double p_sz, p_g, p2g_cut;
+  double gamma[PGDG_MAX_PHASE], pinf[PGDG_MAX_PHASE], ini_rho[PGDG_MAX_PHASE];
+  double inlet_rho, inlet_u[3], inlet_p;
+  double Gp[2], Gw[2];
+  double probe_c[4], probe_s[4], sqrt2;
+  double vol_max, vol_min, dss0;
+};
+
+struct PtlDev {
+  int64_t cap;
+  double *x, *v, *tv, *rhov;                // [3][cap]
+  double *vol, *mass, *h, *P, *rho, *E, *c; // [cap]
+  double *coef;                             // [U*M][cap]
+  int32_t *phase, *mark_in;                 // [cap]
+};
+
+struct CellDev {
+  double *coef, *coef0, *dcoef, *lim, *alph, *rho, *E, *c, *ac, *rhov, *v, *a, *P, *G;
+  int32_t *ph_state, *ph_state_b;
+};
+
+struct WallDev {
+  int64_t n;
+  double *x, *norm, *vol;  // x,norm [3][n]
+};
+
+// Device-resident scalars of the time loop (kept on the device so stage
+// launches can be replayed without host round trips).
+struct DevScalars {
+  double dt;
+  double max_v;
+  double integral_time;
+  double run_time;
+  long long iterate_num;
+  unsigned long long red_dt;    // min-reduction slot (IEEE bits of a positive double)
+  unsigned long long red_maxv;  // max-reduction slot
+  long long pad;
+};
+
+// get_cell_id (S3:glbfunc.h:161-173) of coord - box_l
+template <int D>
+__device__ inline int cell_of(const Geo& g, const double* x) {
+  int p[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    int q = int(floor((x[a] - g.box_l[a]) / g.ps[a]));
+    q = q <= g.N[a] - 1 ? q : g.N[a] - 1;
+    p[a] = q >= 0 ? q : 0;
+  }
+  return (p[0] * g.N[1] + p[1]) * g.N[2] + p[2];
+}
+
+__device__ inline void cell_ijk(const Geo& g, int c, int& i, int& j, int& k) {
+  k = c % g.N[2];
+  int t = c / g.N[2];
+  j = t % g.N[1];
+  i = t / g.N[1];
+}
+
+// Correctly rounded x^(1./3.) (pow(x, 1./3.) of the reference rounded to
+// nearest): Newton for the cube root, an fma-exact residual correction and
+// the first-order factor (1 - eps ln x) for 1./3. = 1/3 - eps.
+__host__ __device__ inline double root13(double x) {
+  if (!(x > 0) || isinf(x)) {
+    if (x == 0) return 0.0;
+    if (isinf(x) && x > 0) return x;
+    return NAN;
+  }
+  int e;
+  double m = frexp(x, &e);
+  int q = e / 3;
+  int r = e - 3 * q;
+  if (r < 0) {
+    r += 3;
+    q -= 1;
+  }
+  double y0 = ldexp(m, r);
+  double t = 0.6 + 0.4 * y0 - 0.02 * y0 * y0;
+  for (int it = 0; it < 6; it++) t = t - (t * t * t - y0) / (3.0 * t * t);
+  double p = t * t;
+  double pe = fma(t, t, -p);
+  double c3 = p * t;
+  double ce = fma(p, t, -c3);
+  double res = ((y0 - c3) - ce) - pe * t;
+  double delta = res / (3.0 * p);
+  double s = (m - 1.0) / (m + 1.0);
+  double s2 = s * s;
+  double lnm = 2.0 * s * (1.0 + s2 * (1.0 / 3.0 + s2 * (0.2 + s2 * (1.0 / 7.0 + s2 * (1.0 / 9.0)))));
+  double lnx = (double)e * 0.6931471805599453 + lnm;
+  const double eps = 1.850371707708594e-17;
+  double corr = delta - eps * lnx * t;
+  return ldexp(t + corr, q);
+}
+
+template <int D>
+__host__ __device__ inline double root_d(double x) {
+  if (D == 2) return sqrt(x);
+  return root13(x);
+}
+
+// Wendland kernel W(r, h) and dW/dr / r (S3:particle.cpp:131-149, 297-315)
+template <int D>
+__device__ inline double kern_C(double h) {
+  if (D == 2) return 1.0 / kPiRef / (h * h) * 7. / 4.;
+  return 1.0 / kPiRef / ((h * h) * h) * 21. / 16.;
+}
+template <int D>
+__device__ inline double kern_W(double dist, double h) {
+  double C = kern_C<D>(h);
+  double s = dist / h;
+  if (s < 2) {
+    double q = 1. - 0.5 * s;
+    return C * (((q * q) * q) * q) * (1. + 2. * s);
+  }
+  return 0.0;
+}
+template <int D>
+__device__ inline double kern_dW(double dist, double h) {
+  double C = kern_C<D>(h);
+  double s = dist / h;
+  if (s < 2) {
+    double q = 1. - 0.5 * s;
+    return -5. * C * ((q * q) * q) * s / h / dist;
+  }
+  return 0.0;
+}
+
+__device__ inline double amax1(double a, double b) { return a >= b ? a : b; }
+__device__ inline double amin1(double a, double b) { return a <= b ? a : b; }
+
+}  // namespace pgdg

@@ -1,0 +1,941 @@
+// step_cells.hip - per-cell kernels of the PG-DG step: MLS P2G, limiter,
+// DG right-hand side (volume + face quadrature with HLLC, MGFM ghost states
+// and boundary fluxes) and the RK stage update.
+//
+// One lane per cell, FP64.  Each kernel keeps the operation order of the
+// reference routine it replaces so that a -ffp-contract=off build matches
+// the FP64 oracle bit for bit:
+//   k_p2g          MPM::MLS_P2G            S3:mpm.cpp:274-382
+//   k_limiter      MPM::Calculate_limiter  S3:mpm.cpp:392-738, T2:mpm.cpp:887-1225
+//   k_rhs          Particle_base::Grid_acceleration S3:particle_flow.cpp:16-405
+//   k_update_grid  MPM::Update_grid        S3:mpm.cpp:768-830, T2:mpm.cpp:1254-1320
+// Face fluxes are accumulated only along the face normal: the reference
+// forms the full U x D flux and contracts it with an axis-aligned normal, so
+// the tangential columns contribute exact zeros.
+#include "dev_common.hpp"
+#include "hip_util.hpp"
+#include "step_kernels.hpp"
+
+namespace pgdg {
+
+#define CIDX(f, u, m) ((((size_t)(f) * U + (u)) * M + (m)) * nc + c)
+
+template <int D>
+__device__ inline void cell_center(const Geo& g, int i, int j, int k, double* x) {
+  x[0] = g.ps[0] * (i + 0.5);
+  x[1] = g.ps[1] * (j + 0.5);
+  x[2] = D == 3 ? g.ps[2] * (k + 0.5) : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// MLS P2G (S3:mpm.cpp:274-382)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, const int* __restrict__ bin_off,
+                                             const int* __restrict__ bin_ent,
+                                             const int32_t* __restrict__ used) {
+  constexpr int U = D + 2, M = D + 1, B = M - 1;
+  const int nc = g.nc;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc || !used[c]) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  double xc[3];
+  cell_center<D>(g, i, j, k, xc);
+  const double hi = g.p_g * 1 * 3.2 / kCutOff;
+  const int64_t cap = P.cap;
+  double mass_t[PGDG_MAX_PHASE] = {0, 0, 0};
+  double pav[PGDG_MAX_PHASE][U];
+  double cf[PGDG_MAX_PHASE][U][B];
+#pragma unroll
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      pav[s][u] = 0;
+#pragma unroll
+      for (int b = 0; b < B; b++) cf[s][u][b] = 0;
+    }
+  }
+  const int kc = D == 3 ? 1 : 0;
+  for (int t = max(i - 1, 0); t <= min(i + 1, g.N[0] - 1); t++)
+    for (int s_ = max(j - 1, 0); s_ <= min(j + 1, g.N[1] - 1); s_++)
+      for (int q = max(k - kc, 0); q <= min(k + kc, g.N[2] - 1); q++) {
+        int cc = (t * g.N[1] + s_) * g.N[2] + q;
+        int e0 = bin_off[cc], e1 = bin_off[cc + 1];
+        for (int e = e0; e < e1; e++) {
+          int p = bin_ent[e];
+          if (p < 0) continue;  // wall particles are not cell members
+          double xij[3];
+#pragma unroll
+          for (int a = 0; a < D; a++) xij[a] = P.x[a * cap + p] - xc[a];
+          double sq = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) sq += xij[a] * xij[a];
+          double dist = sqrt(sq);
+          if (dist < g.p2g_cut) {
+            double ww = kern_W<D>(dist, hi) * P.vol[p];
+            double pj[U];
+            pj[0] = P.rho[p];
+#pragma unroll
+            for (int a = 0; a < D; a++) pj[1 + a] = P.rhov[a * cap + p];
+            pj[U - 1] = P.E[p];
+            double sl[U][B];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+              double acc = 0;
+#pragma unroll
+              for (int b = 0; b < B; b++) {
+                sl[u][b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
+                acc += sl[u][b] * (-xij[b]);
+              }
+              pj[u] += acc;
+            }
+            int ph = P.phase[p] - 1;
+#pragma unroll
+            for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+              if (s < g.F && ph == s) {
+#pragma unroll
+                for (int u = 0; u < U; u++) pav[s][u] += pj[u] * ww;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                  for (int b = 0; b < B; b++) cf[s][u][b] += sl[u][b] * ww;
+                mass_t[s] += ww;
+              }
+            }
+          }
+        }
+      }
+  double mass_total = 0;
+#pragma unroll
+  for (int s = 0; s < PGDG_MAX_PHASE; s++)
+    if (s < g.F) mass_total += mass_t[s];
+  double cv[3] = {C.v[c], C.v[nc + c], C.v[2 * nc + c]};
+  double cP = C.P[c];
+#pragma unroll
+  for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
+    if (ss >= g.F) break;
+    int st = mass_t[ss] > 0 ? 1 : 0;
+    C.ph_state[ss * nc + c] = st;
+    C.alph[ss * nc + c] = mass_t[ss] / (mass_total + 1e-20);
+    int stb = C.ph_state_b[ss * nc + c];
+    if (st == 1 && stb == 0) {
+      double pa[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) pa[u] = pav[ss][u] / (mass_t[ss] + 1e-20);
+      double rho = pa[0];
+      C.coef[CIDX(ss, 0, 0)] = rho;
+      C.rho[ss * nc + c] = rho;
+      double rv[3] = {0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        rv[a] = pa[1 + a];
+        C.coef[CIDX(ss, 1 + a, 0)] = rv[a];
+        C.rhov[(ss * 3 + a) * nc + c] = rv[a];
+      }
+      if (D == 2) rv[2] = C.rhov[(ss * 3 + 2) * nc + c];
+      double E = pa[U - 1];
+      C.coef[CIDX(ss, U - 1, 0)] = E;
+      C.E[ss * nc + c] = E;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int b = 0; b < B; b++) C.coef[CIDX(ss, u, 1 + b)] = cf[ss][u][b] / (mass_t[ss] + 1e-20);
+#pragma unroll
+      for (int a = 0; a < 3; a++) cv[a] = rv[a] / rho;
+      double sq = 0;
+#pragma unroll
+      for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
+      double e_j = E - 0.5 * sq / rho;
+      cP = eos_p(e_j, Eos{g.gamma[ss], g.pinf[ss]});
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int m = 0; m < M; m++) C.coef0[CIDX(ss, u, m)] = C.coef[CIDX(ss, u, m)];
+    }
+    C.ph_state_b[ss * nc + c] = st;
+    if (st == 0) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+          C.coef[CIDX(ss, u, m)] = 0;
+          C.coef0[CIDX(ss, u, m)] = 0;
+        }
+      C.rho[ss * nc + c] = g.ini_rho[ss];
+#pragma unroll
+      for (int a = 0; a < 3; a++) C.rhov[(ss * 3 + a) * nc + c] = 0;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) C.v[a * nc + c] = cv[a];
+  C.P[c] = cP;
+}
+
+// ---------------------------------------------------------------------------
+// Calculate_limiter (S3:mpm.cpp:392-738; T2:mpm.cpp:887-1225)
+// ---------------------------------------------------------------------------
+__device__ inline double limiter_fn(double ds, int bd_fg) {  // S3:glbfunc.h:283-299
+  double val = 1;
+  if (ds > 0 && ds < 0.5) val = 2 * ds;
+  else if (ds >= 0.5) val = 1;
+  if (bd_fg == 1) val = (fabs(ds) + ds) / (fabs(ds) + 1);
+  return val;
+}
+
+// Face geometry of side f of cell (i,j,k): centre, unit normal (axis and
+// sign), tangents, area, neighbour cell (-1 outside) and boundary tag.
+template <int D>
+struct FaceGeo {
+  double cen[3];
+  int axis;
+  double sgn;
+  double tan0[3], tan1[3];
+  double S;
+  int nbr;
+  int b;
+};
+
+template <int D>
+__device__ inline void face_geo(const Geo& g, int i, int j, int k, const double* xc, int f,
+                                FaceGeo<D>& fg) {
+  const double px = g.ps[0], py = g.ps[1], pz = g.ps[2];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    fg.cen[a] = xc[a];
+    fg.tan0[a] = 0;
+    fg.tan1[a] = 0;
+  }
+  int ni = i, nj = j, nk = k;
+  switch (f) {
+    case 0: fg.axis = 1; fg.sgn = -1; fg.tan0[0] = 0.5 * px; fg.cen[1] = xc[1] - 0.5 * py; fg.S = D == 3 ? px * pz : px; nj = j - 1; break;
+    case 1: fg.axis = 0; fg.sgn = 1; fg.tan0[1] = 0.5 * py; fg.cen[0] = xc[0] + 0.5 * px; fg.S = D == 3 ? py * pz : py; ni = i + 1; break;
+    case 2: fg.axis = 1; fg.sgn = 1; fg.tan0[0] = -0.5 * px; fg.cen[1] = xc[1] + 0.5 * py; fg.S = D == 3 ? px * pz : px; nj = j + 1; break;
+    case 3: fg.axis = 0; fg.sgn = -1; fg.tan0[1] = -0.5 * py; fg.cen[0] = xc[0] - 0.5 * px; fg.S = D == 3 ? py * pz : py; ni = i - 1; break;
+    case 4: fg.axis = 2; fg.sgn = -1; fg.tan0[0] = 0.5 * px; fg.tan1[1] = 0.5 * py; fg.cen[2] = xc[2] - 0.5 * pz; fg.S = py * px; nk = k - 1; break;
+    default: fg.axis = 2; fg.sgn = 1; fg.tan0[0] = 0.5 * px; fg.tan1[1] = 0.5 * py; fg.cen[2] = xc[2] + 0.5 * pz; fg.S = py * px; nk = k + 1; break;
+  }
+  if (D == 3 && f < 4) fg.tan1[2] = 0.5 * pz;
+  bool in = ni >= 0 && ni < g.N[0] && nj >= 0 && nj < g.N[1] && nk >= 0 && nk < g.N[2];
+  fg.nbr = in ? (ni * g.N[1] + nj) * g.N[2] + nk : -1;
+  fg.b = in ? 0 : g.bc[f];
+}
+
+template <int D>
+__global__ void __launch_bounds__(128) k_limiter(Geo g, CellDev C, const int32_t* __restrict__ used) {
+  constexpr int U = D + 2, M = D + 1, NF = 2 * D;
+  const int nc = g.nc;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc || !used[c]) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  double xc[3];
+  cell_center<D>(g, i, j, k, xc);
+  const int F = g.F;
+  double umax[PGDG_MAX_PHASE][U], umin[PGDG_MAX_PHASE][U], pi[PGDG_MAX_PHASE][U];
+  double crho[PGDG_MAX_PHASE], calph[PGDG_MAX_PHASE], crv[PGDG_MAX_PHASE][3];
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    if (s >= F) break;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      umax[s][u] = -1e20;
+      umin[s][u] = 1e20;
+    }
+    crho[s] = C.rho[s * nc + c];
+    calph[s] = C.alph[s * nc + c];
+#pragma unroll
+    for (int a = 0; a < 3; a++) crv[s][a] = C.rhov[(s * 3 + a) * nc + c];
+    pi[s][0] = crho[s];
+#pragma unroll
+    for (int a = 0; a < D; a++) pi[s][1 + a] = crv[s][a];
+    pi[s][U - 1] = C.E[s * nc + c];
+  }
+  // velocity gradient G_v (S3:mpm.cpp:471-533)
+  double G[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) G[a][b] = 0.;
+  int bd_fg = 0;
+  double vc[3] = {C.v[c], C.v[nc + c], C.v[2 * nc + c]};
+  {
+    if (i == g.N[0] - 1 || i == 0) {
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a][0] = (vc[a] - vc[a]) / (0.5 * g.ps[0]);
+      bd_fg = 1;
+    } else {
+      int cr = c + g.N[1] * g.N[2], cl = c - g.N[1] * g.N[2];
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a][0] = (C.v[a * nc + cr] - C.v[a * nc + cl]) / (2 * g.ps[0]);
+    }
+    if (j == g.N[1] - 1) {
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a][1] = (0.0 - vc[a]) / (0.5 * g.ps[1]);
+      bd_fg = 1;
+    } else if (j == 0) {
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a][1] = (vc[a] - 0.0) / (0.5 * g.ps[1]);
+      bd_fg = 1;
+    } else {
+      int cr = c + g.N[2], cl = c - g.N[2];
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a][1] = (C.v[a * nc + cr] - C.v[a * nc + cl]) / (2 * g.ps[1]);
+    }
+    if (D == 3) {
+      if (k == g.N[2] - 1) {
+#pragma unroll
+        for (int a = 0; a < D; a++) G[a][2] = (0.0 - vc[a]) / (0.5 * g.ps[2]);
+        bd_fg = 1;
+      } else if (k == 0) {
+#pragma unroll
+        for (int a = 0; a < D; a++) G[a][2] = (vc[a] - 0.0) / (0.5 * g.ps[2]);
+        bd_fg = 1;
+      } else {
+#pragma unroll
+        for (int a = 0; a < D; a++) G[a][2] = (C.v[a * nc + c + 1] - C.v[a * nc + c - 1]) / (2 * g.ps[2]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) C.G[(a * 3 + b) * nc + c] = G[a][b];
+  const bool s3 = g.sem == PGDG_SEM_S3;
+  // neighbour extrema
+  for (int f = 0; f < NF; f++) {
+    FaceGeo<D> fg;
+    face_geo<D>(g, i, j, k, xc, f, fg);
+    if (fg.b == 0) {
+      int nb = fg.nbr;
+      for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+        if (s >= F) break;
+        double pj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pj[u] = 0;
+        for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
+          if (s1 >= F) break;
+          double al = C.alph[s1 * nc + nb];
+          if (al == 0) continue;
+          double rhoj = C.rho[s1 * nc + nb];
+          double rvj[3];
+#pragma unroll
+          for (int a = 0; a < D; a++) rvj[a] = C.rhov[(s1 * 3 + a) * nc + nb];
+          double Ej = C.E[s1 * nc + nb];
+          if (s1 != s) {
+            double vj[3];
+#pragma unroll
+            for (int a = 0; a < D; a++) vj[a] = rvj[a] / rhoj;
+            double sq = 0;
+#pragma unroll
+            for (int a = 0; a < D; a++) sq += vj[a] * vj[a];
+            double rhoe_j = Ej - 0.5 * rhoj * sq;
+            double P_j = eos_p(rhoe_j, Eos{g.gamma[s1], g.pinf[s1]});
+            rhoj = crho[s];
+#pragma unroll
+            for (int a = 0; a < D; a++) rvj[a] = rhoj * vj[a];
+            Ej = eos_rhoe(P_j, Eos{g.gamma[s], g.pinf[s]}) + 0.5 * rhoj * sq;
+          }
+          pj[0] += rhoj * al;
+#pragma unroll
+          for (int a = 0; a < D; a++) pj[1 + a] += rvj[a] * al;
+          pj[U - 1] += Ej * al;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          double d = pj[u] - pi[s][u];
+          umax[s][u] = amax1(umax[s][u], d);
+          umin[s][u] = amin1(umin[s][u], d);
+        }
+      }
+    } else {
+      double pj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) pj[u] = 0;
+      for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+        if (s >= F) break;
+        if (s3) {
+          pj[0] = crho[s];
+#pragma unroll
+          for (int a = 0; a < D; a++) pj[1 + a] = crv[s][a];
+          pj[U - 1] = pi[s][U - 1];
+          if (fg.b == -1 || fg.b == -2)
+#pragma unroll
+            for (int u = 0; u < U; u++) pj[u] = pi[s][u];
+          if (j == g.N[1] - 1 || j == 0) pj[2] = -crv[s][1];
+          if (D == 3 && (k == g.N[2] - 1 || k == 0)) pj[3] = -crv[s][2];
+        } else {
+          if (fg.b == 1) {
+            pj[0] = crho[s];
+            pj[1] = crv[s][0];
+            pj[2] = -crv[s][1];
+            pj[U - 1] = pi[s][U - 1];
+          } else if (fg.b == -1 || fg.b == -2) {
+#pragma unroll
+            for (int u = 0; u < U; u++) pj[u] = pi[s][u];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          double d = pj[u] - pi[s][u];
+          umax[s][u] = amax1(umax[s][u], d);
+          umin[s][u] = amin1(umin[s][u], d);
+        }
+      }
+    }
+  }
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    if (s >= F) break;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      umax[s][u] = amax1(umax[s][u], 0.);
+      umin[s][u] = amin1(umin[s][u], 0.);
+    }
+  }
+  // ratio of the allowed range to the slope increment at neighbour centres
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    if (s >= F) break;
+    double sl[U][M - 1];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int b = 0; b < M - 1; b++) sl[u][b] = C.coef[CIDX(s, u, 1 + b)];
+    double lim[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) lim[u] = 1.;
+    int lim_fg = bd_fg;
+    if (calph[s] != 1) lim_fg = 1;
+    for (int f = 0; f < NF; f++) {
+      FaceGeo<D> fg;
+      face_geo<D>(g, i, j, k, xc, f, fg);
+      double dr[3];
+      if (fg.b == 0) {
+        int ni, nj, nk;
+        cell_ijk(g, fg.nbr, ni, nj, nk);
+        double xn[3];
+        cell_center<D>(g, ni, nj, nk, xn);
+#pragma unroll
+        for (int a = 0; a < 3; a++) dr[a] = xn[a] - xc[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) dr[a] = fg.cen[a] - xc[a];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        double dU = 0;
+#pragma unroll
+        for (int b = 0; b < M - 1; b++) dU += sl[u][b] * dr[b];
+        double sd = 1;
+        if (dU >= 0) {
+          sd = umax[s][u] / (dU + 1e-20);
+          sd = limiter_fn(fabs(sd), lim_fg);
+        } else if (dU < 0) {
+          sd = umin[s][u] / (dU + 1e-20);
+          sd = limiter_fn(fabs(sd), lim_fg);
+        }
+        lim[u] = amin1(lim[u], sd);
+      }
+    }
+    // positivity probe
+    int fgp = 0;
+    double c0[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) c0[u] = C.coef[CIDX(s, u, 0)];
+    for (int jj = 0; jj < 4 && !fgp; jj++) {
+      for (int k1 = 0; k1 < (D == 3 ? 2 : 1) && !fgp; k1++) {
+        double dr[3] = {0, 0, 0};
+        dr[0] = g.sqrt2 * g.p_g * g.probe_c[jj];
+        dr[1] = g.sqrt2 * g.p_g * g.probe_s[jj];
+        if (D == 3) dr[2] = (-0.5 + k1) * g.p_g;
+        double Ui[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          double acc = 0;
+#pragma unroll
+          for (int b = 0; b < M - 1; b++) acc += (lim[u] * sl[u][b]) * dr[b];
+          Ui[u] = c0[u] + acc;
+        }
+        double sq = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++) sq += Ui[1 + a] * Ui[1 + a];
+        double e_p = Ui[U - 1] - 0.5 * sq / Ui[0];
+        double P_p = eos_p(e_p, Eos{g.gamma[s], g.pinf[s]});
+        if (P_p < -0.05 * g.pinf[s] || Ui[0] < 0) fgp = 1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) C.lim[(s * U + u) * nc + c] = fgp ? 0. : lim[u];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DG right-hand side (S3:particle_flow.cpp:16-405)
+// ---------------------------------------------------------------------------
+// Flux column b of DF = U v^T + P I (+ v_b P in the energy row).
+template <int D>
+__device__ inline void flux_col(double rho, const double* rv, double E, const double* v, double P,
+                                int b, double* out) {
+  constexpr int U = D + 2;
+  out[0] = rho * v[b];
+#pragma unroll
+  for (int a = 0; a < D; a++) out[1 + a] = rv[a] * v[b];
+  out[1 + b] += P;
+  out[U - 1] = E * v[b] + v[b] * P;
+}
+
+template <int D>
+__global__ void __launch_bounds__(128) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
+  constexpr int U = D + 2, M = D + 1, B = M - 1, NF = 2 * D;
+  const int nc = g.nc;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc || !used[c]) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  double xc[3];
+  cell_center<D>(g, i, j, k, xc);
+  const bool s3 = g.sem == PGDG_SEM_S3;
+  const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
+  const double voli = D == 3 ? g.ps[0] * g.ps[1] * g.ps[2] : g.ps[0] * g.ps[1];
+  for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
+    if (ss >= g.F) break;
+    if (C.ph_state[ss * nc + c] != 1) continue;
+    const Eos me{g.gamma[ss], g.pinf[ss]};
+    double cf[U][M], lm[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      lm[u] = C.lim[(ss * U + u) * nc + c];
+#pragma unroll
+      for (int m = 0; m < M; m++) cf[u][m] = C.coef[CIDX(ss, u, m)];
+    }
+    double dc[U][M];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int m = 0; m < M; m++) dc[u][m] = 0;
+    // volume integral, Gauss points in g1, g2[, g3] order
+    for (int g1 = 0; g1 < 2; g1++)
+      for (int g2 = 0; g2 < 2; g2++)
+        for (int g3 = 0; g3 < (D == 3 ? 2 : 1); g3++) {
+          double dp[3] = {0, 0, 0};
+          dp[0] = g.Gp[g1] * hp[0];
+          dp[1] = g.Gp[g2] * hp[1];
+          double gw = g.Gw[g1] * g.Gw[g2];
+          if (D == 3) {
+            dp[2] = g.Gp[g3] * hp[2];
+            gw = g.Gw[g1] * g.Gw[g2] * g.Gw[g3];
+          }
+          double Up[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            double acc = 0;
+#pragma unroll
+            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * dp[b];
+            Up[u] = cf[u][0] + lm[u] * acc;
+          }
+          double rv[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+#pragma unroll
+          for (int a = 0; a < D; a++) rv[a] = Up[1 + a];
+#pragma unroll
+          for (int a = 0; a < D; a++) v[a] = rv[a] / Up[0];
+          double sq = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
+          double e_p = Up[U - 1] - 0.5 * sq / Up[0];
+          double P_p = eos_p(e_p, me);
+#pragma unroll
+          for (int bi = 1; bi < M; bi++) {
+            double col[U];
+            flux_col<D>(Up[0], rv, Up[U - 1], v, P_p, bi - 1, col);
+#pragma unroll
+            for (int u = 0; u < U; u++) dc[u][bi] += col[u] * gw * voli * 1;
+          }
+        }
+    // face integral
+    for (int f = 0; f < NF; f++) {
+      FaceGeo<D> fg;
+      face_geo<D>(g, i, j, k, xc, f, fg);
+      const int ax = fg.axis;
+      const double sgn = fg.sgn;
+      double nrm[3] = {0, 0, 0};
+      nrm[ax] = sgn;
+      double DFn[M][U];
+#pragma unroll
+      for (int m = 0; m < M; m++)
+#pragma unroll
+        for (int u = 0; u < U; u++) DFn[m][u] = 0;
+      const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
+      for (int g1 = 0; g1 < 2; g1++)
+        for (int g2 = 0; g2 < (D == 3 ? 2 : 1); g2++) {
+          double dp[3], gw;
+          if (D == 3) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) dp[a] = g.Gp[g1] * fg.tan0[a] + g.Gp[g2] * fg.tan1[a];
+            gw = g.Gw[g1] * g.Gw[g2];
+          } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++) dp[a] = g.Gp[g1] * fg.tan0[a];
+            gw = g.Gw[g1];
+          }
+          double dri[3];
+#pragma unroll
+          for (int a = 0; a < 3; a++) dri[a] = dp[a] + fg.cen[a] - xc[a];
+          double Nb[M];
+          Nb[0] = 1.;
+#pragma unroll
+          for (int a = 0; a < D; a++) Nb[1 + a] = dri[a];
+          double Ui[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            double acc = 0;
+#pragma unroll
+            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * dri[b];
+            Ui[u] = cf[u][0] + lm[u] * acc;
+          }
+          const double rho_i = Ui[0], E_i = Ui[U - 1];
+          double rvi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+#pragma unroll
+          for (int a = 0; a < D; a++) rvi[a] = Ui[1 + a];
+#pragma unroll
+          for (int a = 0; a < D; a++) vi[a] = rvi[a] / rho_i;
+          double sq = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) sq += rvi[a] * rvi[a];
+          double e_i = E_i - 0.5 * sq / rho_i;
+          double P_i = eos_p(e_i, me);
+          double ci = sqrt(g.gamma[ss] * (g.pinf[ss] + P_i) / rho_i);
+          double vni = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
+          double DFi[U];
+          flux_col<D>(rho_i, rvi, E_i, vi, P_i, ax, DFi);
+          if (fg.b == 0) {
+            const int nb = fg.nbr;
+            int ni, nj, nk;
+            cell_ijk(g, nb, ni, nj, nk);
+            double xn[3];
+            cell_center<D>(g, ni, nj, nk, xn);
+            double drj[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) drj[a] = dp[a] + fg.cen[a] - xn[a];
+            for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
+              if (s1 >= g.F) break;
+              double alph_j = C.alph[s1 * nc + nb];
+              if (alph_j == 0) continue;
+              double Uj[U];
+#pragma unroll
+              for (int u = 0; u < U; u++) {
+                double acc = 0;
+#pragma unroll
+                for (int b = 0; b < B; b++) acc += C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * drj[b];
+                Uj[u] = C.coef[(((size_t)s1 * U + u) * M) * nc + nb] + C.lim[(s1 * U + u) * nc + nb] * acc;
+              }
+              double rho_j = Uj[0], E_j = Uj[U - 1];
+              double rvj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
+#pragma unroll
+              for (int a = 0; a < D; a++) rvj[a] = Uj[1 + a];
+#pragma unroll
+              for (int a = 0; a < D; a++) vj[a] = rvj[a] / rho_j;
+              double sqj = 0;
+#pragma unroll
+              for (int a = 0; a < D; a++) sqj += vj[a] * vj[a];
+              double rhoe_j = E_j - 0.5 * sqj * rho_j;
+              double P_j = eos_p(rhoe_j, Eos{g.gamma[s1], g.pinf[s1]});
+              double vnj = 0;
+#pragma unroll
+              for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
+              if (ss != s1) {
+                MgfmOut o = mgfm_solve(vni, vnj, P_i, P_j, rho_i, rho_j, me,
+                                       Eos{g.gamma[s1], g.pinf[s1]}, s3 ? MGFM_S3 : MGFM_T2,
+                                       TOL_REF, 0.0, 50, -1.0);
+                rho_j = o.rho_l;
+                P_j = o.pstar;
+                vnj = o.u_l;
+#pragma unroll
+                for (int a = 0; a < D; a++) vj[a] = vi[a] - vni * nrm[a] + vnj * nrm[a];
+#pragma unroll
+                for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+                rhoe_j = eos_rhoe(P_j, me);
+                double sq2 = 0;
+#pragma unroll
+                for (int a = 0; a < D; a++) sq2 += vj[a] * vj[a];
+                E_j = rhoe_j + 0.5 * rho_j * sq2;
+              }
+              double cj = sqrt(g.gamma[ss] * (g.pinf[ss] + P_j) / rho_j);
+              double Ujs[U];
+              Ujs[0] = rho_j;
+#pragma unroll
+              for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
+              Ujs[U - 1] = E_j;
+              double DFj[U];
+              flux_col<D>(rho_j, rvj, E_j, vj, P_j, ax, DFj);
+              vni = 0;
+#pragma unroll
+              for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
+              vnj = 0;
+#pragma unroll
+              for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
+              double SL = amin1(vni - ci, vnj - cj);
+              double SR = amax1(vni + ci, vnj + cj);
+              double S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
+                           (rho_i * (SL - vni) - rho_j * (SR - vnj));
+              double Fh[U];
+              if (SL > 0) {
+#pragma unroll
+                for (int u = 0; u < U; u++) Fh[u] = DFi[u];
+              } else if (SL <= 0 && S_s >= 0) {
+                double rho_sL = rho_i * (SL - vni) / (SL - S_s);
+                double UsL[U];
+                UsL[0] = rho_sL;
+#pragma unroll
+                for (int a = 0; a < D; a++) UsL[1 + a] = rho_sL * (vi[a] + (S_s - vni) * nrm[a]);
+                UsL[U - 1] = rho_sL * (E_i / rho_i + (S_s - vni) * (S_s + (P_i) / (rho_i * (SL - vni))));
+#pragma unroll
+                for (int u = 0; u < U; u++) Fh[u] = DFi[u] + SL * (UsL[u] - Ui[u]) * sgn;
+              } else if (S_s <= 0 && SR >= 0) {
+                double rho_sR = rho_j * (SR - vnj) / (SR - S_s);
+                double UsR[U];
+                UsR[0] = rho_sR;
+#pragma unroll
+                for (int a = 0; a < D; a++) UsR[1 + a] = rho_sR * (vj[a] + (S_s - vnj) * nrm[a]);
+                UsR[U - 1] = rho_sR * (E_j / rho_j + (S_s - vnj) * (S_s + (P_j) / (rho_j * (SR - vnj))));
+#pragma unroll
+                for (int u = 0; u < U; u++) Fh[u] = DFj[u] + SR * (UsR[u] - Ujs[u]) * sgn;
+              } else if (SR <= 0) {
+#pragma unroll
+                for (int u = 0; u < U; u++) Fh[u] = DFj[u];
+              } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) Fh[u] = 0;
+              }
+#pragma unroll
+              for (int bi = 0; bi < M; bi++)
+#pragma unroll
+                for (int u = 0; u < U; u++) DFn[bi][u] += Fh[u] * Nb[bi] * gw * alph_j;
+            }
+          } else if (wall_like) {
+#pragma unroll
+            for (int bi = 0; bi < M; bi++) DFn[bi][1 + ax] += P_i * Nb[bi] * gw;
+          } else {  // S3 inlet / outlet / symmetry: HLL against a ghost state
+            double rho_j = rho_i;
+            double vj[3] = {0, 0, 0}, rvj[3] = {0, 0, 0};
+#pragma unroll
+            for (int a = 0; a < D; a++) vj[a] = vi[a];
+#pragma unroll
+            for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+            double P_j = P_i, E_j = E_i;
+            if (fg.b == -1) {
+              rho_j = g.inlet_rho;
+#pragma unroll
+              for (int a = 0; a < D; a++) vj[a] = g.inlet_u[a];
+#pragma unroll
+              for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+              P_j = g.inlet_p;
+              double sqv = 0;
+#pragma unroll
+              for (int a = 0; a < D; a++) sqv += vj[a] * vj[a];
+              E_j = eos_rhoe(P_j, me) + 0.5 * sqv * rho_j;
+            }
+            if (j == g.N[1] - 1 || j == 0) {
+              vj[1] = -vi[1];
+#pragma unroll
+              for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+            }
+            if (D == 3 && (k == g.N[2] - 1 || k == 0)) {
+              vj[2] = -vi[2];
+#pragma unroll
+              for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+            }
+            double Ujs[U];
+            Ujs[0] = rho_j;
+#pragma unroll
+            for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
+            Ujs[U - 1] = E_j;
+            double cj = sqrt(g.gamma[ss] * (g.pinf[ss] + P_j) / rho_j);
+            double vnj = 0;
+#pragma unroll
+            for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
+            double DFj[U];
+            flux_col<D>(rho_j, rvj, E_j, vj, P_j, ax, DFj);
+            double SL = amin1(vni - ci, vnj - cj);
+            double SR = amax1(vni + ci, vnj + cj);
+            double Fh[U];
+            if (SL > 0) {
+#pragma unroll
+              for (int u = 0; u < U; u++) Fh[u] = DFi[u];
+            } else if (SL <= 0 && SR >= 0) {
+#pragma unroll
+              for (int u = 0; u < U; u++)
+                Fh[u] = (SR * DFi[u] - SL * DFj[u] + SL * SR * (Ujs[u] - Ui[u]) * sgn) / (SR - SL);
+            } else if (SR < 0) {
+#pragma unroll
+              for (int u = 0; u < U; u++) Fh[u] = DFj[u];
+            } else {
+#pragma unroll
+              for (int u = 0; u < U; u++) Fh[u] = 0;
+            }
+#pragma unroll
+            for (int bi = 0; bi < M; bi++)
+#pragma unroll
+              for (int u = 0; u < U; u++) DFn[bi][u] += Fh[u] * Nb[bi] * gw;
+          }
+        }
+#pragma unroll
+      for (int bi = 0; bi < M; bi++)
+#pragma unroll
+        for (int u = 0; u < U; u++) dc[u][bi] += (-(DFn[bi][u]) * sgn) * fg.S;
+    }
+    // diagonal mass matrix Vol (1, dx^2/12, ...)
+    double mij[M];
+    mij[0] = 1.;
+#pragma unroll
+    for (int a = 0; a < D; a++) mij[1 + a] = hp[a] * hp[a] / 3.;
+#pragma unroll
+    for (int m = 0; m < M; m++) mij[m] *= voli;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      double inv = 1. / (mij[m] + 1e-20);
+#pragma unroll
+      for (int u = 0; u < U; u++) C.dcoef[CIDX(ss, u, m)] = dc[u][m] * inv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Update_grid (S3:mpm.cpp:768-830; T2:mpm.cpp:1254-1320)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int32_t* __restrict__ used,
+                                                     const DevScalars* __restrict__ sc, int md) {
+  constexpr int U = D + 2, M = D + 1;
+  const int nc = g.nc;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc || !used[c]) return;
+  const double dt = sc->dt;
+  const bool s3 = g.sem == PGDG_SEM_S3;
+  double rva[3] = {0, 0, 0};
+  double rho_a = 0, P_a = 0;
+  double cP = C.P[c];
+  double cv[3] = {C.v[c], C.v[nc + c], C.v[2 * nc + c]};
+  double ca[3] = {C.a[c], C.a[nc + c], C.a[2 * nc + c]};
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    if (s >= g.F) break;
+    C.c[s * nc + c] = 0;
+    C.ac[s * nc + c] = 0;
+    double cf[U][M], dcf[U][M];
+#pragma unroll
+    for (int t = 0; t < U; t++) {
+      double l = fabs(C.lim[(s * U + t) * nc + c]);
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        cf[t][m] = C.coef[CIDX(s, t, m)];
+        dcf[t][m] = C.dcoef[CIDX(s, t, m)];
+      }
+#pragma unroll
+      for (int m = 1; m < M; m++) {
+        dcf[t][m] *= l;
+        cf[t][m] *= l;
+      }
+    }
+    double alph = C.alph[s * nc + c];
+    double rho, rv[3], E;
+    if (C.ph_state[s * nc + c] == 1) {
+      if (md == 0) {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int m = 0; m < M; m++) {
+            C.coef0[CIDX(s, u, m)] = cf[u][m];
+            cf[u][m] += dcf[u][m] * dt;
+          }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int m = 0; m < M; m++) cf[u][m] = 0.5 * (dcf[u][m] * dt + cf[u][m] + C.coef0[CIDX(s, u, m)]);
+      }
+      rho = cf[0][0];
+      rv[2] = C.rhov[(s * 3 + 2) * nc + c];
+#pragma unroll
+      for (int a = 0; a < D; a++) rv[a] = cf[1 + a][0];
+      E = cf[U - 1][0];
+      C.rho[s * nc + c] = rho;
+#pragma unroll
+      for (int a = 0; a < D; a++) C.rhov[(s * 3 + a) * nc + c] = rv[a];
+      C.E[s * nc + c] = E;
+      double sq = 0;
+#pragma unroll
+      for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
+      double e_j = E - 0.5 * sq / rho;
+      if (!s3)
+#pragma unroll
+        for (int a = 0; a < 3; a++) cv[a] = rv[a] / rho;
+      cP = eos_p(e_j, Eos{g.gamma[s], g.pinf[s]});
+      double cs = sqrt(g.gamma[s] * (g.pinf[s] + cP) / rho);
+      C.c[s * nc + c] = cs;
+#pragma unroll
+      for (int a = 0; a < D; a++) ca[a] = (dcf[1 + a][0] - dcf[0][0] * cv[a]) / rho;
+      double sqa = 0;
+#pragma unroll
+      for (int a = 0; a < D; a++) sqa += ca[a] * ca[a];
+      C.ac[s * nc + c] = sqrt(sqa);
+    } else {
+      rho = C.rho[s * nc + c];
+#pragma unroll
+      for (int a = 0; a < 3; a++) rv[a] = C.rhov[(s * 3 + a) * nc + c];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        C.coef[CIDX(s, u, m)] = cf[u][m];
+        C.dcoef[CIDX(s, u, m)] = dcf[u][m];
+      }
+#pragma unroll
+    for (int a = 0; a < 3; a++) rva[a] += rv[a] * alph;
+    rho_a += rho * alph;
+    P_a += cP * alph;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) C.v[a * nc + c] = rva[a] / rho_a;
+#pragma unroll
+  for (int a = 0; a < 3; a++) C.a[a * nc + c] = ca[a];
+  C.P[c] = P_a;
+}
+
+#undef CIDX
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int D>
+int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
+                 const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
+                 hipStream_t st) {
+  const int nc = g.nc;
+  switch (which) {
+    case CELL_P2G:
+      hipLaunchKernelGGL(k_p2g<D>, dim3((nc + 127) / 128), dim3(128), 0, st, g, C, P, bin_off,
+                         bin_ent, used);
+      break;
+    case CELL_LIMITER:
+      hipLaunchKernelGGL(k_limiter<D>, dim3((nc + 127) / 128), dim3(128), 0, st, g, C, used);
+      break;
+    case CELL_RHS:
+      hipLaunchKernelGGL(k_rhs<D>, dim3((nc + 127) / 128), dim3(128), 0, st, g, C, used);
+      break;
+    case CELL_UPDATE:
+      hipLaunchKernelGGL(k_update_grid<D>, dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
+                         sc, arg);
+      break;
+    default:
+      return PGDG_ERR_ARG;
+  }
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+template int launch_cells<2>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
+                             const int32_t*, const DevScalars*, int, hipStream_t);
+template int launch_cells<3>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
+                             const int32_t*, const DevScalars*, int, hipStream_t);
+
+}  // namespace pgdg

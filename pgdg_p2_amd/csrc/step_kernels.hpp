@@ -1,0 +1,38 @@
+// step_kernels.hpp - launcher declarations shared by the step translation
+// units (cells, particles, topology) and the context.
+#pragma once
+#include "dev_common.hpp"
+
+namespace pgdg {
+
+enum { CELL_P2G = 0, CELL_LIMITER = 1, CELL_RHS = 2, CELL_UPDATE = 3 };
+
+template <int D>
+int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
+                 const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
+                 hipStream_t st);
+
+// Per-particle stage work.
+struct StageBufs {
+  double *xb, *hb;    // positions / smoothing lengths when the stage's
+                      // neighbour lists are built ([3][cap], [cap])
+  double *x0, *vol0;  // stage-0 start position / volume
+  int32_t* in_ele;    // cell of each particle (membership)
+};
+
+template <int D>
+int launch_stage_save(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, int md,
+                      hipStream_t st);
+template <int D>
+int launch_in_ele(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, hipStream_t st);
+template <int D>
+int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
+               const DevScalars* sc, int md, hipStream_t st);
+template <int D>
+int launch_pst(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S, int64_t n,
+               const int* bin_off, const int* bin_ent, const DevScalars* sc, hipStream_t st);
+template <int D>
+int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,
+                    int64_t n, bool cells, DevScalars* sc, double output_dt, hipStream_t st);
+
+}  // namespace pgdg

@@ -1,0 +1,442 @@
+// step_particles.hip - per-particle kernels of the PG-DG step: G2P state
+// update, particle shifting (PST) with on-the-fly neighbour search, boundary
+// penalty, and the time-step reduction.
+//
+//   k_g2p       Particle_base::Update_state_onestep S3:particle_flow.cpp:413-477
+//   k_pst       Particle_base::Update_PST / PST / Bound_pen S3:particle_flow.cpp:485-622
+//               with the neighbour list of Level_info::Refresh_neighbor_info
+//               (status 2, S3:level_infor.cpp:153-195) generated in place
+//   k_timestep  MPM::Set_timestep S3:mpm.cpp:19-57, Particle_base::Set_timestep
+//               S3:particle.cpp:554-569; Solver::MPM_Solver dt truncation
+//               S3:solver.cpp:33-48
+//
+// Neighbour lists are not stored for the stage update: PST walks the same
+// cells (i, j, k ascending) and bin entries in the same order as the
+// reference list builder, and applies the builder's distance test to the
+// positions and smoothing length saved when the stage started (xb, hb), so
+// it visits exactly the reference list in the reference order.
+#include "dev_common.hpp"
+#include "hip_util.hpp"
+#include "step_kernels.hpp"
+
+namespace pgdg {
+
+template <int D>
+__global__ void k_stage_save(PtlDev P, StageBufs S, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t cap = P.cap;
+#pragma unroll
+  for (int a = 0; a < D; a++) S.xb[a * cap + i] = P.x[a * cap + i];
+  S.hb[i] = P.h[i];
+}
+
+template <int D>
+__global__ void k_in_ele(Geo g, PtlDev P, StageBufs S, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) x[a] = P.x[a * P.cap + i];
+  S.in_ele[i] = cell_of<D>(g, x);
+}
+
+// ---------------------------------------------------------------------------
+// Update_state_onestep (S3:particle_flow.cpp:413-477)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S, int64_t n,
+                                             const DevScalars* __restrict__ sc, int md) {
+  constexpr int U = D + 2, M = D + 1;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t cap = P.cap;
+  const int nc = g.nc;
+  const double dt = sc->dt;
+  const int c = S.in_ele[i];
+  const int ss = P.phase[i] - 1;
+  double divu = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) divu += C.G[(a * 3 + a) * nc + c];
+  double x[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) x[a] = P.x[a * cap + i];
+  double vol = P.vol[i];
+  if (md == 0) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      S.x0[a * cap + i] = x[a];
+      x[a] += P.tv[a * cap + i] * dt;
+    }
+    S.vol0[i] = vol;
+    vol = (1. + divu * dt) * vol;
+  } else {
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = 0.5 * ((S.x0[a * cap + i] + x[a]) + P.tv[a * cap + i] * dt);
+    vol = 0.5 * (S.vol0[i] + vol + divu * dt * vol);
+  }
+  int ci, cj, ck;
+  cell_ijk(g, c, ci, cj, ck);
+  double xc[3] = {g.ps[0] * (ci + 0.5), g.ps[1] * (cj + 0.5), D == 3 ? g.ps[2] * (ck + 0.5) : 0.0};
+  double dri[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) dri[a] = x[a] - xc[a];
+  double Ui[U];
+#pragma unroll
+  for (int t = 0; t < U; t++) {
+    double acc = 0;
+    double c0 = C.coef[(((size_t)ss * U + t) * M) * nc + c];
+    P.coef[(size_t)(t * M) * cap + i] = c0;
+#pragma unroll
+    for (int b = 0; b < M - 1; b++) {
+      double cb = C.coef[(((size_t)ss * U + t) * M + 1 + b) * nc + c];
+      P.coef[(size_t)(t * M + 1 + b) * cap + i] = cb;
+      acc += cb * dri[b];
+    }
+    Ui[t] = c0 + acc;
+  }
+  double rho = Ui[0], E = Ui[U - 1];
+  double rv[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) rv[a] = Ui[1 + a];
+  double v[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) v[a] = rv[a] / rho;
+  double sq = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) sq += v[a] * v[a];
+  double ei = E - 0.5 * rho * sq;
+  double Pp = eos_p(ei, Eos{g.gamma[ss], g.pinf[ss]});
+  double cs = sqrt(g.gamma[ss] * (g.pinf[ss] + Pp) / rho);
+  if (Pp < -0.05 * g.pinf[ss] || rho < 0) {
+#pragma unroll
+    for (int t = 0; t < U; t++)
+#pragma unroll
+      for (int b = 1; b < M; b++) P.coef[(size_t)(t * M + b) * cap + i] = 0;
+    rho = C.rho[ss * nc + c];
+#pragma unroll
+    for (int a = 0; a < 3; a++) rv[a] = C.rhov[(ss * 3 + a) * nc + c];
+    E = C.E[ss * nc + c];
+    cs = C.c[ss * nc + c];
+  }
+#pragma unroll
+  for (int a = 0; a < D; a++) P.x[a * cap + i] = x[a];
+  P.vol[i] = vol;
+  P.rho[i] = rho;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    P.rhov[a * cap + i] = rv[a];
+    P.v[a * cap + i] = v[a];
+  }
+  P.E[i] = E;
+  P.P[i] = Pp;
+  P.c[i] = cs;
+  P.mass[i] = vol * rho;
+  P.h[i] = root_d<D>(vol) * kCellRatio / kCutOff;
+}
+
+// ---------------------------------------------------------------------------
+// Update_PST + PST + Bound_pen (S3:particle_flow.cpp:485-622)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(128) k_pst(Geo g, PtlDev P, WallDev W, StageBufs S, int64_t n,
+                                             const int* __restrict__ bin_off,
+                                             const int* __restrict__ bin_ent,
+                                             const DevScalars* __restrict__ sc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t cap = P.cap;
+  const double dt = sc->dt;
+  const double max_v = sc->max_v;
+  double xi[3], xbi[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    xi[a] = P.x[a * cap + i];
+    xbi[a] = S.xb[a * cap + i];
+  }
+  const double hi = P.h[i], hbi = S.hb[i];
+  const double rhoi = P.rho[i], ci = P.c[i];
+  const int phi = P.phase[i];
+  const double ca = max_v * 5.;
+  const double P_b = rhoi * ca * ca;
+  const double dxk = hi / kCellRatio * kCutOff;
+  const double W_dp = kern_W<D>(dxk, hi);
+  const double rlim = hbi * kCutOff;
+  double pst[3] = {0, 0, 0}, sharp[3] = {0, 0, 0};
+  const int pos = cell_of<D>(g, xbi);
+  int pi, pj, pk;
+  cell_ijk(g, pos, pi, pj, pk);
+  const int r = g.radius;
+  const int rk = D == 3 ? r : 0;
+  for (int t = max(pi - r, 0); t <= min(pi + r, g.N[0] - 1); t++)
+    for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
+      for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
+        int cc = (t * g.N[1] + s) * g.N[2] + m;
+        int e0 = bin_off[cc], e1 = bin_off[cc + 1];
+        for (int e = e0; e < e1; e++) {
+          int ref = bin_ent[e];
+          bool wl = ref_is_wall(ref);
+          int w = wl ? ref_wall(ref) : 0;
+          double xe[3], xeb[3];
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            if (wl) {
+              xe[a] = W.x[a * W.n + w];
+              xeb[a] = xe[a];
+            } else {
+              xe[a] = P.x[a * cap + ref];
+              xeb[a] = S.xb[a * cap + ref];
+            }
+          }
+          double sqb = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            double d = xbi[a] - xeb[a];
+            sqb += d * d;
+          }
+          if (!(sqrt(sqb) <= rlim)) continue;
+          double dr[3];
+#pragma unroll
+          for (int a = 0; a < D; a++) dr[a] = xi[a] - xe[a];
+          double sq = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) sq += dr[a] * dr[a];
+          double dist = sqrt(sq);
+          if (dist != 0) {
+            double vol_j = wl ? W.vol[w] : P.vol[ref];
+            double aw = kern_dW<D>(dist, hi);
+            double Wij = kern_W<D>(dist, hi);
+            double rr = Wij / W_dp;
+            double r2 = rr * rr;
+            double R = 0.2 * (r2 * r2);
+            double tmp1 = -1. * P_b * (1. / rhoi) * (1 + R);
+            if (!wl) {
+              double rhoj = P.rho[ref];
+              double tmp2 = 1. / (rhoi + rhoj);
+#pragma unroll
+              for (int a = 0; a < D; a++) pst[a] += dr[a] * aw * tmp1 * vol_j;
+              if (P.phase[ref] != phi) {
+                double cj = P.c[ref];
+                double kk = -0.1 * tmp2 * (ci * ci * rhoi + cj * cj * rhoj);
+#pragma unroll
+                for (int a = 0; a < D; a++) sharp[a] += kk * (dr[a] * aw) * vol_j;
+              }
+            } else {
+#pragma unroll
+              for (int a = 0; a < D; a++) pst[a] += W.norm[a * W.n + w] * Wij * tmp1 * vol_j;
+            }
+          }
+        }
+      }
+  // Bound_pen over the boundary faces of the particle's cell
+  double pen[3] = {0, 0, 0};
+  {
+    int c = S.in_ele[i];
+    int ci_, cj_, ck_;
+    cell_ijk(g, c, ci_, cj_, ck_);
+    const double xc[3] = {g.ps[0] * (ci_ + 0.5), g.ps[1] * (cj_ + 0.5),
+                          D == 3 ? g.ps[2] * (ck_ + 0.5) : 0.0};
+    for (int f = 0; f < 2 * D; f++) {
+      int ax, ni = ci_, nj = cj_, nk = ck_;
+      double sgn, cen[3] = {xc[0], xc[1], xc[2]};
+      switch (f) {
+        case 0: ax = 1; sgn = -1; cen[1] = xc[1] - 0.5 * g.ps[1]; nj--; break;
+        case 1: ax = 0; sgn = 1; cen[0] = xc[0] + 0.5 * g.ps[0]; ni++; break;
+        case 2: ax = 1; sgn = 1; cen[1] = xc[1] + 0.5 * g.ps[1]; nj++; break;
+        case 3: ax = 0; sgn = -1; cen[0] = xc[0] - 0.5 * g.ps[0]; ni--; break;
+        case 4: ax = 2; sgn = -1; cen[2] = xc[2] - 0.5 * g.ps[2]; nk--; break;
+        default: ax = 2; sgn = 1; cen[2] = xc[2] + 0.5 * g.ps[2]; nk++; break;
+      }
+      bool in = ni >= 0 && ni < g.N[0] && nj >= 0 && nj < g.N[1] && nk >= 0 && nk < g.N[2];
+      if (in || g.bc[f] == 0) continue;
+      double nrm[3] = {0, 0, 0};
+      nrm[ax] = sgn;
+      double ds = 0;
+#pragma unroll
+      for (int a = 0; a < D; a++) ds += (xi[a] - cen[a]) * nrm[a];
+      if (ds < 0)
+#pragma unroll
+        for (int a = 0; a < 3; a++) nrm[a] = -1 * nrm[a];
+      double dis = fabs(ds);
+      if (dis < 0.4 * g.p_sz) {
+        double kk = 0.02 / dis * (ci * ci);
+#pragma unroll
+        for (int a = 0; a < D; a++) pen[a] += kk * nrm[a];
+      }
+    }
+  }
+  const double px = g.p_sz;
+  double dx[3], dx1[3], dx2[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    dx[a] = pst[a] * dt * dt / 2.;
+    dx1[a] = sharp[a] * dt * dt / 2.;
+    dx2[a] = pen[a] * dt * dt / 2.;
+  }
+  auto clampv = [&](double* d, double cap_) {
+    double s2 = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) s2 += d[a] * d[a];
+    double dis = sqrt(s2);
+    double lim = amin1(cap_, dis) / (dis + 1e-20);
+#pragma unroll
+    for (int a = 0; a < D; a++) d[a] *= lim;
+  };
+  clampv(dx, px / 10);
+  clampv(dx1, px / 10);
+  clampv(dx2, px / 2);
+  const bool frozen = g.sem == PGDG_SEM_S3 && P.mark_in[i] == 1;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    double dv = frozen ? 0.0 : (dx[a] + dx1[a] + dx2[a]) / dt;
+    P.tv[a * cap + i] = P.v[a * cap + i] + dv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Set_timestep reduction + Solver dt truncation
+// ---------------------------------------------------------------------------
+__device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+__global__ void k_ts_init(DevScalars* sc, double output_dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (!(sc->integral_time < output_dt - 1.0e-10)) sc->integral_time = 0.0;
+    sc->red_dt = dbits(1.e20);
+    sc->red_maxv = dbits(0.0);
+  }
+}
+
+template <int D, bool CELLS>
+__global__ void __launch_bounds__(256) k_timestep(Geo g, PtlDev P, CellDev C,
+                                                  const int32_t* __restrict__ used, int64_t n,
+                                                  DevScalars* sc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double ts = 1.e20, vm = 0.0;
+  if (i < n && (!CELLS || used[i])) {
+    double v[3], h, c12, a12;
+    if (CELLS) {
+      const int nc = g.nc;
+#pragma unroll
+      for (int a = 0; a < D; a++) v[a] = C.v[a * nc + i];
+      h = kCellRatio / kCutOff * g.p_sz;
+      c12 = amax1(C.c[i], C.c[nc + i]);
+      a12 = amax1(C.ac[i], C.ac[nc + i]);
+    } else {
+#pragma unroll
+      for (int a = 0; a < D; a++) v[a] = P.v[a * P.cap + i];
+      h = P.h[i];
+      c12 = amax1(P.c[i], 0.0);
+      a12 = amax1(0.0, 0.0);
+    }
+    double sq = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) sq += v[a] * v[a];
+    double Uv = sqrt(sq);
+    double cfl = 0.3;
+    double ds = h / kCellRatio * kCutOff;
+    double t1 = cfl * ds / (c12 + Uv);
+    double t2 = cfl * sqrt(ds / a12);
+    ts = amin1(t1, t2);
+    vm = Uv;
+  }
+  // wavefront reductions (min/max are order independent)
+  for (int off = 32; off > 0; off >>= 1) {
+    double o1 = __shfl_down(ts, off, 64);
+    double o2 = __shfl_down(vm, off, 64);
+    ts = amin1(ts, o1);
+    vm = amax1(vm, o2);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (ts > 0) atomicMin(&sc->red_dt, dbits(ts));
+    if (vm >= 0) atomicMax(&sc->red_maxv, dbits(vm));
+  }
+}
+
+__global__ void k_ts_finalize(DevScalars* sc, double output_dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double dt = __longlong_as_double((long long)sc->red_dt);
+    double mv = __longlong_as_double((long long)sc->red_maxv);
+    sc->max_v = mv;
+    double timestep = dt;
+    if (sc->integral_time >= (output_dt - timestep)) timestep = output_dt - sc->integral_time;
+    sc->dt = timestep;
+    sc->integral_time += timestep;
+    sc->run_time += timestep;
+    sc->iterate_num += 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <int D>
+int launch_stage_save(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, int md,
+                      hipStream_t st) {
+  (void)g;
+  (void)md;
+  if (n <= 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_stage_save<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, P, S, n);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+template <int D>
+int launch_in_ele(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, hipStream_t st) {
+  if (n <= 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_in_ele<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, S, n);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+template <int D>
+int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
+               const DevScalars* sc, int md, hipStream_t st) {
+  if (n <= 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_g2p<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, n,
+                     sc, md);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+template <int D>
+int launch_pst(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S, int64_t n,
+               const int* bin_off, const int* bin_ent, const DevScalars* sc, hipStream_t st) {
+  if (n <= 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_pst<D>, dim3(grid_for(n, 128, 1LL << 30)), dim3(128), 0, st, g, P, W, S, n,
+                     bin_off, bin_ent, sc);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+template <int D>
+int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,
+                    int64_t n, bool cells, DevScalars* sc, double output_dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_ts_init, dim3(1), dim3(64), 0, st, sc, output_dt);
+  int64_t cnt = cells ? g.nc : n;
+  if (cnt > 0) {
+    unsigned grid = (unsigned)((cnt + 255) / 256);
+    if (cells)
+      hipLaunchKernelGGL((k_timestep<D, true>), dim3(grid), dim3(256), 0, st, g, P, C, used, cnt, sc);
+    else
+      hipLaunchKernelGGL((k_timestep<D, false>), dim3(grid), dim3(256), 0, st, g, P, C, used, cnt, sc);
+  }
+  hipLaunchKernelGGL(k_ts_finalize, dim3(1), dim3(64), 0, st, sc, output_dt);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+#define PGDG_INST(D)                                                                              \
+  template int launch_stage_save<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, int,     \
+                                    hipStream_t);                                                 \
+  template int launch_in_ele<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, hipStream_t); \
+  template int launch_g2p<D>(const Geo&, const PtlDev&, const CellDev&, const StageBufs&, int64_t, \
+                             const DevScalars*, int, hipStream_t);                                \
+  template int launch_pst<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&, int64_t, \
+                             const int*, const int*, const DevScalars*, hipStream_t);             \
+  template int launch_timestep<D>(const Geo&, const PtlDev&, const CellDev&, const int32_t*,       \
+                                  int64_t, bool, DevScalars*, double, hipStream_t);
+PGDG_INST(2)
+PGDG_INST(3)
+#undef PGDG_INST
+
+}  // namespace pgdg

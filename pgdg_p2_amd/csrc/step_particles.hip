@@ -175,6 +175,7 @@ __global__ void __launch_bounds__(128) k_pst(Geo g, PtlDev P, WallDev W, StageBu
         int e0 = bin_off[cc], e1 = bin_off[cc + 1];
         for (int e = e0; e < e1; e++) {
           int ref = bin_ent[e];
+          if (ref == -1) continue;
           bool wl = ref_is_wall(ref);
           int w = wl ? ref_wall(ref) : 0;
           double xe[3], xeb[3];

@@ -199,6 +199,7 @@ __global__ void __launch_bounds__(128) k_lists(Geo g, const double* __restrict__
         int cc = (t * g.N[1] + s) * g.N[2] + m;
         for (int e = bin_off[cc]; e < bin_off[cc + 1]; e++) {
           int ref = bin_ent[e];
+          if (ref == -1) continue;
           double sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) {

@@ -1,0 +1,199 @@
+"""Benchmark of the PG-DG time step on one MI355X.
+
+Workload (default): the 3-D shock-bubble step, channel 256x128x128 P1 DG cells
+with 8 particles per cell (2x2x2 sub-lattice, seeded jitter), air with a
+helium sphere hit by a Mach 1.22 shock; the reference's two-stage step with
+split/merge, FP64.  `value` is whole-job cell-steps/s with the state resident
+in HBM.  A secondary measurement runs the batched MGFM solve on 1e7 interface
+face pairs (stiffened gas) and reports Newton iterations per face.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size full|medium|small]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+SIZES = {"full": (256, 128, 128), "medium": (128, 64, 64), "small": (32, 16, 16)}
+
+
+def stage_bytes(stage: str, case, n: int, nwall: int) -> float:
+    """Algorithmic HBM bytes of one launch of a stage: every input element
+    read once, every output element written once (FP64 = 8 B, int32 = 4 B)."""
+    D, F = case.dim, case.nphase
+    U, M = D + 2, D + 1
+    nc = case.n[0] * case.n[1] * case.n[2]
+    bins = (n + nwall) * 4 + (nc + 1) * 4
+    if stage == "pst":
+        # x, xb (2D), h, hb, rho, c, vol, v (D) read; phase, mark_in, in_ele;
+        # tv (D) written
+        return n * ((3 * D + 5) * 8 + 12 + D * 8) + bins
+    if stage == "g2p":
+        # x, tv, x0 (3D) + vol, vol0 read; in_ele, phase; cell coef/G/rho.. ;
+        # x, vol, rho, rhov(3), v(3), E, P, c, mass, h, coef written
+        return n * ((3 * D + 2) * 8 + 8 + (D + 11 + 6 + U * M) * 8) + nc * (F * U * M + 9 + 4 * F) * 8
+    if stage == "rhs":
+        # coef, lim, alph, ph_state read; dcoef written
+        return nc * (F * U * M * 16 + F * U * 8 + F * 12)
+    if stage == "p2g":
+        # particle x, vol, rho, rhov, E, coef, phase read; cell alph, ph_state,
+        # ph_state_b, v, P written
+        return n * ((2 * D + 3 + U * M) * 8 + 4) + bins + nc * (F * 16 + 32)
+    if stage == "limiter":
+        return nc * (F * (U * M + 2 + D + 1) * 8 + 24 + F * U * 8 + 72)
+    if stage == "update_grid":
+        return nc * (F * U * M * 8 * 5 + F * U * 8 + F * 8 * 6 + 48)
+    if stage == "lists":
+        return n * ((D + 1) * 8 + 8) + bins
+    if stage == "bin":
+        return (n + nwall) * (D * 8 + 12) + bins
+    return 0.0
+
+
+def cpu_baseline(sample=(48, 24, 24), steps=2):
+    """The oracle (scalar C++ restatement of the reference step) on a reduced
+    grid of the same workload, timed on this host."""
+    from oracle.pyoracle_step import OracleSim
+    from pgdg_p2_amd import cases
+    case, st = cases.shock_bubble_3d(nx=sample[0], ny=sample[1], nz=sample[2], ppc_axis=2,
+                                     jitter=0.25)
+    sim = OracleSim(case, st)
+    sim.step(1)  # first step (particle dt path) excluded
+    t0 = time.perf_counter()
+    sim.step(steps)
+    el = time.perf_counter() - t0
+    nc = sample[0] * sample[1] * sample[2]
+    return {"value": nc * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{sample[0]}x{sample[1]}x{sample[2]} cells x 8 ppc ({st.n} particles), "
+                      f"steps 2..{steps + 1}, same initial condition, oracle FP64 single thread"}
+
+
+def mgfm_bench(n=10_000_000):
+    """Batched MGFM on n stiffened-gas interface face pairs resident on the
+    device: faces/s and Newton iterations per face for the reference
+    tolerance and for 1e-12 relative on p*."""
+    import numpy as np
+    import pgdg_p2_amd as P
+    from pgdg_p2_amd import workloads
+    inp = P.DeviceArray.from_numpy(workloads.mgfm_faces(n))
+    out = P.DeviceArray((3, n), np.float64)
+    its = P.DeviceArray((n,), np.int32)
+    gl, pil, gr, pir = workloads.MGFM_EOS
+    res = {}
+    for name, mode, it in (("reference_tol", P.TOL_REFERENCE, 50), ("rel_1e-12", P.TOL_RELATIVE, 100)):
+        P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12, max_iter=it)
+        P.device_sync()
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12,
+                             max_iter=it)
+        P.device_sync()
+        el = (time.perf_counter() - t0) / reps
+        res[name] = {"faces_per_s": n / el, "newton_its_per_face": float(its.numpy().mean()),
+                     "ms": el * 1e3}
+    res["faces"] = n
+    for d in (inp, out, its):
+        d.free()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", default="full", choices=list(SIZES))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mgfm", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "slab decomposition is a design, not a build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from pgdg_p2_amd import cases
+    from pgdg_p2_amd.step import GpuSim
+
+    nx, ny, nz = SIZES[args.size]
+    t_setup = time.perf_counter()
+    case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25, cap_factor=1.0)
+    n0 = st.n
+    gpu = GpuSim(case, st, cap=int(n0 * 2.2) + 4096)
+    del st
+    print(f"setup {time.perf_counter() - t_setup:.1f}s: {nx}x{ny}x{nz} cells, {n0} particles",
+          file=sys.stderr, flush=True)
+    nc = nx * ny * nz
+    gpu.step(args.warmup)
+    gpu.sync()
+    n_start = gpu.count()
+    t0 = time.perf_counter()
+    gpu.step(args.steps)
+    gpu.sync()
+    el = time.perf_counter() - t0
+    n_end = gpu.count()
+    ms_step = el / args.steps * 1e3
+    value = nc * args.steps / el
+    print(f"timed {args.steps} steps: {ms_step:.2f} ms/step", file=sys.stderr, flush=True)
+
+    # per-stage breakdown on extra steps (HIP events on the context stream)
+    gpu.profile(True)
+    gpu.step(args.profile_steps)
+    gpu.sync()
+    prof = gpu.profile_read()
+    gpu.profile(False)
+    stages = {k: {"ms_total": v[0], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
+    dom = max(stages, key=lambda k: stages[k]["ms_total"])
+    n_mid = gpu.count()
+    from pgdg_p2_amd.state import HostState  # noqa: F401
+    walls = (nx + 1) * (ny + 1) * (nz + 1) - (nx - 1) * (ny - 1) * (nz - 1)
+    per_launch_ms = stages[dom]["ms_total"] / stages[dom]["launches"]
+    alg = stage_bytes(dom, case, n_mid, walls)
+    achieved = alg / (per_launch_ms * 1e-3) / 1e9 if alg > 0 else None
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None, "alg_bytes_per_launch": alg, "avg_launch_ms": per_launch_ms}
+    gpu.close()
+
+    line = {
+        "metric": "P2 DG cell-steps/s + MGFM Newton its/face, mixed vs FP64 plain at equal L1 err",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D shock-bubble step, {nx}x{ny}x{nz} cells, 8 particles/cell, "
+                               "P1 DG (reference basis), reference two-stage step with split/merge, "
+                               "plain FP64",
+                   "cells": nc, "particles_start": n_start, "particles_end": n_end,
+                   "parallelism": "single GPU"},
+        "particles_per_s": (n_start + n_end) / 2 * args.steps / el,
+        "stages_ms": {k: round(v["ms_total"] / args.profile_steps, 4) for k, v in stages.items()},
+        "roofline": roof,
+    }
+    if not args.no_mgfm:
+        line["mgfm"] = mgfm_bench()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"]["host_cpus"] = os.cpu_count()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,14 @@ const char* pgdg_version(void);
 /* Number of visible HIP devices (0 when none).  Makes no allocation. */
 int pgdg_device_count(int* count);
 
+/* Device memory helpers, so that callers without a GPU framework of their
+ * own can stage inputs (no reference counterpart).  kind: 0 host->device,
+ * 1 device->host, 2 device->device.  pgdg_device_sync waits for all work. */
+int pgdg_malloc(void** ptr, int64_t bytes);
+int pgdg_free(void* ptr);
+int pgdg_memcpy(void* dst, const void* src, int64_t bytes, int kind);
+int pgdg_device_sync(void);
+
 /* ======================================================================== *
  *  MGFM two-medium Riemann solve (A9) and stiffened-gas EOS (A10)          *
  * ======================================================================== */
@@ -89,6 +97,11 @@ typedef struct pgdg_mgfm_opts {
 int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, int64_t n,
                     const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts,
                     const double* pstar0, void* stream);
+
+/* Elementwise y = x^(1./dim) rounded to nearest, the particle spacing
+ * pow(vol, 1./DIM) of S3:mpm.cpp:1089 / S3:particle_flow.cpp:473
+ * (dim 2: sqrt; dim 3: correctly rounded x^(1./3.)).  x, y device.       */
+int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream);
 
 /* ======================================================================== *
  *  PG-DG time step: case description and host state layout                *

@@ -23,6 +23,7 @@ def lib() -> C.CDLL:
         L.oracle_mgfm_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                         C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p]
+        L.oracle_root_dim.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
         try:
             from . import pyoracle_step
             pyoracle_step.declare(L)
@@ -50,3 +51,10 @@ def mgfm_batch(inp: np.ndarray, gl, pinfl, gr, pinfr, variant=0, mode=0, rtol=1e
     lib().oracle_mgfm_batch(_ptr(inp), _ptr(out), _ptr(it), n, gl, pinfl, gr, pinfr, variant,
                             mode, rtol, max_iter, g)
     return out, it
+
+
+def root_dim(x: np.ndarray, dim: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().oracle_root_dim(_ptr(x), _ptr(y), x.size, dim)
+    return y

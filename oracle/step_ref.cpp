@@ -1613,5 +1613,8 @@ int oracle_step(void* h, int nsteps) {
   return 0;
 }
 int oracle_call(void* h, int fn, int arg) { return static_cast<oracle::Base*>(h)->call(fn, arg); }
+void oracle_root_dim(const double* x, double* y, int64_t n, int dim) {
+  for (int64_t i = 0; i < n; i++) y[i] = oracle::root_dim(dim, x[i]);
+}
 
 }  // extern "C"

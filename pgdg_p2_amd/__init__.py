@@ -61,6 +61,10 @@ def lib() -> C.CDLL:
         L.pgdg_mgfm_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                       C.POINTER(EosPair), C.POINTER(MgfmOpts), C.c_void_p,
                                       C.c_void_p]
+        L.pgdg_malloc.argtypes = [C.POINTER(C.c_void_p), C.c_int64]
+        L.pgdg_free.argtypes = [C.c_void_p]
+        L.pgdg_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+        L.pgdg_root_dim.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]
         from . import _bind  # noqa: F401  (declares the step API signatures)
         _bind.declare(L)
         _lib = L
@@ -96,3 +100,90 @@ def mgfm_batch(inp, gamma_l, pinf_l, gamma_r, pinf_r, variant=MGFM_S3, tol_mode=
                                 C.c_void_p(iters.data_ptr()), n, C.byref(eos), C.byref(opts),
                                 p0, _stream_ptr(stream)), "pgdg_mgfm_batch")
     return out, iters
+
+
+class DeviceArray:
+    """A device buffer owned through the library's C ABI (no GPU framework
+    needed).  Holds `shape`/`dtype` for round trips with numpy."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        check(lib().pgdg_malloc(C.byref(self.ptr), self.nbytes), "pgdg_malloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        check(lib().pgdg_memcpy(d.ptr, C.c_void_p(a.ctypes.data), d.nbytes, 0), "pgdg_memcpy")
+        return d
+
+    def numpy(self):
+        import numpy as np
+        a = np.empty(self.shape, dtype=self.dtype)
+        check(lib().pgdg_memcpy(C.c_void_p(a.ctypes.data), self.ptr, self.nbytes, 1), "pgdg_memcpy")
+        return a
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            lib().pgdg_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    check(lib().pgdg_device_sync(), "pgdg_device_sync")
+
+
+def mgfm_batch_np(inp, gamma_l, pinf_l, gamma_r, pinf_r, variant=MGFM_S3,
+                  tol_mode=TOL_REFERENCE, rtol=1e-12, max_iter=50, pstar0=None):
+    """Batched MGFM on host numpy input [6, n]; returns (out [3, n], iters [n])."""
+    import numpy as np
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    if inp.ndim != 2 or inp.shape[0] != 6:
+        raise ValueError("inp must have shape [6, n]")
+    n = inp.shape[1]
+    d_in = DeviceArray.from_numpy(inp)
+    d_out = DeviceArray((3, n), np.float64)
+    d_it = DeviceArray((n,), np.int32)
+    d_p0 = DeviceArray.from_numpy(np.asarray(pstar0, dtype=np.float64)) if pstar0 is not None else None
+    mgfm_batch_dev(d_in, d_out, d_it, n, gamma_l, pinf_l, gamma_r, pinf_r, variant, tol_mode,
+                   rtol, max_iter, d_p0)
+    device_sync()
+    return d_out.numpy(), d_it.numpy()
+
+
+def mgfm_batch_dev(d_in, d_out, d_it, n, gamma_l, pinf_l, gamma_r, pinf_r, variant=MGFM_S3,
+                   tol_mode=TOL_REFERENCE, rtol=1e-12, max_iter=50, d_p0=None):
+    """Batched MGFM on DeviceArray buffers (null stream, asynchronous)."""
+    eos = EosPair(gamma_l, pinf_l, gamma_r, pinf_r)
+    opts = MgfmOpts(variant, tol_mode, rtol, max_iter)
+    check(lib().pgdg_mgfm_batch(d_in.ptr, d_out.ptr, d_it.ptr if d_it is not None else None, n,
+                                C.byref(eos), C.byref(opts), d_p0.ptr if d_p0 is not None else None,
+                                None), "pgdg_mgfm_batch")
+
+
+def root_dim_np(x, dim):
+    """x^(1./dim) on the device for a host numpy array."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dx = DeviceArray.from_numpy(x)
+    dy = DeviceArray(x.shape, np.float64)
+    check(lib().pgdg_root_dim(dx.ptr, dy.ptr, x.size, dim, None), "pgdg_root_dim")
+    device_sync()
+    return dy.numpy()
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().pgdg_device_count(C.byref(n)), "pgdg_device_count")
+    return n.value

@@ -94,31 +94,33 @@ def _init_cells(st: HostState, case: Case):
 
 
 def _walls_nodes(case: Case, spacing, vol):
-    """Boundary nodes of a lattice with `spacing` per axis, in sx, sy[, sz] order."""
+    """Boundary nodes of a lattice with `spacing` per axis, in sx, sy[, sz]
+    order, with the first matching outward normal of the reference
+    (sx==0, sx==N, sy==0, sy==N, sz==0, sz==N)."""
     D = case.dim
     cnt = [round((case.box_r[a] - case.box_l[a]) / spacing[a]) for a in range(D)]
-    xs, ns = [], []
-    rng = [range(cnt[a] + 1) for a in range(D)]
-    import itertools
-    for idx in itertools.product(*rng):
-        on = [idx[a] == 0 or idx[a] == cnt[a] for a in range(D)]
-        if not any(on):
-            continue
-        x = [case.box_l[a] + spacing[a] * (idx[a] + 0.0) for a in range(D)]
-        nrm = [0.0, 0.0, 0.0]
-        # first matching condition: sx==0, sx==N, sy==0, sy==N, sz==0, sz==N
-        for a in range(D):
-            if idx[a] == 0:
-                nrm[a] = -1.0
-                break
-            if idx[a] == cnt[a]:
-                nrm[a] = 1.0
-                break
-        xs.append(x + [0.0] * (3 - D))
-        ns.append(nrm)
-    x = np.array(xs).T.copy()
-    nrm = np.array(ns).T.copy()
-    return x, nrm, np.full(x.shape[1], vol)
+    idx = np.meshgrid(*[np.arange(cnt[a] + 1) for a in range(D)], indexing="ij")
+    idx = [g.reshape(-1) for g in idx]
+    lo = [idx[a] == 0 for a in range(D)]
+    hi = [idx[a] == cnt[a] for a in range(D)]
+    on = np.zeros_like(lo[0])
+    for a in range(D):
+        on |= lo[a] | hi[a]
+    sel = np.nonzero(on)[0]
+    nw = sel.size
+    x = np.zeros((3, nw))
+    for a in range(D):
+        x[a] = case.box_l[a] + spacing[a] * (idx[a][sel] + 0.0)
+    nrm = np.zeros((3, nw))
+    done = np.zeros(nw, dtype=bool)
+    for a in range(D):
+        m = lo[a][sel] & ~done
+        nrm[a, m] = -1.0
+        done |= m
+        m = hi[a][sel] & ~done
+        nrm[a, m] = 1.0
+        done |= m
+    return x, nrm, np.full(nw, vol)
 
 
 def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nphase=2,

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <new>
+#include <vector>
 
 #include "dev_common.hpp"
 #include "hip_util.hpp"
@@ -42,8 +43,56 @@ __global__ void k_used_mark(Geo g, const int32_t* ph_state, int32_t* used, int D
       for (int q = max(k - rk, 0); q <= min(k + rk, g.N[2] - 1); q++) used[(t * g.N[1] + s) * g.N[2] + q] = 1;
 }
 
+// Stage timer: when enabled, every stage is bracketed by two HIP events on
+// the context stream; pgdg_profile_read sums the elapsed times per slot.
+enum {
+  PS_TIMESTEP = 0, PS_USED, PS_BIN, PS_SPLIT, PS_MERGE, PS_MEMBERSHIP, PS_STAGE_SAVE, PS_P2G,
+  PS_LIMITER, PS_RHS, PS_UPDATE_GRID, PS_G2P, PS_PST, PS_LISTS, PS_NSLOT
+};
+
+struct Profiler {
+  bool on = false;
+  struct Rec {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  int begin(int slot, hipStream_t st, Rec& r) {
+    r.slot = slot;
+    PGDG_HIP_TRY(hipEventCreate(&r.a));
+    PGDG_HIP_TRY(hipEventCreate(&r.b));
+    PGDG_HIP_TRY(hipEventRecord(r.a, st));
+    return PGDG_OK;
+  }
+  int end(Rec& r, hipStream_t st) {
+    PGDG_HIP_TRY(hipEventRecord(r.b, st));
+    recs.push_back(r);
+    return PGDG_OK;
+  }
+  int read(double* ms, int64_t* cnt, int nslot) {
+    for (int k = 0; k < nslot; k++) {
+      ms[k] = 0;
+      cnt[k] = 0;
+    }
+    for (auto& r : recs) {
+      PGDG_HIP_TRY(hipEventSynchronize(r.b));
+      float t = 0;
+      PGDG_HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+      if (r.slot < nslot) {
+        ms[r.slot] += t;
+        cnt[r.slot] += 1;
+      }
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    recs.clear();
+    return PGDG_OK;
+  }
+};
+
 struct Ctx {
   pgdg_case cs;
+  Profiler prof;
   Geo g;
   int D, U, M, F, nc;
   hipStream_t st = nullptr;
@@ -58,6 +107,9 @@ struct Ctx {
   int32_t* used = nullptr;
   StageBufs S{};
   double* s_slab = nullptr;
+  BinData B{};
+  double* b_slab = nullptr;
+  int32_t* bi_slab = nullptr;
   Topo T;
   DevScalars* sc = nullptr;
   int64_t iterate_num = 0;
@@ -76,6 +128,8 @@ struct Ctx {
   int run_simulation(int64_t n);
   template <int D_>
   int stage(int fn, int arg);
+  template <int D_>
+  int stage_impl(int fn, int arg);
   int step(int nsteps);
 };
 
@@ -168,11 +222,26 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   S.x0 = s_slab + 4 * cap;
   S.vol0 = s_slab + 7 * cap;
   PGDG_HIP_TRY(hipMalloc((void**)&S.in_ele, sizeof(int32_t) * cap));
+  // bin-ordered neighbour data: x[3], xb[3], vol, rho, c per entry; boxes
+  const int64_t ne = cap + nwall;
+  B.ne_cap = ne;
+  PGDG_HIP_TRY(hipMalloc((void**)&b_slab, sizeof(double) * (9 * ne + 6 * (size_t)nc)));
+  PGDG_HIP_TRY(hipMalloc((void**)&bi_slab, sizeof(int32_t) * 2 * ne));
+  B.x = b_slab;
+  B.xb = b_slab + 3 * ne;
+  B.vol = b_slab + 6 * ne;
+  B.rho = b_slab + 7 * ne;
+  B.c = b_slab + 8 * ne;
+  B.bmin = b_slab + 9 * ne;
+  B.bmax = B.bmin + 3 * (size_t)nc;
+  B.ph = bi_slab;
+  B.ref = bi_slab + ne;
   PGDG_HIP_TRY(hipMalloc((void**)&sc, sizeof(DevScalars)));
   PGDG_HIP_TRY(hipMemset(sc, 0, sizeof(DevScalars)));
   PGDG_TRY(T.init(cap, nwall, nc, st));
   PGDG_HIP_TRY(hipStreamSynchronize(st));
-  bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall) + T.bytes_total;
+  bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall +
+                            9 * (size_t)ne + 6 * (size_t)nc) + T.bytes_total;
   return PGDG_OK;
 }
 
@@ -180,7 +249,7 @@ void Ctx::release() {
   if (st) (void)hipStreamSynchronize(st);
   PA.release();
   T.release();
-  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc};
+  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (own_stream && st) (void)hipStreamDestroy(st);
@@ -342,8 +411,36 @@ enum {
   FN_UPDATE_PARTICLE = 11, FN_RUN = 12
 };
 
+static int prof_slot(int fn) {
+  switch (fn) {
+    case 0: return PS_TIMESTEP;
+    case 1: return PS_USED;
+    case 2: return PS_BIN;
+    case 3: return PS_SPLIT;
+    case 4: return PS_MERGE;
+    case 5: return PS_MEMBERSHIP;
+    case 6: return PS_STAGE_SAVE;
+    case 7: return PS_P2G;
+    case 8: return PS_LIMITER;
+    case 9: return PS_RHS;
+    case 10: return PS_UPDATE_GRID;
+    default: return -1;
+  }
+}
+
 template <int D_>
 int Ctx::stage(int fn, int arg) {
+  int slot = prof.on ? prof_slot(fn) : -1;
+  if (slot < 0) return stage_impl<D_>(fn, arg);
+  Profiler::Rec r;
+  PGDG_TRY(prof.begin(slot, st, r));
+  int rc = stage_impl<D_>(fn, arg);
+  PGDG_TRY(prof.end(r, st));
+  return rc;
+}
+
+template <int D_>
+int Ctx::stage_impl(int fn, int arg) {
   PtlDev P = PA.view();
   switch (fn) {
     case FN_SET_TIMESTEP:
@@ -381,9 +478,18 @@ int Ctx::stage(int fn, int arg) {
       return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
     case FN_UPDATE_GRID:
       return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, arg, st);
-    case FN_UPDATE_PARTICLE:
+    case FN_UPDATE_PARTICLE: {
+      Profiler::Rec r;
+      if (prof.on) PGDG_TRY(prof.begin(PS_G2P, st, r));
       PGDG_TRY(launch_g2p<D_>(g, P, C, S, np, sc, arg, st));
-      return launch_pst<D_>(g, P, W, S, np, T.bin_off, T.bin_ent, sc, st);
+      if (prof.on) {
+        PGDG_TRY(prof.end(r, st));
+        PGDG_TRY(prof.begin(PS_PST, st, r));
+      }
+      PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, T.n_ent, T.bin_off, T.bin_ent, sc, st));
+      if (prof.on) PGDG_TRY(prof.end(r, st));
+      return PGDG_OK;
+    }
     case FN_RUN:
       return run_simulation<D_>(arg);
     default:
@@ -409,17 +515,21 @@ int Ctx::run_simulation(int64_t n) {
   }
   // lists the next step's split/merge passes read (built from the last
   // stage's neighbour-list inputs)
-  return T.end_of_step_lists<D_>(g, PA.view(), S.xb, S.hb, W, np);
+  Profiler::Rec r;
+  if (prof.on) PGDG_TRY(prof.begin(PS_LISTS, st, r));
+  PGDG_TRY(T.end_of_step_lists<D_>(g, PA.view(), S.xb, S.hb, W, np));
+  if (prof.on) PGDG_TRY(prof.end(r, st));
+  return PGDG_OK;
 }
 
 int Ctx::step(int nsteps) {
   for (int s = 0; s < nsteps; s++) {
     if (D == 2) {
-      PGDG_TRY(set_timestep<2>());
+      PGDG_TRY(stage<2>(FN_SET_TIMESTEP, 0));
       iterate_num++;
       PGDG_TRY(run_simulation<2>(iterate_num));
     } else {
-      PGDG_TRY(set_timestep<3>());
+      PGDG_TRY(stage<3>(FN_SET_TIMESTEP, 0));
       iterate_num++;
       PGDG_TRY(run_simulation<3>(iterate_num));
     }
@@ -504,6 +614,17 @@ int pgdg_particle_count(pgdg_ctx* x, int64_t* n) {
   if (!x || !n) return PGDG_ERR_ARG;
   *n = x->c.np;
   return PGDG_OK;
+}
+
+int pgdg_profile_enable(pgdg_ctx* x, int on) {
+  if (!x) return PGDG_ERR_ARG;
+  x->c.prof.on = on != 0;
+  return PGDG_OK;
+}
+
+int pgdg_profile_read(pgdg_ctx* x, double* ms, int64_t* launches, int nslot) {
+  if (!x || !ms || !launches || nslot < 0) return PGDG_ERR_ARG;
+  return x->c.prof.read(ms, launches, nslot);
 }
 
 int pgdg_memory_bytes(pgdg_ctx* x, int64_t* bytes) {

@@ -4,6 +4,7 @@
 // Layout: SoA FP64 columns so every load and store of a wavefront is one
 // contiguous 512-byte segment; one face per lane, grid-stride loop.
 #include "hip_util.hpp"
+#include "dev_common.hpp"
 #include "riemann.hpp"
 
 namespace pgdg {
@@ -52,7 +53,49 @@ extern "C" int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, in
   return PGDG_OK;
 }
 
+namespace pgdg {
+__global__ void k_root_dim(const double* __restrict__ x, double* __restrict__ y, int64_t n, int dim) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = dim == 2 ? root_d<2>(x[i]) : root_d<3>(x[i]);
+}
+}  // namespace pgdg
+
+extern "C" int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream) {
+  if (n < 0 || (dim != 2 && dim != 3)) return PGDG_ERR_ARG;
+  if (n == 0) return PGDG_OK;
+  if (!x || !y) return PGDG_ERR_ARG;
+  hipLaunchKernelGGL(pgdg::k_root_dim, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     pgdg::as_stream(stream), x, y, n, dim);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
 extern "C" const char* pgdg_version(void) { return "pgdg-mi355x 0.1 gfx950"; }
+
+extern "C" int pgdg_malloc(void** ptr, int64_t bytes) {
+  if (!ptr || bytes < 0) return PGDG_ERR_ARG;
+  PGDG_HIP_TRY(hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 16));
+  return PGDG_OK;
+}
+
+extern "C" int pgdg_free(void* ptr) {
+  if (ptr) PGDG_HIP_TRY(hipFree(ptr));
+  return PGDG_OK;
+}
+
+extern "C" int pgdg_memcpy(void* dst, const void* src, int64_t bytes, int kind) {
+  if (bytes < 0 || kind < 0 || kind > 2) return PGDG_ERR_ARG;
+  if (bytes == 0) return PGDG_OK;
+  if (!dst || !src) return PGDG_ERR_ARG;
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  PGDG_HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, k));
+  return PGDG_OK;
+}
+
+extern "C" int pgdg_device_sync(void) {
+  PGDG_HIP_TRY(hipDeviceSynchronize());
+  return PGDG_OK;
+}
 
 extern "C" int pgdg_device_count(int* count) {
   if (!count) return PGDG_ERR_ARG;

@@ -28,9 +28,23 @@ int launch_in_ele(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, 
 template <int D>
 int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
                const DevScalars* sc, int md, hipStream_t st);
+// Neighbour data in cell-bin order (entry e of the bins): positions after
+// and before the stage update, volume, density, sound speed, phase (10 for
+// walls) and the particle reference; plus the bounding box of the stage-start
+// positions of every cell.
+struct BinData {
+  double *x, *xb;        // [3][ne]
+  double *vol, *rho, *c; // [ne]
+  int32_t *ph, *ref;     // [ne]
+  double *bmin, *bmax;   // [3][nc]
+  int64_t ne_cap;
+};
+
 template <int D>
-int launch_pst(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S, int64_t n,
-               const int* bin_off, const int* bin_ent, const DevScalars* sc, hipStream_t st);
+int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                      const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
+                      const DevScalars* sc, hipStream_t st);
+
 template <int D>
 int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,
                     int64_t n, bool cells, DevScalars* sc, double output_dt, hipStream_t st);

@@ -3,7 +3,7 @@
 // penalty, and the time-step reduction.
 //
 //   k_g2p       Particle_base::Update_state_onestep S3:particle_flow.cpp:413-477
-//   k_pst       Particle_base::Update_PST / PST / Bound_pen S3:particle_flow.cpp:485-622
+//   k_pst_binned Particle_base::Update_PST / PST / Bound_pen S3:particle_flow.cpp:485-622
 //               with the neighbour list of Level_info::Refresh_neighbor_info
 //               (status 2, S3:level_infor.cpp:153-195) generated in place
 //   k_timestep  MPM::Set_timestep S3:mpm.cpp:19-57, Particle_base::Set_timestep
@@ -136,32 +136,103 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
 }
 
 // ---------------------------------------------------------------------------
-// Update_PST + PST + Bound_pen (S3:particle_flow.cpp:485-622)
+// Update_PST + PST + Bound_pen (S3:particle_flow.cpp:485-622) over
+// bin-ordered neighbour data
 // ---------------------------------------------------------------------------
+// Gather the data PST reads from its neighbours into bin order.
 template <int D>
-__global__ void __launch_bounds__(128) k_pst(Geo g, PtlDev P, WallDev W, StageBufs S, int64_t n,
-                                             const int* __restrict__ bin_off,
-                                             const int* __restrict__ bin_ent,
-                                             const DevScalars* __restrict__ sc) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void k_bin_gather(PtlDev P, WallDev W, StageBufs S, BinData B, const int* __restrict__ bin_ent,
+                             int64_t ne) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int ref = bin_ent[e];
   const int64_t cap = P.cap;
+  B.ref[e] = ref;
+  if (ref_is_wall(ref)) {
+    const int w = ref_wall(ref);
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      double xa = W.x[a * W.n + w];
+      B.x[a * B.ne_cap + e] = xa;
+      B.xb[a * B.ne_cap + e] = xa;
+    }
+    B.vol[e] = W.vol[w];
+    B.rho[e] = 0.0;
+    B.c[e] = 0.0;
+    B.ph[e] = 10;
+  } else {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      B.x[a * B.ne_cap + e] = P.x[a * cap + ref];
+      B.xb[a * B.ne_cap + e] = S.xb[a * cap + ref];
+    }
+    B.vol[e] = P.vol[ref];
+    B.rho[e] = P.rho[ref];
+    B.c[e] = P.c[ref];
+    B.ph[e] = P.phase[ref];
+  }
+}
+
+// Bounding box of the stage-start positions of each cell's entries (empty
+// cells get an empty box).
+template <int D>
+__global__ void k_cell_bbox(Geo g, BinData B, const int* __restrict__ bin_off) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int e = bin_off[c]; e < bin_off[c + 1]; e++) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      double v = B.xb[a * B.ne_cap + e];
+      lo[a] = v < lo[a] ? v : lo[a];
+      hi[a] = v > hi[a] ? v : hi[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    B.bmin[a * g.nc + c] = lo[a];
+    B.bmax[a * g.nc + c] = hi[a];
+  }
+}
+
+// The list of the particle is the reference's status-2 list: cells in i, j,
+// k order, bin order within a cell, entries within h*CUT_OFF of the
+// stage-start position.  Threads run over bin
+// entries, so a wavefront holds particles of the same or adjacent cells and
+// reads the same candidate cells.  A cell is skipped when the distance from
+// the particle's stage-start position to the cell's box exceeds the list
+// radius by a relative margin far above rounding; a candidate's squared
+// distance is compared with the squared radius with the same margin, and
+// only candidates inside the margin band take the exact sqrt test.
+template <int D>
+__global__ void __launch_bounds__(256) k_pst_binned(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
+                                                    int64_t ne, const int* __restrict__ bin_off,
+                                                    const DevScalars* __restrict__ sc) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int i = B.ref[e];
+  if (i < 0) return;  // wall entry
+  const int64_t cap = P.cap;
+  const int64_t nb = B.ne_cap;
   const double dt = sc->dt;
   const double max_v = sc->max_v;
   double xi[3], xbi[3];
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    xi[a] = P.x[a * cap + i];
-    xbi[a] = S.xb[a * cap + i];
+    xi[a] = B.x[a * nb + e];
+    xbi[a] = B.xb[a * nb + e];
   }
   const double hi = P.h[i], hbi = S.hb[i];
-  const double rhoi = P.rho[i], ci = P.c[i];
-  const int phi = P.phase[i];
+  const double rhoi = B.rho[e], ci = B.c[e];
+  const int phi = B.ph[e];
   const double ca = max_v * 5.;
   const double P_b = rhoi * ca * ca;
   const double dxk = hi / kCellRatio * kCutOff;
   const double W_dp = kern_W<D>(dxk, hi);
   const double rlim = hbi * kCutOff;
+  const double r2 = rlim * rlim;
+  const double r2_in = r2 * (1.0 - 1e-12), r2_out = r2 * (1.0 + 1e-12);
+  const double tmp1_base = -1. * P_b * (1. / rhoi);
   double pst[3] = {0, 0, 0}, sharp[3] = {0, 0, 0};
   const int pos = cell_of<D>(g, xbi);
   int pi, pj, pk;
@@ -171,65 +242,60 @@ __global__ void __launch_bounds__(128) k_pst(Geo g, PtlDev P, WallDev W, StageBu
   for (int t = max(pi - r, 0); t <= min(pi + r, g.N[0] - 1); t++)
     for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
       for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
-        int cc = (t * g.N[1] + s) * g.N[2] + m;
-        int e0 = bin_off[cc], e1 = bin_off[cc + 1];
-        for (int e = e0; e < e1; e++) {
-          int ref = bin_ent[e];
-          if (ref == -1) continue;
-          bool wl = ref_is_wall(ref);
-          int w = wl ? ref_wall(ref) : 0;
-          double xe[3], xeb[3];
+        const int cc = (t * g.N[1] + s) * g.N[2] + m;
+        double bd = 0;
 #pragma unroll
-          for (int a = 0; a < D; a++) {
-            if (wl) {
-              xe[a] = W.x[a * W.n + w];
-              xeb[a] = xe[a];
-            } else {
-              xe[a] = P.x[a * cap + ref];
-              xeb[a] = S.xb[a * cap + ref];
-            }
-          }
+        for (int a = 0; a < D; a++) {
+          double lo = B.bmin[a * g.nc + cc], hb_ = B.bmax[a * g.nc + cc];
+          double d = xbi[a] < lo ? lo - xbi[a] : (xbi[a] > hb_ ? xbi[a] - hb_ : 0.0);
+          bd += d * d;
+        }
+        if (!(bd <= r2_out)) continue;  // empty cell or wholly outside
+        const int e1 = bin_off[cc + 1];
+        for (int f = bin_off[cc]; f < e1; f++) {
           double sqb = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) {
-            double d = xbi[a] - xeb[a];
+            double d = xbi[a] - B.xb[a * nb + f];
             sqb += d * d;
           }
-          if (!(sqrt(sqb) <= rlim)) continue;
+          if (sqb > r2_out) continue;
+          if (!(sqb <= r2_in) && !(sqrt(sqb) <= rlim)) continue;
           double dr[3];
 #pragma unroll
-          for (int a = 0; a < D; a++) dr[a] = xi[a] - xe[a];
+          for (int a = 0; a < D; a++) dr[a] = xi[a] - B.x[a * nb + f];
           double sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) sq += dr[a] * dr[a];
           double dist = sqrt(sq);
           if (dist != 0) {
-            double vol_j = wl ? W.vol[w] : P.vol[ref];
+            const int phj = B.ph[f];
+            const bool wl = phj == 10;
+            double vol_j = B.vol[f];
             double aw = kern_dW<D>(dist, hi);
             double Wij = kern_W<D>(dist, hi);
             double rr = Wij / W_dp;
-            double r2 = rr * rr;
-            double R = 0.2 * (r2 * r2);
-            double tmp1 = -1. * P_b * (1. / rhoi) * (1 + R);
+            double q2 = rr * rr;
+            double R = 0.2 * (q2 * q2);
+            double tmp1 = tmp1_base * (1 + R);
             if (!wl) {
-              double rhoj = P.rho[ref];
-              double tmp2 = 1. / (rhoi + rhoj);
 #pragma unroll
               for (int a = 0; a < D; a++) pst[a] += dr[a] * aw * tmp1 * vol_j;
-              if (P.phase[ref] != phi) {
-                double cj = P.c[ref];
+              if (phj != phi) {
+                double rhoj = B.rho[f], cj = B.c[f];
+                double tmp2 = 1. / (rhoi + rhoj);
                 double kk = -0.1 * tmp2 * (ci * ci * rhoi + cj * cj * rhoj);
 #pragma unroll
                 for (int a = 0; a < D; a++) sharp[a] += kk * (dr[a] * aw) * vol_j;
               }
             } else {
+              const int w = ref_wall(B.ref[f]);
 #pragma unroll
               for (int a = 0; a < D; a++) pst[a] += W.norm[a * W.n + w] * Wij * tmp1 * vol_j;
             }
           }
         }
       }
-  // Bound_pen over the boundary faces of the particle's cell
   double pen[3] = {0, 0, 0};
   {
     int c = S.in_ele[i];
@@ -292,6 +358,20 @@ __global__ void __launch_bounds__(128) k_pst(Geo g, PtlDev P, WallDev W, StageBu
     double dv = frozen ? 0.0 : (dx[a] + dx1[a] + dx2[a]) / dt;
     P.tv[a * cap + i] = P.v[a * cap + i] + dv;
   }
+}
+
+template <int D>
+int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                      const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
+                      const DevScalars* sc, hipStream_t st) {
+  if (n_ent <= 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S,
+                     B, bin_ent, n_ent);
+  hipLaunchKernelGGL(k_cell_bbox<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, B, bin_off);
+  hipLaunchKernelGGL(k_pst_binned<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, g, P, W,
+                     S, B, n_ent, bin_off, sc);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -400,16 +480,6 @@ int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs&
 }
 
 template <int D>
-int launch_pst(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S, int64_t n,
-               const int* bin_off, const int* bin_ent, const DevScalars* sc, hipStream_t st) {
-  if (n <= 0) return PGDG_OK;
-  hipLaunchKernelGGL(k_pst<D>, dim3(grid_for(n, 128, 1LL << 30)), dim3(128), 0, st, g, P, W, S, n,
-                     bin_off, bin_ent, sc);
-  PGDG_HIP_TRY(hipGetLastError());
-  return PGDG_OK;
-}
-
-template <int D>
 int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,
                     int64_t n, bool cells, DevScalars* sc, double output_dt, hipStream_t st) {
   hipLaunchKernelGGL(k_ts_init, dim3(1), dim3(64), 0, st, sc, output_dt);
@@ -432,8 +502,9 @@ int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32
   template int launch_in_ele<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, hipStream_t); \
   template int launch_g2p<D>(const Geo&, const PtlDev&, const CellDev&, const StageBufs&, int64_t, \
                              const DevScalars*, int, hipStream_t);                                \
-  template int launch_pst<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&, int64_t, \
-                             const int*, const int*, const DevScalars*, hipStream_t);             \
+  template int launch_pst_binned<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \
+                                    const BinData&, int64_t, const int*, const int*,              \
+                                    const DevScalars*, hipStream_t);                              \
   template int launch_timestep<D>(const Geo&, const PtlDev&, const CellDev&, const int32_t*,       \
                                   int64_t, bool, DevScalars*, double, hipStream_t);
 PGDG_INST(2)

@@ -11,6 +11,10 @@ FN = {"set_timestep": 0, "used_state_update": 1, "bin": 2, "split": 3, "merge": 
       "update_particle": 11, "run_simulation": 12}
 
 
+PROFILE_SLOTS = ["set_timestep", "used_state_update", "bin", "split", "merge", "membership",
+                 "stage_save", "p2g", "limiter", "rhs", "update_grid", "g2p", "pst", "lists"]
+
+
 class GpuSim:
     """One case resident on the GPU (a pgdg_ctx)."""
 
@@ -47,6 +51,20 @@ class GpuSim:
         s = C.c_void_p()
         check(self.L.pgdg_get_stream(self.h, C.byref(s)))
         return s.value or 0
+
+    def profile(self, on: bool = True):
+        check(self.L.pgdg_profile_enable(self.h, 1 if on else 0), "pgdg_profile_enable")
+
+    def profile_read(self) -> dict:
+        ms = (C.c_double * len(PROFILE_SLOTS))()
+        cnt = (C.c_int64 * len(PROFILE_SLOTS))()
+        check(self.L.pgdg_profile_read(self.h, ms, cnt, len(PROFILE_SLOTS)), "pgdg_profile_read")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(PROFILE_SLOTS)}
+
+    def memory_bytes(self) -> int:
+        b = C.c_int64()
+        check(self.L.pgdg_memory_bytes(self.h, C.byref(b)))
+        return b.value
 
     def count(self) -> int:
         n = C.c_int64()

@@ -14,7 +14,8 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def cuda():
-    import torch
-    if not torch.cuda.is_available():
+    """A HIP device seen through the library's own runtime (no torch)."""
+    import pgdg_p2_amd as P
+    if P.device_count() < 1:
         pytest.skip("no HIP device")
-    return torch.device("cuda:0")
+    return 0

@@ -106,6 +106,49 @@ def mgfm_bench(n=10_000_000):
     return res
 
 
+def mls_bench(n=128, reps=5):
+    """MLS fit of 6 fields on n^3 cells (Poisson(8) particles per cell,
+    inputs resident on the device): FP64 and mixed forms, with the mixed
+    form's per-field deviation from the FP64 fit at the cell centres."""
+    import numpy as np
+    import pgdg_p2_amd as P
+    from pgdg_p2_amd import mls, workloads
+    t0 = time.perf_counter()
+    d = workloads.mls_particles(n)
+    npart = int(d["x"].shape[1])
+    f = mls.MlsFit(d)
+    print(f"mls setup {time.perf_counter() - t0:.1f}s: {n}^3 cells, {npart} particles",
+          file=sys.stderr, flush=True)
+    del d
+    nc = n ** 3
+    res = {"cells": nc, "particles": npart, "fields": f.nf, "modes": 10}
+    tm = P.EventTimer()
+    coefs = {}
+    for name, prec, bpp in (("fp64", mls.PREC_FP64, 80), ("mixed", mls.PREC_MIXED, 40)):
+        f.launch(prec)
+        tm.start()
+        for _ in range(reps):
+            f.launch(prec)
+        tm.stop()
+        ms = tm.ms() / reps
+        coefs[name] = f.coef.numpy()
+        alg = npart * bpp + nc * (8 + 4 + f.nf * 10 * 8)
+        res[name] = {"ms": ms, "cells_per_s": nc / (ms * 1e-3),
+                     "particles_per_s": npart / (ms * 1e-3),
+                     "alg_GBs": alg / (ms * 1e-3) / 1e9}
+    tm.close()
+    v64 = mls.center_values(coefs["fp64"])
+    v32 = mls.center_values(coefs["mixed"])
+    res["mixed_vs_fp64"] = {
+        "centre_L1_rel": [float(np.abs(v32[i] - v64[i]).mean() / np.abs(v64[i]).mean())
+                          for i in range(f.nf)],
+        "centre_Linf_rel": [float(np.abs(v32[i] - v64[i]).max() / np.abs(v64[i]).max())
+                            for i in range(f.nf)]}
+    res["speedup_mixed"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+    f.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +157,9 @@ def main():
     ap.add_argument("--size", default="full", choices=list(SIZES))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mgfm", action="store_true")
+    ap.add_argument("--no-mls", action="store_true")
+    ap.add_argument("--mls-n", type=int, default=128)
+    ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -121,7 +167,10 @@ def main():
               "slab decomposition is a design, not a build", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.only_mls:
+        print(json.dumps({"mls": mls_bench(args.mls_n)}))
+        return
+
     from pgdg_p2_amd import cases
     from pgdg_p2_amd.step import GpuSim
 
@@ -189,6 +238,8 @@ def main():
     }
     if not args.no_mgfm:
         line["mgfm"] = mgfm_bench()
+    if not args.no_mls:
+        line["mls"] = mls_bench(args.mls_n)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
         line["cpu_baseline"]["host_cpus"] = os.cpu_count()

@@ -51,6 +51,13 @@ int pgdg_free(void* ptr);
 int pgdg_memcpy(void* dst, const void* src, int64_t bytes, int kind);
 int pgdg_device_sync(void);
 
+/* Stream-ordered timing (HIP events) for callers without a GPU framework:
+ * record on the stream the work is launched on; elapsed waits for `stop`. */
+int pgdg_event_create(void** ev);
+int pgdg_event_record(void* ev, void* stream);
+int pgdg_event_elapsed_ms(void* start, void* stop, float* ms);
+int pgdg_event_destroy(void* ev);
+
 /* ======================================================================== *
  *  MGFM two-medium Riemann solve (A9) and stiffened-gas EOS (A10)          *
  * ======================================================================== */
@@ -102,6 +109,43 @@ int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, int64_t n,
  * pow(vol, 1./DIM) of S3:mpm.cpp:1089 / S3:particle_flow.cpp:473
  * (dim 2: sqrt; dim 3: correctly rounded x^(1./3.)).  x, y device.       */
 int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream);
+
+/* ======================================================================== *
+ *  MLS particle-to-grid moment assembly with a per-cell least-squares      *
+ *  solve (extension of MPM::MLS_P2G, S3:mpm.cpp:274-382)                   *
+ * ======================================================================== */
+
+/* Precision of the assembly.  The per-cell factorisation and solve are
+ * FP64 in both.
+ *   PGDG_PREC_FP64  : FP64 inputs (absolute positions), FP64 sums;
+ *   PGDG_PREC_MIXED : FP32 inputs (cell-local coordinates) and FP32 sums. */
+#define PGDG_PREC_FP64  0
+#define PGDG_PREC_MIXED 1
+
+typedef struct pgdg_mls_problem {
+  int     n[3];          /* cells per axis on [0, n*dx]^3, index (i*n1+j)*n2+k */
+  double  dx;            /* cell size                                        */
+  double  cut;           /* particle cut-off distance (reference 1.5*p_sz)   */
+  double  h;             /* Wendland kernel width (reference 1.6*p_g)        */
+  int     nfields;       /* 1..6                                             */
+  int     order;         /* 2: P2 (10 zero-mean modes), 1: P1 (4), 0: P0     */
+  int64_t np;            /* particles, sorted by cell                        */
+  const int64_t* off;    /* device [nc+1] first particle of each cell        */
+  const double*  x;      /* device [3][np] positions        (FP64 path)      */
+  const double*  vol;    /* device [np] volumes             (FP64 path)      */
+  const double*  val;    /* device [nf][np] field values    (FP64 path)      */
+  const float*   u;      /* device [3][np] cell-local coords in [0,1) (mixed)*/
+  const float*   volf;   /* device [np]                     (mixed)          */
+  const float*   valf;   /* device [nf][np]                 (mixed)          */
+} pgdg_mls_problem;
+
+/* Per-cell weighted least-squares fit of the fields of the particles of the
+ * 3x3x3 neighbour cells within `cut` (weights W(r, h) * vol, the reference's
+ * MLS_P2G weights) in the zero-mean P2 basis of the cell-scaled coordinate.
+ * coef: device [nf][10][nc]; order_used: device [nc] (order actually
+ * solved: a singular moment matrix falls back to P1, then P0) or NULL.    */
+int pgdg_mls_ls_fit(const pgdg_mls_problem* p, int precision, double* coef, int32_t* order_used,
+                    void* stream);
 
 /* ======================================================================== *
  *  PG-DG time step: case description and host state layout                *

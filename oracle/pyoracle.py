@@ -58,3 +58,26 @@ def root_dim(x: np.ndarray, dim: int) -> np.ndarray:
     y = np.empty_like(x)
     lib().oracle_root_dim(_ptr(x), _ptr(y), x.size, dim)
     return y
+
+
+def mls_ls(n, dx, off, x, vol, val, order=2, cut=None, h=None):
+    """FP64 MLS least-squares fit per cell: coef [nf][10][nc], order used [nc]."""
+    n3 = (C.c_int * 3)(*([n] * 3 if np.isscalar(n) else n))
+    nc = n3[0] * n3[1] * n3[2]
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    vol = np.ascontiguousarray(vol, dtype=np.float64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    nf = val.shape[0]
+    npart = x.shape[1]
+    cut = 1.5 * dx if cut is None else cut
+    h = 1.6 * dx if h is None else h
+    coef = np.zeros((nf, 10, nc))
+    ordv = np.zeros(nc, dtype=np.int32)
+    L = lib()
+    L.oracle_mls_ls.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
+                                C.c_void_p, C.c_void_p]
+    L.oracle_mls_ls(n3, dx, _ptr(off), _ptr(x), _ptr(vol), _ptr(val), npart, nf, order, cut, h,
+                    _ptr(coef), _ptr(ordv))
+    return coef, ordv

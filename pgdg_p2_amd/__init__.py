@@ -65,6 +65,10 @@ def lib() -> C.CDLL:
         L.pgdg_free.argtypes = [C.c_void_p]
         L.pgdg_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
         L.pgdg_root_dim.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]
+        L.pgdg_event_create.argtypes = [C.POINTER(C.c_void_p)]
+        L.pgdg_event_record.argtypes = [C.c_void_p, C.c_void_p]
+        L.pgdg_event_elapsed_ms.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
+        L.pgdg_event_destroy.argtypes = [C.c_void_p]
         from . import _bind  # noqa: F401  (declares the step API signatures)
         _bind.declare(L)
         _lib = L
@@ -181,6 +185,34 @@ def root_dim_np(x, dim):
     check(lib().pgdg_root_dim(dx.ptr, dy.ptr, x.size, dim, None), "pgdg_root_dim")
     device_sync()
     return dy.numpy()
+
+
+class EventTimer:
+    """Times work enqueued on `stream` (None: the null stream) between
+    start() and stop() with HIP events; ms() waits for the stop event."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        check(lib().pgdg_event_create(C.byref(self.a)), "pgdg_event_create")
+        check(lib().pgdg_event_create(C.byref(self.b)), "pgdg_event_create")
+
+    def start(self):
+        check(lib().pgdg_event_record(self.a, self.stream), "pgdg_event_record")
+
+    def stop(self):
+        check(lib().pgdg_event_record(self.b, self.stream), "pgdg_event_record")
+
+    def ms(self) -> float:
+        v = C.c_float()
+        check(lib().pgdg_event_elapsed_ms(self.a, self.b, C.byref(v)), "pgdg_event_elapsed_ms")
+        return float(v.value)
+
+    def close(self):
+        for e in (self.a, self.b):
+            if e.value:
+                lib().pgdg_event_destroy(e)
+        self.a, self.b = C.c_void_p(), C.c_void_p()
 
 
 def device_count() -> int:

@@ -97,6 +97,32 @@ extern "C" int pgdg_device_sync(void) {
   return PGDG_OK;
 }
 
+extern "C" int pgdg_event_create(void** ev) {
+  if (!ev) return PGDG_ERR_ARG;
+  hipEvent_t e = nullptr;
+  PGDG_HIP_TRY(hipEventCreate(&e));
+  *ev = e;
+  return PGDG_OK;
+}
+
+extern "C" int pgdg_event_record(void* ev, void* stream) {
+  if (!ev) return PGDG_ERR_ARG;
+  PGDG_HIP_TRY(hipEventRecord((hipEvent_t)ev, pgdg::as_stream(stream)));
+  return PGDG_OK;
+}
+
+extern "C" int pgdg_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (!start || !stop || !ms) return PGDG_ERR_ARG;
+  PGDG_HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+  PGDG_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return PGDG_OK;
+}
+
+extern "C" int pgdg_event_destroy(void* ev) {
+  if (ev) PGDG_HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+  return PGDG_OK;
+}
+
 extern "C" int pgdg_device_count(int* count) {
   if (!count) return PGDG_ERR_ARG;
   int c = 0;

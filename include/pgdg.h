@@ -233,6 +233,73 @@ typedef struct pgdg_scalars {
   int64_t n_split, n_inlet, n_outlet, n_merge;  /* event counts, last step */
 } pgdg_scalars;
 
+/* ======================================================================== *
+ *  PG-DG time step: context and entry points                               *
+ *  One context owns one case's state on the current GPU; every call is     *
+ *  ordered on the context's stream.                                        *
+ * ======================================================================== */
+typedef struct pgdg_ctx pgdg_ctx;
+
+/* Allocates the device state for `c` with room for particle_cap fluid
+ * particles and n_wall wall particles (replaces the state ownership of
+ * MPM::MPM / MPM::Initialize_case, S3:mpm.h:160-194).  Supported: order 1,
+ * rk_stages 2, dim 2 (PGDG_SEM_T2) or 3 (PGDG_SEM_S3), 2..3 phases.       */
+int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_ctx** out);
+int pgdg_destroy(pgdg_ctx* ctx);
+
+/* Host <-> device copies of the whole state (the reference's in-memory
+ * particle_fluid / particle_bound / cell arrays, S3:mpm.h:160-194).       */
+int pgdg_upload_state(pgdg_ctx* ctx, const pgdg_particles* p, const pgdg_walls* w,
+                      const pgdg_cells* c, const pgdg_scalars* s);
+int pgdg_download_state(pgdg_ctx* ctx, pgdg_particles* p, pgdg_cells* c, pgdg_scalars* s);
+
+/* nsteps iterations of the Solver::MPM_Solver loop body (S3:solver.cpp:33-51):
+ * Set_timestep, output-instant truncation of dt, MPM_run_simulation.     */
+int pgdg_step(pgdg_ctx* ctx, int nsteps);
+
+/* Reference-named stages; each replaces the MPM member of the same name.  */
+int pgdg_set_timestep(pgdg_ctx* ctx);              /* MPM::Set_timestep, S3:mpm.cpp:19          */
+int pgdg_used_state_update(pgdg_ctx* ctx);         /* MPM::Used_state_update, S3:mpm.cpp:184    */
+int pgdg_map_particles_to_cells(pgdg_ctx* ctx);    /* Reset_cell_list_info + Map_the_particle_to_tree,
+                                                      S3:mpm.cpp:65-96                           */
+int pgdg_split_particle_fast(pgdg_ctx* ctx);       /* MPM::Split_particle_fast, S3:mpm.cpp:1036 */
+int pgdg_merge_particle_fast(pgdg_ctx* ctx);       /* MPM::Merge_particle_fast, S3:mpm.cpp:1536 */
+int pgdg_refresh_cell_membership(pgdg_ctx* ctx);   /* Refresh_neighbor_info(4), S3:mpm.cpp:137-161 */
+int pgdg_refresh_neighbor_info(pgdg_ctx* ctx, int stage); /* Refresh_neighbor_info(2),
+                                                      S3:level_infor.cpp:153                     */
+int pgdg_mls_p2g(pgdg_ctx* ctx);                   /* MPM::MLS_P2G, S3:mpm.cpp:274             */
+int pgdg_calculate_limiter(pgdg_ctx* ctx);         /* MPM::Calculate_limiter, S3:mpm.cpp:392   */
+int pgdg_grid_acceleration(pgdg_ctx* ctx);         /* MPM::Grid_acceleration, S3:mpm.cpp:747   */
+int pgdg_update_grid(pgdg_ctx* ctx, int md);       /* MPM::Update_grid, S3:mpm.cpp:768         */
+int pgdg_update_particle(pgdg_ctx* ctx, int md);   /* MPM::Update_particle, S3:mpm.cpp:839     */
+int pgdg_run_simulation(pgdg_ctx* ctx, int64_t n); /* MPM::MPM_run_simulation, S3:mpm_ini.cpp:682 */
+
+/* Stage by number (PGDG_FN_*), as used by the entries above.              */
+#define PGDG_FN_SET_TIMESTEP    0
+#define PGDG_FN_USED            1
+#define PGDG_FN_BIN             2
+#define PGDG_FN_SPLIT           3
+#define PGDG_FN_MERGE           4
+#define PGDG_FN_MEMBERSHIP      5
+#define PGDG_FN_NEIGHBORS       6
+#define PGDG_FN_P2G             7
+#define PGDG_FN_LIMITER         8
+#define PGDG_FN_RHS             9
+#define PGDG_FN_UPDATE_GRID    10
+#define PGDG_FN_UPDATE_PARTICLE 11
+#define PGDG_FN_RUN            12
+int pgdg_stage(pgdg_ctx* ctx, int fn, int arg);
+
+/* Utilities: wait for the context's stream, its HIP stream handle, the
+ * live fluid particle count, device bytes held, and per-stage HIP-event
+ * timing (slot order: the profile slots of pgdg_p2_amd.step).             */
+int pgdg_sync(pgdg_ctx* ctx);
+int pgdg_get_stream(pgdg_ctx* ctx, void** stream);
+int pgdg_particle_count(pgdg_ctx* ctx, int64_t* n);
+int pgdg_memory_bytes(pgdg_ctx* ctx, int64_t* bytes);
+int pgdg_profile_enable(pgdg_ctx* ctx, int on);
+int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
+
 #ifdef __cplusplus
 }
 #endif

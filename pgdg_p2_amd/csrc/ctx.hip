@@ -410,6 +410,13 @@ enum {
   FN_NEIGHBORS = 6, FN_P2G = 7, FN_LIMITER = 8, FN_RHS = 9, FN_UPDATE_GRID = 10,
   FN_UPDATE_PARTICLE = 11, FN_RUN = 12
 };
+static_assert(FN_SET_TIMESTEP == PGDG_FN_SET_TIMESTEP && FN_USED == PGDG_FN_USED &&
+                  FN_BIN == PGDG_FN_BIN && FN_SPLIT == PGDG_FN_SPLIT && FN_MERGE == PGDG_FN_MERGE &&
+                  FN_MEMBERSHIP == PGDG_FN_MEMBERSHIP && FN_NEIGHBORS == PGDG_FN_NEIGHBORS &&
+                  FN_P2G == PGDG_FN_P2G && FN_LIMITER == PGDG_FN_LIMITER && FN_RHS == PGDG_FN_RHS &&
+                  FN_UPDATE_GRID == PGDG_FN_UPDATE_GRID &&
+                  FN_UPDATE_PARTICLE == PGDG_FN_UPDATE_PARTICLE && FN_RUN == PGDG_FN_RUN,
+              "stage numbers must match include/pgdg.h");
 
 static int prof_slot(int fn) {
   switch (fn) {

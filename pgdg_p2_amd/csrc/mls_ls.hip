@@ -40,20 +40,32 @@ __device__ inline void basis10(T x, T y, T z, T* phi) {
   phi[9] = y * z;
 }
 
+// acc[tri(a, b)] (a <= b) += (w phi_b) phi_a and acc[kNA + 6a + f] +=
+// (w phi_a) v_f.  FP64: separate multiply and add, in the oracle's operand
+// order, so the FP64 form reproduces it bit for bit; FP32: fused.
 template <typename T>
 __device__ inline void accumulate(T* acc, T w, const T* phi, const T* v, int nf) {
+  constexpr bool F32 = std::is_same<T, float>::value;
   T wphi[10];
 #pragma unroll
   for (int a = 0; a < 10; a++) wphi[a] = w * phi[a];
 #pragma unroll
   for (int a = 0; a < 10; a++)
 #pragma unroll
-    for (int b = a; b < 10; b++) acc[tri(a, b)] = fma(wphi[a], phi[b], acc[tri(a, b)]);
+    for (int b = a; b < 10; b++) {
+      T& r = acc[tri(a, b)];
+      if constexpr (F32) r = fmaf(wphi[b], phi[a], r);
+      else r = r + wphi[b] * phi[a];
+    }
 #pragma unroll
   for (int f = 0; f < 6; f++)
     if (f < nf)
 #pragma unroll
-      for (int a = 0; a < 10; a++) acc[kNA + a * 6 + f] = fma(wphi[a], v[f], acc[kNA + a * 6 + f]);
+      for (int a = 0; a < 10; a++) {
+        T& r = acc[kNA + a * 6 + f];
+        if constexpr (F32) r = fmaf(wphi[a], v[f], r);
+        else r = r + wphi[a] * v[f];
+      }
 }
 
 template <bool MIXED>
@@ -71,11 +83,11 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
 #pragma unroll
   for (int q = 0; q < kNR; q++) acc[q] = T(0);
 
-  const double inv_dx = 1.0 / p.dx, inv_h = 1.0 / p.h;
+  const double inv_dx = 1.0 / p.dx;
   const double Ck = 1.0 / kPiRef / ((p.h * p.h) * p.h) * 21. / 16.;
   const double xc0 = p.dx * (ci + 0.5), xc1 = p.dx * (cj + 0.5), xc2 = p.dx * (ck + 0.5);
   const float cut_xi2 = (float)((p.cut * inv_dx) * (p.cut * inv_dx));
-  const float dx_h = (float)(p.dx * inv_h);
+  const float dx_h = (float)(p.dx / p.h);
 
   // flattened walk over the candidates of the 27 neighbour cells
   int nb = -1, di = 0, dj = 0, dk = 0;
@@ -112,7 +124,7 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
       const double d0 = p.x[k] - xc0, d1 = p.x[p.np + k] - xc1, d2 = p.x[2 * p.np + k] - xc2;
       const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
       if (r < p.cut) {
-        const double sk = r * inv_h;
+        const double sk = r / p.h;
         double w = 0;
         if (sk < 2) {
           const double qq = 1. - 0.5 * sk;
@@ -120,7 +132,7 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
         }
         w *= p.vol[k];
         double phi[10], v[6];
-        basis10(d0 * inv_dx, d1 * inv_dx, d2 * inv_dx, phi);
+        basis10(d0 / p.dx, d1 / p.dx, d2 / p.dx, phi);
 #pragma unroll
         for (int f = 0; f < 6; f++) v[f] = f < nf ? p.val[f * p.np + k] : 0.0;
         accumulate(acc, w, phi, v, nf);
@@ -173,7 +185,8 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
     for (int i = 9; i >= 0; i--) {
       double s = b[i];
 #pragma unroll
-      for (int q = i + 1; q < 10; q++) s -= A[tri(i, q)] * b[q];
+      for (int q = i + 1; q < 10; q++)
+        if (q < m) s -= A[tri(i, q)] * b[q];
       b[i] = i < m ? s / A[tri(i, i)] : 0.0;
     }
 #pragma unroll

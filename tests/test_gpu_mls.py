@@ -1,7 +1,10 @@
 """GPU MLS fit (pgdg_mls_ls_fit) against the FP64 oracle.
 
-FP64 form: every coefficient within 1e-10 of the oracle, relative to the
-field's largest coefficient of that mode, and identical solved orders.
+FP64 form: identical solved orders and every coefficient within 1e-10 of
+the oracle, relative to the field's largest coefficient of that mode (the
+kernel performs the oracle's operations in the oracle's order with correctly
+rounded division and square root, so it is in fact bit-identical, which is
+asserted too).
 Mixed form (FP32 assembly, FP64 solve): per-field L1 and Linf deviation
 from the FP64 fit stated and bounded, and the L1 error of the fitted cell
 centre values against the exact field equal to the FP64 fit's within 1%.
@@ -45,6 +48,7 @@ def test_mls_fp64_matches_oracle(cuda, order):
     np.testing.assert_array_equal(ordg, ordr)
     assert (ordr == order).all()
     assert _rel_err(coef, ref) <= FP64_RTOL
+    np.testing.assert_array_equal(coef, ref)
 
 
 @pytest.mark.gpu
@@ -61,6 +65,7 @@ def test_mls_fallback_orders(cuda):
     np.testing.assert_array_equal(ordg, ordr)
     assert len(set(ordr.tolist())) >= 3, set(ordr.tolist())
     assert _rel_err(coef, ref) <= FP64_RTOL
+    np.testing.assert_array_equal(coef, ref)
 
 
 @pytest.mark.gpu
@@ -101,3 +106,23 @@ def test_mls_empty_and_single_cell(cuda):
     assert o.tolist() == [0]
     np.testing.assert_allclose(coef[:, 0, 0], np.arange(1.0, 7.0), rtol=1e-15)
     assert not coef[:, 1:].any()
+
+
+@pytest.mark.gpu
+def test_mls_empty_neighbourhood(cuda):
+    # 4x1x1 cells, one particle near the left face of cell 0: cells 0 and 1
+    # fall back to P0, cells 2 and 3 see no particle (order -1, zero fit).
+    from pgdg_p2_amd import mls
+    x = np.array([[0.1], [0.5], [0.5]])
+    d = {"n": (4, 1, 1), "dx": 1.0, "off": np.array([0, 1, 1, 1, 1], np.int64), "x": x,
+         "u": x.copy(), "vol": np.ones(1), "val": np.full((2, 1), 3.0)}
+    ref, ordr = pyoracle.mls_ls((4, 1, 1), 1.0, d["off"], x, d["vol"], d["val"])
+    f = mls.MlsFit(d)
+    coef, o = f.run(mls.PREC_FP64)
+    cm, om = f.run(mls.PREC_MIXED)
+    f.close()
+    assert ordr.tolist() == [0, 0, -1, -1]
+    np.testing.assert_array_equal(o, ordr)
+    np.testing.assert_array_equal(om, ordr)
+    np.testing.assert_array_equal(coef, ref)
+    np.testing.assert_allclose(cm, ref, rtol=1e-6, atol=0)

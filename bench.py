@@ -161,12 +161,28 @@ def main():
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one plain copy kernel over this many MiB (byte-counter "
+                         "calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "slab decomposition is a design, not a build", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_mib > 0:
+        import ctypes as C
+        import numpy as np
+        import pgdg_p2_amd as P
+        nbytes = args.calib_copy_mib << 20
+        a = P.DeviceArray((nbytes,), np.uint8)
+        b = P.DeviceArray((nbytes,), np.uint8)
+        P.check(P.lib().pgdg_copy_kernel(b.ptr, a.ptr, nbytes, None), "pgdg_copy_kernel")
+        P.device_sync()
+        a.free()
+        b.free()
+        print(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr,
+              flush=True)
     if args.only_mls:
         print(json.dumps({"mls": mls_bench(args.mls_n)}))
         return

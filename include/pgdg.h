@@ -51,6 +51,11 @@ int pgdg_free(void* ptr);
 int pgdg_memcpy(void* dst, const void* src, int64_t bytes, int kind);
 int pgdg_device_sync(void);
 
+/* Streaming device-to-device copy by a plain kernel (16-byte aligned
+ * buffers, bytes a multiple of 16): known HBM traffic for calibrating
+ * hardware byte counters against algorithmic bytes.                       */
+int pgdg_copy_kernel(void* dst, const void* src, int64_t bytes, void* stream);
+
 /* Stream-ordered timing (HIP events) for callers without a GPU framework:
  * record on the stream the work is launched on; elapsed waits for `stop`. */
 int pgdg_event_create(void** ev);

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         L.pgdg_free.argtypes = [C.c_void_p]
         L.pgdg_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
         L.pgdg_root_dim.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p]
+        L.pgdg_copy_kernel.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.pgdg_event_create.argtypes = [C.POINTER(C.c_void_p)]
         L.pgdg_event_record.argtypes = [C.c_void_p, C.c_void_p]
         L.pgdg_event_elapsed_ms.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]

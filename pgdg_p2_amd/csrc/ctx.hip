@@ -109,6 +109,7 @@ struct Ctx {
   double* s_slab = nullptr;
   BinData B{};
   double* b_slab = nullptr;
+  P2gBuf PB{};
   int32_t* bi_slab = nullptr;
   Topo T;
   DevScalars* sc = nullptr;
@@ -236,12 +237,17 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   B.bmax = B.bmin + 3 * (size_t)nc;
   B.ph = bi_slab;
   B.ref = bi_slab + ne;
+  const int p2g_rows_n = kP2gLight + (D == 3 ? p2g_heavy<3>() : p2g_heavy<2>());
+  PB.ne = ne;
+  PGDG_HIP_TRY(hipMalloc((void**)&PB.light, sizeof(double) * p2g_rows_n * (size_t)ne));
+  PB.heavy = PB.light + kP2gLight * (size_t)ne;
   PGDG_HIP_TRY(hipMalloc((void**)&sc, sizeof(DevScalars)));
   PGDG_HIP_TRY(hipMemset(sc, 0, sizeof(DevScalars)));
   PGDG_TRY(T.init(cap, nwall, nc, st));
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall +
-                            9 * (size_t)ne + 6 * (size_t)nc) + T.bytes_total;
+                            9 * (size_t)ne + 6 * (size_t)nc + p2g_rows_n * (size_t)ne) +
+          T.bytes_total;
   return PGDG_OK;
 }
 
@@ -249,7 +255,7 @@ void Ctx::release() {
   if (st) (void)hipStreamSynchronize(st);
   PA.release();
   T.release();
-  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab};
+  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (own_stream && st) (void)hipStreamDestroy(st);
@@ -478,7 +484,7 @@ int Ctx::stage_impl(int fn, int arg) {
     case FN_NEIGHBORS:
       return launch_stage_save<D_>(g, P, S, np, arg, st);
     case FN_P2G:
-      return launch_cells<D_>(CELL_P2G, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
+      return launch_p2g<D_>(g, C, P, T.bin_off, T.bin_ent, np + W.n, used, PB, st);
     case FN_LIMITER:
       return launch_cells<D_>(CELL_LIMITER, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
     case FN_RHS:

@@ -34,4 +34,25 @@ inline unsigned grid_for(long long n, int block, long long cap = 1 << 20) {
   return (unsigned)g;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by workgroup id, and
+// each XCD has its own L2.  xcd_block() renumbers them so that each XCD
+// walks runs of kXcdChunk consecutive logical workgroups (neighbouring
+// workgroups read overlapping neighbour data and then share one L2), with
+// the runs dealt round-robin so that spatially uneven work stays balanced
+// across XCDs.  The grid must come from grid_xcd (a multiple of
+// 8 * kXcdChunk workgroups); logical ids past the work exit.
+constexpr int kXcds = 8;
+constexpr int kXcdChunk = 16;
+inline unsigned grid_xcd(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  const long long q = (long long)kXcds * kXcdChunk;
+  g = (g + q - 1) / q * q;
+  return (unsigned)g;
+}
+__device__ inline unsigned xcd_block() {
+  const unsigned x = blockIdx.x % kXcds, s = blockIdx.x / kXcds;
+  return ((s / kXcdChunk) * kXcds + x) * kXcdChunk + s % kXcdChunk;
+}
+
 }  // namespace pgdg

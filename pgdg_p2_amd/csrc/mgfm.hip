@@ -97,6 +97,25 @@ extern "C" int pgdg_device_sync(void) {
   return PGDG_OK;
 }
 
+// Plain streaming copy (16-byte vector loads and stores, grid-stride): a
+// kernel of known HBM traffic, for calibrating hardware byte counters.
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" int pgdg_copy_kernel(void* dst, const void* src, int64_t bytes, void* stream) {
+  if (!dst || !src || bytes < 0 || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return PGDG_ERR_ARG;
+  const int64_t n16 = bytes / 16;
+  if (n16 == 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_copy16, dim3(pgdg::grid_for(n16, 256, 8192)), dim3(256), 0,
+                     pgdg::as_stream(stream), (const uint4*)src, (uint4*)dst, n16);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
 extern "C" int pgdg_event_create(void** ev) {
   if (!ev) return PGDG_ERR_ARG;
   hipEvent_t e = nullptr;

@@ -31,19 +31,57 @@ __device__ inline void cell_center(const Geo& g, int i, int j, int k, double* x)
 // MLS P2G (S3:mpm.cpp:274-382)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, const int* __restrict__ bin_off,
-                                             const int* __restrict__ bin_ent,
+__global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent,
+                             const int* __restrict__ bin_total, P2gBuf pb) {
+  constexpr int U = D + 2, M = D + 1, B = M - 1, H = p2g_heavy<D>();
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pb.ne || e >= *bin_total) return;
+  const int64_t cap = P.cap;
+  double* l = pb.light + e * kP2gLight;
+  const int p = bin_ent[e];
+  if (p < 0) {
+    *reinterpret_cast<double2*>(l + 4) = make_double2(-1.0, 0.0);
+    return;
+  }
+  double r[kP2gLight] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) r[a] = P.x[a * cap + p];
+  r[3] = P.vol[p];
+  r[4] = (double)(P.phase[p] - 1);
+#pragma unroll
+  for (int q = 0; q < kP2gLight; q += 2) *reinterpret_cast<double2*>(l + q) = make_double2(r[q], r[q + 1]);
+  double h[H];
+  h[0] = P.rho[p];
+#pragma unroll
+  for (int a = 0; a < D; a++) h[1 + a] = P.rhov[a * cap + p];
+  h[U - 1] = P.E[p];
+#pragma unroll
+  for (int u = 0; u < U; u++)
+#pragma unroll
+    for (int b = 0; b < B; b++) h[U + u * B + b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
+  if (H > U * M) h[H - 1] = 0.0;
+  double* hp = pb.heavy + e * H;
+#pragma unroll
+  for (int q = 0; q < H; q += 2) *reinterpret_cast<double2*>(hp + q) = make_double2(h[q], h[q + 1]);
+}
+
+// One lane per cell, cells taken in XCD-local runs (xcd_block).  The first
+// walk over the 3^D neighbour bins reads only the light records and forms
+// the phase masses; a second walk, in the same order and only for cells
+// where a phase appears (ph_state 0 -> 1), forms the weighted state and
+// slope averages from the heavy records.
+template <int D>
+__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const int* __restrict__ bin_off,
                                              const int32_t* __restrict__ used) {
-  constexpr int U = D + 2, M = D + 1, B = M - 1;
+  constexpr int U = D + 2, M = D + 1, B = M - 1, H = p2g_heavy<D>();
   const int nc = g.nc;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
   int i, j, k;
   cell_ijk(g, c, i, j, k);
   double xc[3];
   cell_center<D>(g, i, j, k, xc);
   const double hi = g.p_g * 1 * 3.2 / kCutOff;
-  const int64_t cap = P.cap;
   double mass_t[PGDG_MAX_PHASE] = {0, 0, 0};
   double pav[PGDG_MAX_PHASE][U];
   double cf[PGDG_MAX_PHASE][U][B];
@@ -57,55 +95,104 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, const i
     }
   }
   const int kc = D == 3 ? 1 : 0;
-  for (int t = max(i - 1, 0); t <= min(i + 1, g.N[0] - 1); t++)
-    for (int s_ = max(j - 1, 0); s_ <= min(j + 1, g.N[1] - 1); s_++)
-      for (int q = max(k - kc, 0); q <= min(k + kc, g.N[2] - 1); q++) {
+  const int t0 = max(i - 1, 0), t1 = min(i + 1, g.N[0] - 1);
+  const int s0 = max(j - 1, 0), s1 = min(j + 1, g.N[1] - 1);
+  const int q0 = max(k - kc, 0), q1 = min(k + kc, g.N[2] - 1);
+  // pass 1: phase masses
+  for (int t = t0; t <= t1; t++)
+    for (int s_ = s0; s_ <= s1; s_++)
+      for (int q = q0; q <= q1; q++) {
         int cc = (t * g.N[1] + s_) * g.N[2] + q;
         int e0 = bin_off[cc], e1 = bin_off[cc + 1];
         for (int e = e0; e < e1; e++) {
-          int p = bin_ent[e];
-          if (p < 0) continue;  // wall particles are not cell members
-          double xij[3];
-#pragma unroll
-          for (int a = 0; a < D; a++) xij[a] = P.x[a * cap + p] - xc[a];
+          const double* __restrict__ lp = pb.light + (int64_t)e * kP2gLight;
+          const double2 l01 = *reinterpret_cast<const double2*>(lp);
+          const double2 l23 = *reinterpret_cast<const double2*>(lp + 2);
+          const double2 l45 = *reinterpret_cast<const double2*>(lp + 4);
+          const int ph = (int)l45.x;
+          if (ph < 0) continue;  // wall particles are not cell members
+          const double xr[3] = {l01.x, l01.y, l23.x};
           double sq = 0;
 #pragma unroll
-          for (int a = 0; a < D; a++) sq += xij[a] * xij[a];
+          for (int a = 0; a < D; a++) {
+            const double xa = xr[a] - xc[a];
+            sq += xa * xa;
+          }
           double dist = sqrt(sq);
           if (dist < g.p2g_cut) {
-            double ww = kern_W<D>(dist, hi) * P.vol[p];
-            double pj[U];
-            pj[0] = P.rho[p];
+            double ww = kern_W<D>(dist, hi) * l23.y;
 #pragma unroll
-            for (int a = 0; a < D; a++) pj[1 + a] = P.rhov[a * cap + p];
-            pj[U - 1] = P.E[p];
-            double sl[U][B];
+            for (int s = 0; s < PGDG_MAX_PHASE; s++)
+              if (s < g.F && ph == s) mass_t[s] += ww;
+          }
+        }
+      }
+  // which phases appear in this cell
+  unsigned appear = 0;
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-              double acc = 0;
+  for (int s = 0; s < PGDG_MAX_PHASE; s++)
+    if (s < g.F && mass_t[s] > 0 && C.ph_state_b[s * nc + c] == 0) appear |= 1u << s;
+  if (appear) {
+    // pass 2: state and slope averages of the appearing phases
+    for (int t = t0; t <= t1; t++)
+      for (int s_ = s0; s_ <= s1; s_++)
+        for (int q = q0; q <= q1; q++) {
+          int cc = (t * g.N[1] + s_) * g.N[2] + q;
+          int e0 = bin_off[cc], e1 = bin_off[cc + 1];
+          for (int e = e0; e < e1; e++) {
+            const double* __restrict__ lp = pb.light + (int64_t)e * kP2gLight;
+            const double2 l01 = *reinterpret_cast<const double2*>(lp);
+            const double2 l23 = *reinterpret_cast<const double2*>(lp + 2);
+            const double2 l45 = *reinterpret_cast<const double2*>(lp + 4);
+            const int ph = (int)l45.x;
+            if (ph < 0 || !((appear >> ph) & 1u)) continue;
+            const double xr[3] = {l01.x, l01.y, l23.x};
+            double xij[3];
 #pragma unroll
-              for (int b = 0; b < B; b++) {
-                sl[u][b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
-                acc += sl[u][b] * (-xij[b]);
+            for (int a = 0; a < D; a++) xij[a] = xr[a] - xc[a];
+            double sq = 0;
+#pragma unroll
+            for (int a = 0; a < D; a++) sq += xij[a] * xij[a];
+            double dist = sqrt(sq);
+            if (dist < g.p2g_cut) {
+              double ww = kern_W<D>(dist, hi) * l23.y;
+              const double* __restrict__ hp = pb.heavy + (int64_t)e * H;
+              double h[H];
+#pragma unroll
+              for (int q2 = 0; q2 < H; q2 += 2) {
+                double2 v = *reinterpret_cast<const double2*>(hp + q2);
+                h[q2] = v.x;
+                h[q2 + 1] = v.y;
               }
-              pj[u] += acc;
-            }
-            int ph = P.phase[p] - 1;
+              double pj[U];
 #pragma unroll
-            for (int s = 0; s < PGDG_MAX_PHASE; s++) {
-              if (s < g.F && ph == s) {
+              for (int u = 0; u < U; u++) pj[u] = h[u];
+              double sl[U][B];
 #pragma unroll
-                for (int u = 0; u < U; u++) pav[s][u] += pj[u] * ww;
+              for (int u = 0; u < U; u++) {
+                double acc = 0;
 #pragma unroll
-                for (int u = 0; u < U; u++)
+                for (int b = 0; b < B; b++) {
+                  sl[u][b] = h[U + u * B + b];
+                  acc += sl[u][b] * (-xij[b]);
+                }
+                pj[u] += acc;
+              }
 #pragma unroll
-                  for (int b = 0; b < B; b++) cf[s][u][b] += sl[u][b] * ww;
-                mass_t[s] += ww;
+              for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+                if (s < g.F && ph == s) {
+#pragma unroll
+                  for (int u = 0; u < U; u++) pav[s][u] += pj[u] * ww;
+#pragma unroll
+                  for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int b = 0; b < B; b++) cf[s][u][b] += sl[u][b] * ww;
+                }
               }
             }
           }
         }
-      }
+  }
   double mass_total = 0;
 #pragma unroll
   for (int s = 0; s < PGDG_MAX_PHASE; s++)
@@ -225,7 +312,7 @@ template <int D>
 __global__ void __launch_bounds__(128) k_limiter(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = D + 1, NF = 2 * D;
   const int nc = g.nc;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
   int i, j, k;
   cell_ijk(g, c, i, j, k);
@@ -486,7 +573,7 @@ template <int D>
 __global__ void __launch_bounds__(128) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = D + 1, B = M - 1, NF = 2 * D;
   const int nc = g.nc;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
   int i, j, k;
   cell_ijk(g, c, i, j, k);
@@ -913,14 +1000,12 @@ int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, con
   const int nc = g.nc;
   switch (which) {
     case CELL_P2G:
-      hipLaunchKernelGGL(k_p2g<D>, dim3((nc + 127) / 128), dim3(128), 0, st, g, C, P, bin_off,
-                         bin_ent, used);
-      break;
+      return PGDG_ERR_ARG;  // launch_p2g
     case CELL_LIMITER:
-      hipLaunchKernelGGL(k_limiter<D>, dim3((nc + 127) / 128), dim3(128), 0, st, g, C, used);
+      hipLaunchKernelGGL(k_limiter<D>, dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used);
       break;
     case CELL_RHS:
-      hipLaunchKernelGGL(k_rhs<D>, dim3((nc + 127) / 128), dim3(128), 0, st, g, C, used);
+      hipLaunchKernelGGL(k_rhs<D>, dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used);
       break;
     case CELL_UPDATE:
       hipLaunchKernelGGL(k_update_grid<D>, dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
@@ -932,6 +1017,23 @@ int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, con
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
+
+template <int D>
+int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off, const int* bin_ent,
+               int64_t n_ent, const int32_t* used, const P2gBuf& pb, hipStream_t st) {
+  // every bin entry (the device-side total bin_off[nc]) is gathered; the
+  // grid covers the buffer's capacity
+  (void)n_ent;
+  hipLaunchKernelGGL(k_p2g_gather<D>, dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P,
+                     bin_ent, bin_off + g.nc, pb);
+  hipLaunchKernelGGL(k_p2g<D>, dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, pb, bin_off, used);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+template int launch_p2g<2>(const Geo&, const CellDev&, const PtlDev&, const int*, const int*, int64_t,
+                           const int32_t*, const P2gBuf&, hipStream_t);
+template int launch_p2g<3>(const Geo&, const CellDev&, const PtlDev&, const int*, const int*, int64_t,
+                           const int32_t*, const P2gBuf&, hipStream_t);
 
 template int launch_cells<2>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
                              const int32_t*, const DevScalars*, int, hipStream_t);

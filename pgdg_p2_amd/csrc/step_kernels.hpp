@@ -12,6 +12,23 @@ int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, con
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
                  hipStream_t st);
 
+// Particle data MLS_P2G reads, gathered into bin-entry order.  Every cell
+// needs the kernel weights (position, volume, phase: the 8-double `light`
+// record, phase -1 for walls); only cells where a phase appears also need
+// the state and slopes (the `heavy` record of p2g_heavy<D>() doubles).
+struct P2gBuf {
+  double* light;   // [ne][kP2gLight]
+  double* heavy;   // [ne][p2g_heavy<D>()]
+  int64_t ne;
+};
+constexpr int kP2gLight = 6;  // x[3] (or x[2], 0), vol, phase, pad
+template <int D>
+constexpr int p2g_heavy() { return ((D + 2) * (D + 1) + 1) / 2 * 2; }
+
+template <int D>
+int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off, const int* bin_ent,
+               int64_t n_ent, const int32_t* used, const P2gBuf& pb, hipStream_t st);
+
 // Per-particle stage work.
 struct StageBufs {
   double *xb, *hb;    // positions / smoothing lengths when the stage's

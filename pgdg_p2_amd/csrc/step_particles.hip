@@ -208,7 +208,7 @@ template <int D>
 __global__ void __launch_bounds__(256) k_pst_binned(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                     int64_t ne, const int* __restrict__ bin_off,
                                                     const DevScalars* __restrict__ sc) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int i = B.ref[e];
   if (i < 0) return;  // wall entry
@@ -368,8 +368,8 @@ int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
   hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S,
                      B, bin_ent, n_ent);
   hipLaunchKernelGGL(k_cell_bbox<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, B, bin_off);
-  hipLaunchKernelGGL(k_pst_binned<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, g, P, W,
-                     S, B, n_ent, bin_off, sc);
+  hipLaunchKernelGGL(k_pst_binned<D>, dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S, B,
+                     n_ent, bin_off, sc);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

@@ -191,44 +191,62 @@ def main():
     from pgdg_p2_amd.step import GpuSim
 
     nx, ny, nz = SIZES[args.size]
-    t_setup = time.perf_counter()
-    case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25, cap_factor=1.0)
-    n0 = st.n
-    gpu = GpuSim(case, st, cap=int(n0 * 2.2) + 4096)
-    del st
-    print(f"setup {time.perf_counter() - t_setup:.1f}s: {nx}x{ny}x{nz} cells, {n0} particles",
-          file=sys.stderr, flush=True)
     nc = nx * ny * nz
-    gpu.step(args.warmup)
-    gpu.sync()
-    n_start = gpu.count()
-    t0 = time.perf_counter()
-    gpu.step(args.steps)
-    gpu.sync()
-    el = time.perf_counter() - t0
-    n_end = gpu.count()
+    walls = (nx + 1) * (ny + 1) * (nz + 1) - (nx - 1) * (ny - 1) * (nz - 1)
+
+    def run_form(precision: int):
+        t_setup = time.perf_counter()
+        case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25,
+                                         cap_factor=1.0)
+        case.precision = precision
+        n0 = st.n
+        gpu = GpuSim(case, st, cap=int(n0 * 2.2) + 4096)
+        st.p = {}  # keep only the cell arrays, for the density comparison
+        name = "mixed" if precision else "fp64"
+        print(f"[{name}] setup {time.perf_counter() - t_setup:.1f}s: {nx}x{ny}x{nz} cells, "
+              f"{n0} particles", file=sys.stderr, flush=True)
+        gpu.step(args.warmup)
+        gpu.sync()
+        n_start = gpu.count()
+        t0 = time.perf_counter()
+        gpu.step(args.steps)
+        gpu.sync()
+        el = time.perf_counter() - t0
+        n_end = gpu.count()
+        print(f"[{name}] timed {args.steps} steps: {el / args.steps * 1e3:.2f} ms/step",
+              file=sys.stderr, flush=True)
+        # per-stage breakdown on extra steps (HIP events on the context stream)
+        gpu.profile(True)
+        gpu.step(args.profile_steps)
+        gpu.sync()
+        prof = gpu.profile_read()
+        gpu.profile(False)
+        stages = {k: {"ms_total": v[0], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
+        n_mid = gpu.count()
+        gpu.save_cells(st)
+        rho = (st.c["alph"] * st.c["rho"]).sum(axis=0)
+        gpu.close()
+        return {"case": case, "el": el, "n_start": n_start, "n_end": n_end, "n_mid": n_mid,
+                "stages": stages, "rho": rho}
+
+    plain = run_form(0)
+    mixed = run_form(1)
+    import numpy as np
+    drho = np.abs(mixed["rho"] - plain["rho"])
+    l1 = float(drho.sum() / np.abs(plain["rho"]).sum())
+    linf = float(drho.max() / np.abs(plain["rho"]).max())
+
+    R = mixed
+    case, el, stages = R["case"], R["el"], R["stages"]
     ms_step = el / args.steps * 1e3
     value = nc * args.steps / el
-    print(f"timed {args.steps} steps: {ms_step:.2f} ms/step", file=sys.stderr, flush=True)
-
-    # per-stage breakdown on extra steps (HIP events on the context stream)
-    gpu.profile(True)
-    gpu.step(args.profile_steps)
-    gpu.sync()
-    prof = gpu.profile_read()
-    gpu.profile(False)
-    stages = {k: {"ms_total": v[0], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
     dom = max(stages, key=lambda k: stages[k]["ms_total"])
-    n_mid = gpu.count()
-    from pgdg_p2_amd.state import HostState  # noqa: F401
-    walls = (nx + 1) * (ny + 1) * (nz + 1) - (nx - 1) * (ny - 1) * (nz - 1)
     per_launch_ms = stages[dom]["ms_total"] / stages[dom]["launches"]
-    alg = stage_bytes(dom, case, n_mid, walls)
+    alg = stage_bytes(dom, case, R["n_mid"], walls)
     achieved = alg / (per_launch_ms * 1e-3) / 1e9 if alg > 0 else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": None, "alg_bytes_per_launch": alg, "avg_launch_ms": per_launch_ms}
-    gpu.close()
 
     line = {
         "metric": "P2 DG cell-steps/s + MGFM Newton its/face, mixed vs FP64 plain at equal L1 err",
@@ -241,15 +259,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 state and fluxes, f32 particle-shifting pair terms (f64 sums)",
         "data": "synthetic",
         "config": {"workload": f"3-D shock-bubble step, {nx}x{ny}x{nz} cells, 8 particles/cell, "
                                "P1 DG (reference basis), reference two-stage step with split/merge, "
-                               "plain FP64",
-                   "cells": nc, "particles_start": n_start, "particles_end": n_end,
+                               "mixed form",
+                   "cells": nc, "particles_start": R["n_start"], "particles_end": R["n_end"],
                    "parallelism": "single GPU"},
-        "particles_per_s": (n_start + n_end) / 2 * args.steps / el,
+        "particles_per_s": (R["n_start"] + R["n_end"]) / 2 * args.steps / el,
         "stages_ms": {k: round(v["ms_total"] / args.profile_steps, 4) for k, v in stages.items()},
+        "plain_fp64": {
+            "value": nc * args.steps / plain["el"], "ms_per_step": plain["el"] / args.steps * 1e3,
+            "particles_end": plain["n_end"],
+            "stages_ms": {k: round(v["ms_total"] / args.profile_steps, 4)
+                          for k, v in plain["stages"].items()}},
+        "mixed_vs_plain": {
+            "speedup": plain["el"] / el,
+            "density_L1_rel": l1, "density_Linf_rel": linf,
+            "after_steps": args.warmup + args.steps + args.profile_steps,
+            "same_particle_count": plain["n_end"] == R["n_end"]},
         "roofline": roof,
     }
     if not args.no_mgfm:

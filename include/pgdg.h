@@ -198,6 +198,10 @@ typedef struct pgdg_case {
                             whole cell only at one particle per cell       */
   double output_dt;      /* dt is truncated to land on multiples of this    */
   double max_v0;         /* initial max_v (reference 1e-7)                  */
+  int    precision;      /* PGDG_PREC_FP64: the reference's FP64 arithmetic
+                            and order (bit-identical to the CPU restatement);
+                            PGDG_PREC_MIXED: particle shifting evaluated in
+                            FP32 in cell-local coordinates with FP64 sums   */
 } pgdg_case;
 
 /* Host SoA views.  Arrays are sized by capacity `cap` (particles) or by the

@@ -110,6 +110,8 @@ struct Ctx {
   BinData B{};
   double* b_slab = nullptr;
   P2gBuf PB{};
+  MixedBins MB{};
+  void* mb_slab = nullptr;
   int32_t* bi_slab = nullptr;
   Topo T;
   DevScalars* sc = nullptr;
@@ -241,13 +243,27 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PB.ne = ne;
   PGDG_HIP_TRY(hipMalloc((void**)&PB.light, sizeof(double) * p2g_rows_n * (size_t)ne));
   PB.heavy = PB.light + kP2gLight * (size_t)ne;
+  size_t mb_bytes = 0;
+  if (cs.precision == PGDG_PREC_MIXED) {
+    // r1, r2 (16 B), r3 (8 B), ecell (4 B) per entry; boxes 6 floats per cell
+    mb_bytes = (size_t)ne * (16 + 16 + 8 + 4) + 6 * sizeof(float) * (size_t)nc + 64;
+    PGDG_HIP_TRY(hipMalloc(&mb_slab, mb_bytes));
+    char* q = (char*)mb_slab;
+    MB.r1 = (float4*)q; q += 16 * (size_t)ne;
+    MB.r2 = (float4*)q; q += 16 * (size_t)ne;
+    MB.r3 = (float2*)q; q += 8 * (size_t)ne;
+    MB.ecell = (int32_t*)q; q += 4 * (size_t)ne;
+    MB.bmin = (float*)q; q += 3 * sizeof(float) * (size_t)nc;
+    MB.bmax = (float*)q;
+    MB.ne_cap = ne;
+  }
   PGDG_HIP_TRY(hipMalloc((void**)&sc, sizeof(DevScalars)));
   PGDG_HIP_TRY(hipMemset(sc, 0, sizeof(DevScalars)));
   PGDG_TRY(T.init(cap, nwall, nc, st));
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall +
                             9 * (size_t)ne + 6 * (size_t)nc + p2g_rows_n * (size_t)ne) +
-          T.bytes_total;
+          T.bytes_total + mb_bytes;
   return PGDG_OK;
 }
 
@@ -255,7 +271,7 @@ void Ctx::release() {
   if (st) (void)hipStreamSynchronize(st);
   PA.release();
   T.release();
-  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light};
+  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light, mb_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (own_stream && st) (void)hipStreamDestroy(st);
@@ -499,7 +515,10 @@ int Ctx::stage_impl(int fn, int arg) {
         PGDG_TRY(prof.end(r, st));
         PGDG_TRY(prof.begin(PS_PST, st, r));
       }
-      PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, T.n_ent, T.bin_off, T.bin_ent, sc, st));
+      if (cs.precision == PGDG_PREC_MIXED)
+        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, T.n_ent, T.bin_off, T.bin_ent, sc, st));
+      else
+        PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, T.n_ent, T.bin_off, T.bin_ent, sc, st));
       if (prof.on) PGDG_TRY(prof.end(r, st));
       return PGDG_OK;
     }
@@ -566,6 +585,7 @@ int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_c
   if (c->nphase < 2 || c->nphase > PGDG_MAX_PHASE) return PGDG_ERR_ARG;
   if (c->order != 1 || c->rk_stages != 2) return PGDG_ERR_ARG;
   if (c->semantics != PGDG_SEM_T2 && c->semantics != PGDG_SEM_S3) return PGDG_ERR_ARG;
+  if (c->precision != PGDG_PREC_FP64 && c->precision != PGDG_PREC_MIXED) return PGDG_ERR_ARG;
   if (particle_cap > (int64_t)0x7ffffff0) return PGDG_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PGDG_ERR_NODEVICE;

@@ -62,6 +62,24 @@ int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
                       const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
                       const DevScalars* sc, hipStream_t st);
 
+// Mixed-precision particle shifting: FP32 records in bin-entry order with
+// positions relative to the lower corner of the entry's bin cell.
+//   r1 = (stage-start position, phase bits), r2 = (current position, vol),
+//   r3 = (rho, c); ecell = bin cell of the entry; bmin/bmax = per-cell box
+//   of the stage-start positions in the cell's own frame.
+struct MixedBins {
+  float4 *r1, *r2;
+  float2* r3;
+  int32_t* ecell;
+  float *bmin, *bmax;  // [3][nc]
+  int64_t ne_cap;
+};
+
+template <int D>
+int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                     const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st);
+
 template <int D>
 int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,
                     int64_t n, bool cells, DevScalars* sc, double output_dt, hipStream_t st);

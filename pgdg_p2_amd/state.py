@@ -20,6 +20,7 @@ class Case(C.Structure):
         ("ini_rho", C.c_double * PGDG_MAX_PHASE),
         ("inlet_rho", C.c_double), ("inlet_u", C.c_double * 3), ("inlet_p", C.c_double),
         ("p2g_cut", C.c_double), ("output_dt", C.c_double), ("max_v0", C.c_double),
+        ("precision", C.c_int),
     ]
 
     @property

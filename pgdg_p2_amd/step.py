@@ -38,6 +38,13 @@ class GpuSim:
         st.n = P.n
         return st
 
+    def save_cells(self, st: HostState) -> HostState:
+        """Download only the cell arrays and scalars."""
+        Cc = st.cells()
+        check(self.L.pgdg_download_state(self.h, None, C.byref(Cc), C.byref(st.s)),
+              "pgdg_download_state")
+        return st
+
     def step(self, n: int = 1):
         check(self.L.pgdg_step(self.h, n), "pgdg_step")
 

@@ -116,6 +116,67 @@ int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, int64_t n,
 int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream);
 
 /* ======================================================================== *
+ *  Learned initial guess for the MGFM Newton solve (no reference          *
+ *  counterpart: the reference starts from max(pL, pR), S3:mpm.cpp:891)    *
+ * ======================================================================== */
+
+/* A 4-layer MLP 10 -> 64 -> 64 -> 64 -> 2 (tanh) mapping the standardised
+ * face features (ln pL, ln pR, ln rhoL, ln rhoR, uL, uR, gammaL,
+ * ln(1 + PinfL), gammaR, ln(1 + PinfR)) to the standardised targets
+ * (ln p* - ln max(pL, pR), u*).  Matrix products run on MFMA with BF16
+ * operands and FP32 accumulation; the master parameters and the optimiser
+ * (Adam) are FP32.  Parameter vector layout (FP32, PGDG_MLP_NPARAM):
+ * W0[16][64] (rows 10..15 zero), b0[64], W1[64][64], b1[64], W2[64][64],
+ * b2[64], W3[64][16] (columns 2..15 zero), b3[16] (entries 2..15 zero).  */
+#define PGDG_MLP_NIN     10
+#define PGDG_MLP_NPAD    16
+#define PGDG_MLP_WIDTH   64
+#define PGDG_MLP_NOUT    2
+#define PGDG_MLP_NPARAM  10448
+
+typedef struct pgdg_mlp pgdg_mlp;
+
+typedef struct pgdg_mlp_norm {
+  float feat_mu[PGDG_MLP_NIN], feat_sd[PGDG_MLP_NIN];
+  float out_mu[PGDG_MLP_NOUT], out_sd[PGDG_MLP_NOUT];
+} pgdg_mlp_norm;
+
+typedef struct pgdg_mlp_train_opts {
+  int      epochs;       /* passes over the training faces                  */
+  int      batch;        /* samples per optimiser step (multiple of 64)     */
+  float    lr;           /* Adam step size (cosine-decayed to lr/20)        */
+  uint64_t seed;         /* parameter initialisation                        */
+  int      variant;      /* MGFM variant that labels the faces              */
+} pgdg_mlp_train_opts;
+
+int pgdg_mlp_create(pgdg_mlp** out);
+int pgdg_mlp_destroy(pgdg_mlp* m);
+/* Labels the n device faces in[6][n] (ul, ur, pl, pr, rhol, rhor) with the
+ * exact MGFM solve (relative tolerance 1e-12), sets the normalisation from
+ * them and trains.  loss_hist (host, may be NULL) receives the mean loss of
+ * each optimiser step, up to hist_cap entries; *steps the number of steps. */
+int pgdg_mlp_train(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair* eos,
+                   const pgdg_mlp_train_opts* opts, float* loss_hist, int hist_cap, int* steps,
+                   void* stream);
+/* p*0 for each face (device in[6][n] -> device pstar0[n]); a non-finite or
+ * non-positive prediction falls back to max(pl, pr).                      */
+int pgdg_mlp_guess(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair* eos,
+                   double* pstar0, void* stream);
+/* pgdg_mgfm_batch with the learned p*0 (the Newton iteration still runs to
+ * the requested tolerance).                                                */
+int pgdg_mgfm_batch_learned(pgdg_mlp* m, const double* in, double* out, int32_t* iters, int64_t n,
+                            const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts, void* stream);
+/* Parameters and normalisation (host buffers).                             */
+int pgdg_mlp_get_params(pgdg_mlp* m, float* params, pgdg_mlp_norm* norm);
+int pgdg_mlp_set_params(pgdg_mlp* m, const float* params, const pgdg_mlp_norm* norm);
+/* One forward (y[n][2], standardised) or forward + backward on standardised
+ * device inputs x[n][16] and targets t[n][2] (n a multiple of 64): mean loss
+ * 0.5 * sum_o (y - t)^2 / n and its gradient grad[PGDG_MLP_NPARAM] (device).  */
+int pgdg_mlp_forward(pgdg_mlp* m, const float* x, float* y, int64_t n, void* stream);
+int pgdg_mlp_loss_grad(pgdg_mlp* m, const float* x, const float* t, int64_t n, float* grad,
+                       float* loss, void* stream);
+
+/* ======================================================================== *
  *  MLS particle-to-grid moment assembly with a per-cell least-squares      *
  *  solve (extension of MPM::MLS_P2G, S3:mpm.cpp:274-382)                   *
  * ======================================================================== */

@@ -1,0 +1,125 @@
+"""Test infrastructure only: numpy restatement of the learned MGFM guess
+network (pgdg_p2_amd/csrc/mlp.hip) for checking the GPU forward pass and
+gradients.  Not used by the product.
+
+The network is 10 -> 64 -> 64 -> 64 -> 2 with tanh, parameters in the flat
+layout of include/pgdg.h (PGDG_MLP_NPARAM).  With emulate=True the BF16
+roundings of the GPU kernels are applied at the same points (inputs,
+weights, activations, output and hidden gradients entering a matrix
+product); products are then accumulated in FP64, so the result differs
+from the GPU's FP32 accumulation by rounding only.  With emulate=False the
+whole computation is FP64 and the gradient is the exact derivative of the
+loss, which the tests pin with central finite differences.
+
+The reference code has no counterpart (its Newton solve starts from
+max(pL, pR), S3:mpm.cpp:891): parity of this part rests on the finite
+difference check and on the GPU-vs-restatement comparison.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NIN, NPAD, H, NOUT, OPAD = 10, 16, 64, 2, 16
+OW0 = 0
+OB0 = OW0 + NPAD * H
+OW1 = OB0 + H
+OB1 = OW1 + H * H
+OW2 = OB1 + H
+OB2 = OW2 + H * H
+OW3 = OB2 + H
+OB3 = OW3 + H * OPAD
+NPARAM = OB3 + OPAD
+
+
+def live_mask() -> np.ndarray:
+    m = np.ones(NPARAM, dtype=bool)
+    w0 = m[OW0:OB0].reshape(NPAD, H)
+    w0[NIN:] = False
+    w3 = m[OW3:OB3].reshape(H, OPAD)
+    w3[:, NOUT:] = False
+    m[OB3 + NOUT:] = False
+    return m
+
+
+def bf16(a) -> np.ndarray:
+    """Round to BF16 (nearest even), returned as float64 values."""
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    nan = np.isnan(f)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    r = u.astype(np.uint32).view(np.float32).astype(np.float64)
+    r[nan] = np.nan
+    return r
+
+
+def unpack(P):
+    P = np.asarray(P, dtype=np.float64)
+    return (P[OW0:OB0].reshape(NPAD, H), P[OB0:OW1], P[OW1:OB1].reshape(H, H), P[OB1:OW2],
+            P[OW2:OB2].reshape(H, H), P[OB2:OW3], P[OW3:OB3].reshape(H, OPAD), P[OB3:NPARAM])
+
+
+def _q(emulate):
+    return bf16 if emulate else (lambda a: np.asarray(a, dtype=np.float64))
+
+
+def forward(P, X, emulate=True):
+    """X [n][16] standardised features -> (Y [n][2], cache)."""
+    q = _q(emulate)
+    W0, b0, W1, b1, W2, b2, W3, b3 = unpack(P)
+    W0q, W1q, W2q, W3q = q(W0), q(W1), q(W2), q(W3)
+    Xq = q(X)
+    H0 = q(np.tanh(Xq @ W0q + b0))
+    H1 = q(np.tanh(H0 @ W1q + b1))
+    H2 = q(np.tanh(H1 @ W2q + b2))
+    Y = H2 @ W3q + b3
+    return Y[:, :NOUT], (Xq, H0, H1, H2, W0q, W1q, W2q, W3q)
+
+
+def loss(P, X, T, emulate=False):
+    Y, _ = forward(P, X, emulate)
+    d = Y - T
+    return 0.5 * float((d * d).sum()) / X.shape[0]
+
+
+def loss_grad(P, X, T, emulate=True):
+    """Mean loss 0.5 * sum_o (y - t)^2 / n and its gradient (flat layout)."""
+    q = _q(emulate)
+    n = X.shape[0]
+    Y, (Xq, H0, H1, H2, W0q, W1q, W2q, W3q) = forward(P, X, emulate)
+    d = Y - T
+    L = 0.5 * float((d * d).sum()) / n
+    dY = np.zeros((n, OPAD))
+    dY[:, :NOUT] = d / n
+    if emulate:
+        dY = dY.astype(np.float32).astype(np.float64)
+    dYq = q(dY)
+    g = np.zeros(NPARAM)
+    g[OW3:OB3] = (H2.T @ dYq).ravel()
+    g[OB3:NPARAM] = dY.sum(axis=0)
+    dZ2 = q((dYq @ W3q.T) * (1.0 - H2 * H2))
+    g[OW2:OB2] = (H1.T @ dZ2).ravel()
+    g[OB2:OW3] = dZ2.sum(axis=0)
+    dZ1 = q((dZ2 @ W2q.T) * (1.0 - H1 * H1))
+    g[OW1:OB1] = (H0.T @ dZ1).ravel()
+    g[OB1:OW2] = dZ1.sum(axis=0)
+    dZ0 = q((dZ1 @ W1q.T) * (1.0 - H0 * H0))
+    g[OW0:OB0] = (Xq.T @ dZ0).ravel()
+    g[OB0:OW1] = dZ0.sum(axis=0)
+    g[~live_mask()] = 0.0
+    return L, g
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    P = np.zeros(NPARAM)
+    for off, rows, cols, lr, lc, fi, fo in ((OW0, NPAD, H, NIN, H, NIN, H), (OW1, H, H, H, H, H, H),
+                                             (OW2, H, H, H, H, H, H),
+                                             (OW3, H, OPAD, H, NOUT, H, NOUT)):
+        a = np.sqrt(6.0 / (fi + fo))
+        w = np.zeros((rows, cols))
+        w[:lr, :lc] = rng.uniform(-a, a, (lr, lc))
+        P[off:off + rows * cols] = w.ravel()
+    for off in (OB0, OB1, OB2):
+        P[off:off + H] = rng.uniform(-0.1, 0.1, H)
+    P[OB3:OB3 + NOUT] = rng.uniform(-0.1, 0.1, NOUT)
+    return P

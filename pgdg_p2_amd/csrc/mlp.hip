@@ -1,0 +1,709 @@
+// mlp.hip - learned initial guess for the MGFM Newton solve.
+//
+// A 10 -> 64 -> 64 -> 64 -> 2 tanh MLP.  Every matrix product is a set of
+// 16x16 output tiles computed with v_mfma_f32_16x16x16_bf16 (BF16 operands
+// staged in LDS, FP32 accumulation); a workgroup of four wavefronts handles
+// 64 samples.  Training keeps FP32 master parameters; each workgroup writes
+// its partial gradient and a second kernel sums the partials in a fixed
+// order and applies Adam, so training is deterministic.  The reference has
+// no counterpart: its Newton solve starts from max(pL, pR) (S3:mpm.cpp:891)
+// or (pL + pR) / 2 (T2:mpm.cpp:1383).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "hip_util.hpp"
+#include "riemann.hpp"
+
+namespace pgdg {
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = PGDG_MLP_NIN, kInPad = PGDG_MLP_NPAD, kH = PGDG_MLP_WIDTH;
+constexpr int kOut = PGDG_MLP_NOUT, kOutPad = 16;
+constexpr int oW0 = 0;
+constexpr int oB0 = oW0 + kInPad * kH;
+constexpr int oW1 = oB0 + kH;
+constexpr int oB1 = oW1 + kH * kH;
+constexpr int oW2 = oB1 + kH;
+constexpr int oB2 = oW2 + kH * kH;
+constexpr int oW3 = oB2 + kH;
+constexpr int oB3 = oW3 + kH * kOutPad;
+constexpr int kNParam = oB3 + kOutPad;
+static_assert(kNParam == PGDG_MLP_NPARAM, "parameter count");
+constexpr int kTile = 64;     // samples per workgroup
+constexpr int kThreads = 256;
+
+// padded parameters stay zero
+__host__ __device__ inline bool param_live(int p) {
+  if (p < oB0) return (p - oW0) / kH < kIn;
+  if (p >= oW3 && p < oB3) return (p - oW3) % kOutPad < kOut;
+  if (p >= oB3) return p - oB3 < kOut;
+  return true;
+}
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet)
+__device__ inline unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// One 16x16 tile of C = A * B over K (a multiple of 16) from BF16 LDS
+// operands with element strides: A(m, k) = A[m * ars + k * acs],
+// B(k, n) = B[k * brs + n * bcs].  Lane l holds C(m0 + 4 (l / 16) + i,
+// n0 + l % 16) in component i.
+__device__ inline f32x4_t mma_tile(const unsigned short* A, int ars, int acs,
+                                   const unsigned short* B, int brs, int bcs, int m0, int n0,
+                                   int K) {
+  const int l = threadIdx.x & 63;
+  const int r = l & 15, kq = (l >> 4) * 4;
+  f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4_t a, b;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      a[i] = (short)A[(m0 + r) * ars + (k0 + kq + i) * acs];
+      b[i] = (short)B[(k0 + kq + i) * brs + (n0 + r) * bcs];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+struct FwdLds {
+  unsigned short X[kTile * kInPad];
+  unsigned short W0[kInPad * kH], W1[kH * kH], W2[kH * kH], W3[kH * kOutPad];
+  unsigned short H0[kTile * kH], H1[kTile * kH], H2[kTile * kH];
+  float b0[kH], b1[kH], b2[kH], b3[kOutPad];
+};
+// Training adds the output gradient (FP32 and BF16) and the BF16 gradient
+// of the current hidden layer; tanh' is taken from the stored BF16
+// activations, 1 - h^2.  Total below 64 KiB.
+struct TrainLds {
+  FwdLds f;
+  float DYf[kTile * kOutPad];
+  unsigned short DY[kTile * kOutPad];
+  unsigned short DZ[kTile * kH];
+  float lossv[kTile];
+};
+static_assert(sizeof(TrainLds) <= 65536, "training LDS");
+
+// Load the parameters (FP32 -> BF16 weights, FP32 biases) into LDS.
+__device__ inline void load_params(const float* __restrict__ P, FwdLds& s) {
+  for (int q = threadIdx.x; q < kInPad * kH; q += kThreads) s.W0[q] = f2bf(P[oW0 + q]);
+  for (int q = threadIdx.x; q < kH * kH; q += kThreads) {
+    s.W1[q] = f2bf(P[oW1 + q]);
+    s.W2[q] = f2bf(P[oW2 + q]);
+  }
+  for (int q = threadIdx.x; q < kH * kOutPad; q += kThreads) s.W3[q] = f2bf(P[oW3 + q]);
+  for (int q = threadIdx.x; q < kH; q += kThreads) {
+    s.b0[q] = P[oB0 + q];
+    s.b1[q] = P[oB1 + q];
+    s.b2[q] = P[oB2 + q];
+  }
+  for (int q = threadIdx.x; q < kOutPad; q += kThreads) s.b3[q] = P[oB3 + q];
+}
+
+// Hidden layer: Hout = tanh(Hin * W + b), 64 x 64, K = kin.
+__device__ inline void hidden(const unsigned short* Hin, int kin, const unsigned short* W,
+                              const float* b, unsigned short* Hout) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  for (int t = w; t < 16; t += 4) {
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    f32x4_t acc = mma_tile(Hin, kin, 1, W, kH, 1, m0, n0, kin);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
+      Hout[row * kH + col] = f2bf(tanhf(acc[i] + b[col]));
+    }
+  }
+}
+
+// Forward through the block's 64 samples (X already in LDS); output layer
+// values are passed to `emit(row, col, y)` for col < kOut.
+template <typename Emit>
+__device__ inline void forward(FwdLds& s, Emit emit) {
+  hidden(s.X, kInPad, s.W0, s.b0, s.H0);
+  __syncthreads();
+  hidden(s.H0, kH, s.W1, s.b1, s.H1);
+  __syncthreads();
+  hidden(s.H1, kH, s.W2, s.b2, s.H2);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int m0 = w * 16;
+  f32x4_t acc = mma_tile(s.H2, kH, 1, s.W3, kOutPad, 1, m0, 0, kH);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = m0 + (l >> 4) * 4 + i, col = l & 15;
+    emit(row, col, acc[i] + s.b3[col]);
+  }
+}
+
+// standardised face features (FP32, padded to 16)
+__device__ inline void face_features(const double* __restrict__ in, int64_t n, int64_t i, Eos ml,
+                                     Eos mr, const pgdg_mlp_norm& nm, float* f) {
+  const double ul = in[i], ur = in[n + i], pl = in[2 * n + i], pr = in[3 * n + i];
+  const double rl = in[4 * n + i], rr = in[5 * n + i];
+  double raw[kIn] = {log(fmax(pl, 1e-30)), log(fmax(pr, 1e-30)), log(fmax(rl, 1e-30)),
+                     log(fmax(rr, 1e-30)), ul, ur, ml.gamma, log1p(fmax(ml.pinf, 0.0)),
+                     mr.gamma, log1p(fmax(mr.pinf, 0.0))};
+#pragma unroll
+  for (int k = 0; k < kIn; k++) f[k] = (float)((raw[k] - nm.feat_mu[k]) / nm.feat_sd[k]);
+#pragma unroll
+  for (int k = kIn; k < kInPad; k++) f[k] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// inference: p*0 for every face
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_mlp_guess(const float* __restrict__ P, pgdg_mlp_norm nm,
+                                                        const double* __restrict__ in, int64_t n,
+                                                        Eos ml, Eos mr, double* __restrict__ pstar0) {
+  __shared__ FwdLds s;
+  load_params(P, s);
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  if (threadIdx.x < kTile) {
+    float f[kInPad];
+    const int64_t i = base + threadIdx.x;
+    if (i < n) face_features(in, n, i, ml, mr, nm, f);
+    else
+      for (int k = 0; k < kInPad; k++) f[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < kInPad; k++) s.X[threadIdx.x * kInPad + k] = f2bf(f[k]);
+  }
+  __syncthreads();
+  forward(s, [&](int row, int col, float y) {
+    if (col != 0) return;
+    const int64_t i = base + row;
+    if (i >= n) return;
+    const double pl = in[2 * n + i], pr = in[3 * n + i];
+    const double pm = pl >= pr ? pl : pr;
+    const double lr = (double)y * nm.out_sd[0] + nm.out_mu[0];
+    double g = pm * exp(lr);
+    if (!(g > 0) || !(g < 1e300)) g = pm;
+    pstar0[i] = g;
+  });
+}
+
+// forward on standardised inputs (y[n][2] standardised)
+__global__ void __launch_bounds__(kThreads) k_mlp_forward(const float* __restrict__ P,
+                                                          const float* __restrict__ x, int64_t n,
+                                                          float* __restrict__ y) {
+  __shared__ FwdLds s;
+  load_params(P, s);
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int q = threadIdx.x; q < kTile * kInPad; q += kThreads) {
+    const int64_t i = base + q / kInPad;
+    s.X[q] = f2bf(i < n ? x[(base * kInPad) + q] : 0.f);
+  }
+  __syncthreads();
+  forward(s, [&](int row, int col, float v) {
+    const int64_t i = base + row;
+    if (col < kOut && i < n) y[i * kOut + col] = v;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// training: forward + backward of 64 samples, partial gradient per block
+// ---------------------------------------------------------------------------
+// Writes the tile of dW (rows m0.., cols n0..) of a [rows][ncols] weight.
+__device__ inline void put_tile(float* g, int ncols, int m0, int n0, f32x4_t acc) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 4; i++) g[(m0 + (l >> 4) * 4 + i) * ncols + n0 + (l & 15)] = acc[i];
+}
+
+// Hidden-layer backward: from DZ (BF16 dL/dZ of the layer) compute
+// dW = Hin^T DZ and db = column sums of DZ; if `below`, also dH = DZ W^T and
+// DZ <- dH * (1 - Hin^2) for the layer underneath (Hin is its output).
+__device__ inline void backward_layer(TrainLds& s, const unsigned short* Hin, int kin,
+                                      const unsigned short* W, float* gW, float* gb, bool below) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // dW[k][n] = sum_s Hin[s][k] DZ[s][n]
+  const int ntile_k = kin / 16;
+  for (int t = w; t < ntile_k * 4; t += 4) {
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    put_tile(gW, kH, m0, n0, mma_tile(Hin, 1, kin, s.DZ, kH, 1, m0, n0, kTile));
+  }
+  if (threadIdx.x < kH) {
+    float acc = 0.f;
+    for (int r = 0; r < kTile; r++) acc += bf2f(s.DZ[r * kH + threadIdx.x]);
+    gb[threadIdx.x] = acc;
+  }
+  if (!below) return;
+  // dH[s][k] = sum_n DZ[s][n] W[k][n]
+  f32x4_t dh[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int t = w + 4 * q;
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    dh[q] = mma_tile(s.DZ, kH, 1, W, 1, kH, m0, n0, kH);
+  }
+  __syncthreads();  // every read of DZ is done
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int t = w + 4 * q;
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
+      const float h = bf2f(Hin[row * kH + col]);
+      s.DZ[row * kH + col] = f2bf(dh[q][i] * (1.f - h * h));
+    }
+  }
+  __syncthreads();
+}
+
+// x[n][16] standardised features, t[n][2] standardised targets; samples
+// [b0, b0 + gridDim.x * 64) (all < n); inv_b = 1 / batch size.  Partial
+// gradient of block b at part[b][kNParam], partial loss at lpart[b].
+__global__ void __launch_bounds__(kThreads) k_mlp_fwd_bwd(const float* __restrict__ P,
+                                                          const float* __restrict__ x,
+                                                          const float* __restrict__ tg, int64_t b0,
+                                                          float inv_b, float* __restrict__ part,
+                                                          float* __restrict__ lpart) {
+  __shared__ TrainLds s;
+  load_params(P, s.f);
+  const int64_t base = b0 + (int64_t)blockIdx.x * kTile;
+  for (int q = threadIdx.x; q < kTile * kInPad; q += kThreads) s.f.X[q] = f2bf(x[base * kInPad + q]);
+  __syncthreads();
+  forward(s.f, [&](int row, int col, float y) {
+    float dy = 0.f;
+    if (col < kOut) {
+      const float d = y - tg[(base + row) * kOut + col];
+      dy = d * inv_b;
+    }
+    s.DYf[row * kOutPad + col] = dy;
+    s.DY[row * kOutPad + col] = f2bf(dy);
+  });
+  __syncthreads();
+  if (threadIdx.x < kTile) {
+    // 0.5 * sum_o d^2 of this sample (d = dy / inv_b)
+    float a = 0.f;
+    for (int o = 0; o < kOut; o++) {
+      const float d = s.DYf[threadIdx.x * kOutPad + o] / inv_b;
+      a += 0.5f * d * d;
+    }
+    s.lossv[threadIdx.x] = a;
+  }
+  float* g = part + (size_t)blockIdx.x * kNParam;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // output layer: dW3[k][o] = sum_s H2[s][k] DY[s][o]; db3; dH2 = DY W3^T
+  put_tile(g + oW3, kOutPad, w * 16, 0, mma_tile(s.f.H2, 1, kH, s.DY, kOutPad, 1, w * 16, 0, kTile));
+  if (threadIdx.x < kOutPad) {
+    float acc = 0.f;
+    for (int r = 0; r < kTile; r++) acc += s.DYf[r * kOutPad + threadIdx.x];
+    g[oB3 + threadIdx.x] = acc;
+  }
+  if (threadIdx.x == 0) {
+    float a = 0.f;
+    for (int r = 0; r < kTile; r++) a += s.lossv[r];
+    lpart[blockIdx.x] = a;
+  }
+  {
+    f32x4_t dh[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int t = w + 4 * q;
+      const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+      dh[q] = mma_tile(s.DY, kOutPad, 1, s.f.W3, 1, kOutPad, m0, n0, kOutPad);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int t = w + 4 * q;
+      const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
+        const float h = bf2f(s.f.H2[row * kH + col]);
+        s.DZ[row * kH + col] = f2bf(dh[q][i] * (1.f - h * h));
+      }
+    }
+  }
+  __syncthreads();
+  backward_layer(s, s.f.H1, kH, s.f.W2, g + oW2, g + oB2, true);
+  backward_layer(s, s.f.H0, kH, s.f.W1, g + oW1, g + oB1, true);
+  backward_layer(s, s.f.X, kInPad, s.f.W0, g + oW0, g + oB0, false);
+}
+
+// Sum the block partials in block order (padded parameters get zero) and
+// the block losses, scaled to the mean.
+__global__ void k_mlp_reduce(const float* __restrict__ part, const float* __restrict__ lpart, int nb,
+                             float lscale, float* __restrict__ grad, float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kNParam) {
+    float acc = 0.f;
+    for (int b = 0; b < nb; b++) acc += part[(size_t)b * kNParam + p];
+    grad[p] = param_live(p) ? acc : 0.f;
+  }
+  if (p == 0 && loss) {
+    float a = 0.f;
+    for (int b = 0; b < nb; b++) a += lpart[b];
+    *loss = a * lscale;
+  }
+}
+
+__global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                           const float* __restrict__ grad, float lr, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNParam || !param_live(p)) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gp = grad[p];
+  const float mp = b1 * m[p] + (1.f - b1) * gp;
+  const float vp = b2 * v[p] + (1.f - b2) * gp * gp;
+  m[p] = mp;
+  v[p] = vp;
+  P[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+}
+
+// training set: standardised features and targets from labelled faces
+__global__ void k_mlp_build_set(const double* __restrict__ in, const double* __restrict__ out,
+                                int64_t n, Eos ml, Eos mr, pgdg_mlp_norm nm, float* __restrict__ x,
+                                float* __restrict__ t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f[kInPad];
+  face_features(in, n, i, ml, mr, nm, f);
+#pragma unroll
+  for (int k = 0; k < kInPad; k++) x[i * kInPad + k] = f[k];
+  const double pl = in[2 * n + i], pr = in[3 * n + i];
+  const double pm = pl >= pr ? pl : pr;
+  const double y0 = log(out[n + i]) - log(pm), y1 = out[2 * n + i];
+  t[i * kOut] = (float)((y0 - nm.out_mu[0]) / nm.out_sd[0]);
+  t[i * kOut + 1] = (float)((y1 - nm.out_mu[1]) / nm.out_sd[1]);
+}
+
+// raw feature / target moments: per-block FP64 sums of (v, v^2) for the
+// 10 features and 2 targets
+__global__ void k_mlp_moments(const double* __restrict__ in, const double* __restrict__ out,
+                              int64_t n, Eos ml, Eos mr, double* __restrict__ part) {
+  constexpr int NV = kIn + kOut;
+  __shared__ double red[kThreads];
+  double acc[2 * NV];
+  for (int q = 0; q < 2 * NV; q++) acc[q] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double ul = in[i], ur = in[n + i], pl = in[2 * n + i], pr = in[3 * n + i];
+    const double rl = in[4 * n + i], rr = in[5 * n + i];
+    const double pm = pl >= pr ? pl : pr;
+    const double v[NV] = {log(fmax(pl, 1e-30)), log(fmax(pr, 1e-30)), log(fmax(rl, 1e-30)),
+                          log(fmax(rr, 1e-30)), ul, ur, ml.gamma, log1p(fmax(ml.pinf, 0.0)),
+                          mr.gamma, log1p(fmax(mr.pinf, 0.0)), log(out[n + i]) - log(pm),
+                          out[2 * n + i]};
+    for (int q = 0; q < NV; q++) {
+      acc[q] += v[q];
+      acc[NV + q] += v[q] * v[q];
+    }
+  }
+  for (int q = 0; q < 2 * NV; q++) {
+    red[threadIdx.x] = acc[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0;
+      for (int t = 0; t < kThreads; t++) a += red[t];
+      part[(size_t)blockIdx.x * 2 * NV + q] = a;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace pgdg
+
+struct pgdg_mlp {
+  float* params = nullptr;  // device FP32 master parameters
+  float* m = nullptr;
+  float* v = nullptr;
+  float* grad = nullptr;
+  float* part = nullptr;
+  float* lpart = nullptr;
+  float* loss = nullptr;
+  int part_blocks = 0;
+  pgdg_mlp_norm norm{};
+  double* guess = nullptr;
+  int64_t guess_cap = 0;
+  bool ready = false;
+};
+
+namespace {
+
+int ensure_partials(pgdg_mlp* m, int nb) {
+  if (nb <= m->part_blocks) return PGDG_OK;
+  if (m->part) (void)hipFree(m->part);
+  if (m->lpart) (void)hipFree(m->lpart);
+  m->part = nullptr;
+  m->lpart = nullptr;
+  PGDG_HIP_TRY(hipMalloc((void**)&m->part, sizeof(float) * (size_t)nb * pgdg::kNParam));
+  PGDG_HIP_TRY(hipMalloc((void**)&m->lpart, sizeof(float) * (size_t)nb));
+  m->part_blocks = nb;
+  return PGDG_OK;
+}
+
+int launch_fwd_bwd(pgdg_mlp* m, const float* x, const float* t, int64_t b0, int64_t cnt, float* grad,
+                   float* loss, hipStream_t st) {
+  using namespace pgdg;
+  const int nb = (int)(cnt / kTile);
+  PGDG_TRY(ensure_partials(m, nb));
+  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nb), dim3(kThreads), 0, st, m->params, x, t, b0,
+                     1.0f / (float)cnt, m->part, m->lpart);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNParam + 255) / 256), dim3(256), 0, st, m->part,
+                     m->lpart, nb, 1.0f / (float)cnt, grad, loss);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+void init_params(std::vector<float>& P, uint64_t seed) {
+  using namespace pgdg;
+  std::mt19937_64 rng(seed);
+  P.assign(kNParam, 0.f);
+  auto fill = [&](int off, int rows, int cols, int live_rows, int live_cols, int fan_in,
+                  int fan_out) {
+    const double a = std::sqrt(6.0 / (fan_in + fan_out));
+    std::uniform_real_distribution<double> U(-a, a);
+    for (int r = 0; r < rows; r++)
+      for (int c = 0; c < cols; c++) {
+        double v = U(rng);
+        P[off + r * cols + c] = (r < live_rows && c < live_cols) ? (float)v : 0.f;
+      }
+  };
+  fill(oW0, kInPad, kH, kIn, kH, kIn, kH);
+  fill(oW1, kH, kH, kH, kH, kH, kH);
+  fill(oW2, kH, kH, kH, kH, kH, kH);
+  fill(oW3, kH, kOutPad, kH, kOut, kH, kOut);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgdg_mlp_create(pgdg_mlp** out) {
+  using namespace pgdg;
+  if (!out) return PGDG_ERR_ARG;
+  pgdg_mlp* m = new (std::nothrow) pgdg_mlp();
+  if (!m) return PGDG_ERR_CAPACITY;
+  auto fail = [&](int r) {
+    pgdg_mlp_destroy(m);
+    return r;
+  };
+  if (hipMalloc((void**)&m->params, sizeof(float) * kNParam) != hipSuccess ||
+      hipMalloc((void**)&m->m, sizeof(float) * kNParam) != hipSuccess ||
+      hipMalloc((void**)&m->v, sizeof(float) * kNParam) != hipSuccess ||
+      hipMalloc((void**)&m->grad, sizeof(float) * kNParam) != hipSuccess ||
+      hipMalloc((void**)&m->loss, sizeof(float) * 4) != hipSuccess)
+    return fail(PGDG_ERR_HIP);
+  std::vector<float> P;
+  init_params(P, 0x5EED0004ull);
+  if (hipMemcpy(m->params, P.data(), sizeof(float) * kNParam, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(PGDG_ERR_HIP);
+  for (int k = 0; k < kIn; k++) {
+    m->norm.feat_mu[k] = 0.f;
+    m->norm.feat_sd[k] = 1.f;
+  }
+  for (int k = 0; k < kOut; k++) {
+    m->norm.out_mu[k] = 0.f;
+    m->norm.out_sd[k] = 1.f;
+  }
+  *out = m;
+  return PGDG_OK;
+}
+
+int pgdg_mlp_destroy(pgdg_mlp* m) {
+  if (!m) return PGDG_ERR_ARG;
+  void* ptrs[] = {m->params, m->m, m->v, m->grad, m->part, m->lpart, m->loss, m->guess};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete m;
+  return PGDG_OK;
+}
+
+int pgdg_mlp_get_params(pgdg_mlp* m, float* params, pgdg_mlp_norm* norm) {
+  if (!m) return PGDG_ERR_ARG;
+  if (params)
+    PGDG_HIP_TRY(hipMemcpy(params, m->params, sizeof(float) * pgdg::kNParam, hipMemcpyDeviceToHost));
+  if (norm) *norm = m->norm;
+  return PGDG_OK;
+}
+
+int pgdg_mlp_set_params(pgdg_mlp* m, const float* params, const pgdg_mlp_norm* norm) {
+  using namespace pgdg;
+  if (!m) return PGDG_ERR_ARG;
+  if (params) {
+    std::vector<float> P(params, params + kNParam);
+    for (int p = 0; p < kNParam; p++)
+      if (!param_live(p)) P[p] = 0.f;
+    PGDG_HIP_TRY(hipMemcpy(m->params, P.data(), sizeof(float) * kNParam, hipMemcpyHostToDevice));
+  }
+  if (norm) {
+    for (int k = 0; k < kIn; k++)
+      if (!(norm->feat_sd[k] > 0)) return PGDG_ERR_ARG;
+    for (int k = 0; k < kOut; k++)
+      if (!(norm->out_sd[k] > 0)) return PGDG_ERR_ARG;
+    m->norm = *norm;
+  }
+  m->ready = true;
+  return PGDG_OK;
+}
+
+int pgdg_mlp_forward(pgdg_mlp* m, const float* x, float* y, int64_t n, void* stream) {
+  using namespace pgdg;
+  if (!m || n < 0 || (n > 0 && (!x || !y))) return PGDG_ERR_ARG;
+  if (n == 0) return PGDG_OK;
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kThreads), 0,
+                     as_stream(stream), m->params, x, n, y);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+int pgdg_mlp_loss_grad(pgdg_mlp* m, const float* x, const float* t, int64_t n, float* grad,
+                       float* loss, void* stream) {
+  using namespace pgdg;
+  if (!m || !x || !t || !grad || n <= 0 || n % kTile) return PGDG_ERR_ARG;
+  if (n / kTile > (1 << 20)) return PGDG_ERR_ARG;
+  return launch_fwd_bwd(m, x, t, 0, n, grad, loss ? loss : m->loss, as_stream(stream));
+}
+
+int pgdg_mlp_train(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair* eos,
+                   const pgdg_mlp_train_opts* o, float* loss_hist, int hist_cap, int* steps,
+                   void* stream) {
+  using namespace pgdg;
+  if (!m || !in || !eos || !o || n < kTile) return PGDG_ERR_ARG;
+  if (o->epochs < 1 || o->batch < kTile || o->batch % kTile || !(o->lr > 0)) return PGDG_ERR_ARG;
+  if (o->variant != MGFM_S3 && o->variant != MGFM_T2) return PGDG_ERR_ARG;
+  if (hist_cap < 0 || (hist_cap > 0 && !loss_hist)) return PGDG_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  const Eos ml{eos->gamma_l, eos->pinf_l}, mr{eos->gamma_r, eos->pinf_r};
+  const int64_t nt = n / o->batch * o->batch;  // whole batches
+  if (nt == 0) return PGDG_ERR_ARG;
+  // 1. label with the exact solve
+  double* lab = nullptr;
+  float *x = nullptr, *t = nullptr, *hist = nullptr;
+  double* mom = nullptr;
+  int rc = PGDG_OK;
+  const int mblocks = 1024;
+  constexpr int NV = kIn + kOut;
+  auto cleanup = [&]() {
+    void* ptrs[] = {lab, x, t, hist, mom};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+  };
+#define MLP_TRY(expr)                   \
+  do {                                  \
+    hipError_t e_ = (expr);             \
+    if (e_ != hipSuccess) {             \
+      cleanup();                        \
+      return PGDG_ERR_HIP;              \
+    }                                   \
+  } while (0)
+  MLP_TRY(hipMalloc((void**)&lab, sizeof(double) * 3 * (size_t)n));
+  MLP_TRY(hipMalloc((void**)&x, sizeof(float) * kInPad * (size_t)n));
+  MLP_TRY(hipMalloc((void**)&t, sizeof(float) * kOut * (size_t)n));
+  MLP_TRY(hipMalloc((void**)&mom, sizeof(double) * 2 * NV * mblocks));
+  {
+    pgdg_mgfm_opts lo{o->variant, PGDG_TOL_RELATIVE, 1e-12, 100};
+    rc = pgdg_mgfm_batch(in, lab, nullptr, n, eos, &lo, nullptr, stream);
+    if (rc != PGDG_OK) {
+      cleanup();
+      return rc;
+    }
+  }
+  // 2. normalisation from the labelled set (block sums added in block order)
+  hipLaunchKernelGGL(k_mlp_moments, dim3(mblocks), dim3(kThreads), 0, st, in, lab, n, ml, mr, mom);
+  std::vector<double> hm((size_t)2 * NV * mblocks);
+  MLP_TRY(hipMemcpyAsync(hm.data(), mom, sizeof(double) * hm.size(), hipMemcpyDeviceToHost, st));
+  MLP_TRY(hipStreamSynchronize(st));
+  double s1[NV] = {0}, s2[NV] = {0};
+  for (int b = 0; b < mblocks; b++)
+    for (int q = 0; q < NV; q++) {
+      s1[q] += hm[(size_t)b * 2 * NV + q];
+      s2[q] += hm[(size_t)b * 2 * NV + NV + q];
+    }
+  pgdg_mlp_norm nm{};
+  for (int q = 0; q < NV; q++) {
+    const double mu = s1[q] / (double)n;
+    const double var = std::max(0.0, s2[q] / (double)n - mu * mu);
+    double sd = std::sqrt(var);
+    if (!(sd > 1e-6 * std::max(1.0, std::fabs(mu)))) sd = 1.0;
+    if (q < kIn) {
+      nm.feat_mu[q] = (float)mu;
+      nm.feat_sd[q] = (float)sd;
+    } else {
+      nm.out_mu[q - kIn] = (float)mu;
+      nm.out_sd[q - kIn] = (float)sd;
+    }
+  }
+  m->norm = nm;
+  hipLaunchKernelGGL(k_mlp_build_set, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, lab,
+                     n, ml, mr, nm, x, t);
+  // 3. parameters
+  std::vector<float> P;
+  init_params(P, o->seed);
+  MLP_TRY(hipMemcpyAsync(m->params, P.data(), sizeof(float) * kNParam, hipMemcpyHostToDevice, st));
+  MLP_TRY(hipMemsetAsync(m->m, 0, sizeof(float) * kNParam, st));
+  MLP_TRY(hipMemsetAsync(m->v, 0, sizeof(float) * kNParam, st));
+  // 4. Adam over minibatches, cosine-decayed step size
+  const int64_t per_epoch = nt / o->batch;
+  const int64_t total = per_epoch * o->epochs;
+  const int nrec = hist_cap > 0 ? (int)std::min<int64_t>(total, hist_cap) : 0;
+  if (nrec > 0) MLP_TRY(hipMalloc((void**)&hist, sizeof(float) * nrec));
+  int64_t step = 0;
+  for (int ep = 0; ep < o->epochs; ep++)
+    for (int64_t bi = 0; bi < per_epoch; bi++, step++) {
+      float* lslot = (step < nrec) ? hist + step : m->loss;
+      rc = launch_fwd_bwd(m, x, t, bi * o->batch, o->batch, m->grad, lslot, st);
+      if (rc != PGDG_OK) {
+        cleanup();
+        return rc;
+      }
+      const double frac = (double)step / (double)std::max<int64_t>(1, total - 1);
+      const double lr = o->lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * frac)));
+      const double c1 = 1.0 - std::pow(0.9, (double)(step + 1));
+      const double c2 = 1.0 - std::pow(0.999, (double)(step + 1));
+      hipLaunchKernelGGL(k_mlp_adam, dim3((kNParam + 255) / 256), dim3(256), 0, st, m->params, m->m,
+                         m->v, m->grad, (float)lr, (float)c1, (float)c2);
+    }
+  MLP_TRY(hipGetLastError());
+  if (nrec > 0)
+    MLP_TRY(hipMemcpyAsync(loss_hist, hist, sizeof(float) * nrec, hipMemcpyDeviceToHost, st));
+  MLP_TRY(hipStreamSynchronize(st));
+  if (steps) *steps = (int)total;
+  cleanup();
+#undef MLP_TRY
+  m->ready = true;
+  return PGDG_OK;
+}
+
+int pgdg_mlp_guess(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair* eos,
+                   double* pstar0, void* stream) {
+  using namespace pgdg;
+  if (!m || !eos || n < 0 || (n > 0 && (!in || !pstar0))) return PGDG_ERR_ARG;
+  if (!m->ready) return PGDG_ERR_STATE;
+  if (n == 0) return PGDG_OK;
+  const Eos ml{eos->gamma_l, eos->pinf_l}, mr{eos->gamma_r, eos->pinf_r};
+  hipLaunchKernelGGL(k_mlp_guess, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kThreads), 0,
+                     as_stream(stream), m->params, m->norm, in, n, ml, mr, pstar0);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+int pgdg_mgfm_batch_learned(pgdg_mlp* m, const double* in, double* out, int32_t* iters, int64_t n,
+                            const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts, void* stream) {
+  if (!m || !eos || !opts || n < 0) return PGDG_ERR_ARG;
+  if (n == 0) return PGDG_OK;
+  if (n > m->guess_cap) {
+    if (m->guess) PGDG_HIP_TRY(hipFree(m->guess));
+    m->guess = nullptr;
+    m->guess_cap = 0;
+    PGDG_HIP_TRY(hipMalloc((void**)&m->guess, sizeof(double) * (size_t)n));
+    m->guess_cap = n;
+  }
+  PGDG_TRY(pgdg_mlp_guess(m, in, n, eos, m->guess, stream));
+  return pgdg_mgfm_batch(in, out, iters, n, eos, opts, m->guess, stream);
+}
+
+}  // extern "C"

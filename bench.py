@@ -76,33 +76,72 @@ def cpu_baseline(sample=(48, 24, 24), steps=2):
                       f"steps 2..{steps + 1}, same initial condition, oracle FP64 single thread"}
 
 
-def mgfm_bench(n=10_000_000):
+def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
     """Batched MGFM on n stiffened-gas interface face pairs resident on the
     device: faces/s and Newton iterations per face for the reference
-    tolerance and for 1e-12 relative on p*."""
+    tolerance and for 1e-12 relative on p*, from the reference's initial
+    guess and from the learned guess (MLP trained on n_train solver-labelled
+    faces, seed 0x5EED0004; training time reported)."""
     import numpy as np
     import pgdg_p2_amd as P
     from pgdg_p2_amd import workloads
+    from pgdg_p2_amd.mlp import Mlp
+    eos = workloads.MGFM_EOS
+    gl, pil, gr, pir = eos
+    res = {"faces": n, "train_faces": n_train, "epochs": epochs}
+    d_tr = P.DeviceArray.from_numpy(workloads.mgfm_faces(n_train, seed=0x5EED0004))
+    mlp = Mlp()
+    P.device_sync()
+    t0 = time.perf_counter()
+    hist = mlp.train(d_tr, n_train, eos, epochs=epochs, batch=4096, lr=3e-3)
+    res["train_s"] = time.perf_counter() - t0
+    res["train_loss_first_last"] = [float(hist[:32].mean()), float(hist[-32:].mean())]
+    d_tr.free()
     inp = P.DeviceArray.from_numpy(workloads.mgfm_faces(n))
     out = P.DeviceArray((3, n), np.float64)
     its = P.DeviceArray((n,), np.int32)
-    gl, pil, gr, pir = workloads.MGFM_EOS
-    res = {}
+    tm = P.EventTimer()
+    reps = 5
+    p_ref = None
     for name, mode, it in (("reference_tol", P.TOL_REFERENCE, 50), ("rel_1e-12", P.TOL_RELATIVE, 100)):
-        P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12, max_iter=it)
-        P.device_sync()
-        reps = 5
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12,
-                             max_iter=it)
-        P.device_sync()
-        el = (time.perf_counter() - t0) / reps
-        res[name] = {"faces_per_s": n / el, "newton_its_per_face": float(its.numpy().mean()),
-                     "ms": el * 1e3}
-    res["faces"] = n
-    for d in (inp, out, its):
+        for guess in ("plain", "learned"):
+            def run():
+                if guess == "plain":
+                    P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12,
+                                     max_iter=it)
+                else:
+                    mlp.mgfm_dev(inp, out, its, n, eos, tol_mode=mode, rtol=1e-12, max_iter=it)
+            run()
+            tm.start()
+            for _ in range(reps):
+                run()
+            tm.stop()
+            ms = tm.ms() / reps
+            o, itv = out.numpy(), its.numpy()
+            r = {"faces_per_s": n / (ms * 1e-3), "newton_its_per_face": float(itv.mean()), "ms": ms}
+            if mode == P.TOL_RELATIVE:
+                if guess == "plain":
+                    p_ref = o[1].copy()
+                else:
+                    r["pstar_max_rel_diff_vs_plain"] = float(np.max(np.abs(o[1] - p_ref) /
+                                                                    np.abs(p_ref)))
+            res[f"{name}_{guess}"] = r
+    # the network alone: one inference pass over the n faces
+    g0 = P.DeviceArray((n,), np.float64)
+    tm.start()
+    for _ in range(reps):
+        mlp.guess_dev(inp, n, eos, g0)
+    tm.stop()
+    ms = tm.ms() / reps
+    res["mlp_inference"] = {"ms": ms, "faces_per_s": n / (ms * 1e-3),
+                            "bf16_mfma_TFLOPs": 17920.0 * n / (ms * 1e-3) / 1e12}
+    for key in ("reference_tol", "rel_1e-12"):
+        res[f"{key}_speedup_learned"] = res[f"{key}_learned"]["faces_per_s"] / \
+            res[f"{key}_plain"]["faces_per_s"]
+    tm.close()
+    for d in (inp, out, its, g0):
         d.free()
+    mlp.close()
     return res
 
 
@@ -160,6 +199,7 @@ def main():
     ap.add_argument("--no-mls", action="store_true")
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
+    ap.add_argument("--only-mgfm", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel over this many MiB (byte-counter "
@@ -185,6 +225,9 @@ def main():
               flush=True)
     if args.only_mls:
         print(json.dumps({"mls": mls_bench(args.mls_n)}))
+        return
+    if args.only_mgfm:
+        print(json.dumps({"mgfm": mgfm_bench()}))
         return
 
     from pgdg_p2_amd import cases

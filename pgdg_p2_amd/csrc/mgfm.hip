@@ -9,24 +9,52 @@
 
 namespace pgdg {
 
+// Each wavefront owns a chunk of kMgfmChunk consecutive faces and keeps all
+// 64 lanes busy: a lane whose face has converged writes it out and takes
+// the next face of the chunk (ballot + prefix count, no atomics), so the
+// wavefront's time is the chunk's total Newton work rather than 64 times
+// its slowest face.  Every face runs exactly the iterations of mgfm_solve.
+constexpr int kMgfmChunk = 2048;
+
 __global__ void __launch_bounds__(256) mgfm_batch_kernel(
     const double* __restrict__ in, double* __restrict__ out, int32_t* __restrict__ iters,
     int64_t n, Eos ml, Eos mr, int variant, int tol_mode, double rtol, int max_iter,
     const double* __restrict__ pstar0) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    double ul = in[i];
-    double ur = in[n + i];
-    double pl = in[2 * n + i];
-    double pr = in[3 * n + i];
-    double rl = in[4 * n + i];
-    double rr = in[5 * n + i];
-    double g = pstar0 ? pstar0[i] : -1.0;
-    MgfmOut o = mgfm_solve(ul, ur, pl, pr, rl, rr, ml, mr, variant, tol_mode, rtol, max_iter, g);
-    out[i] = o.rho_l;
-    out[n + i] = o.pstar;
-    out[2 * n + i] = o.u_l;
-    if (iters) iters[i] = o.iters;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t c0 = wave * kMgfmChunk;
+  if (c0 >= n) return;
+  const int64_t c1 = c0 + kMgfmChunk < n ? c0 + kMgfmChunk : n;
+  int64_t next = c0;  // wave-uniform
+  int64_t i = -1;
+  bool done = false;
+  MgfmState S;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (;;) {
+    const bool need = !done && i < 0;
+    const unsigned long long needm = __ballot(need);
+    if (needm) {
+      if (need) {
+        const int64_t j = next + __popcll(needm & below);
+        if (j < c1) {
+          i = j;
+          mgfm_begin(S, in[j], in[n + j], in[2 * n + j], in[3 * n + j], in[4 * n + j],
+                     in[5 * n + j], variant, pstar0 ? pstar0[j] : -1.0);
+        } else {
+          done = true;
+        }
+      }
+      next += __popcll(needm);
+    }
+    if (__ballot(!done) == 0ull) break;
+    if (i >= 0 && !mgfm_iterate(S, ml, mr, variant, tol_mode, rtol, max_iter)) {
+      const MgfmOut o = mgfm_end(S, ml, variant, tol_mode);
+      out[i] = o.rho_l;
+      out[n + i] = o.pstar;
+      out[2 * n + i] = o.u_l;
+      if (iters) iters[i] = o.iters;
+      i = -1;
+    }
   }
 }
 
@@ -45,7 +73,8 @@ extern "C" int pgdg_mgfm_batch(const double* in, double* out, int32_t* iters, in
   Eos ml{eos->gamma_l, eos->pinf_l};
   Eos mr{eos->gamma_r, eos->pinf_r};
   const int block = 256;
-  unsigned grid = grid_for(n, block, 256 * 64);
+  const int64_t waves = (n + kMgfmChunk - 1) / kMgfmChunk;
+  const unsigned grid = (unsigned)((waves + block / 64 - 1) / (block / 64));
   hipLaunchKernelGGL(mgfm_batch_kernel, dim3(grid), dim3(block), 0, as_stream(stream), in, out,
                      iters, n, ml, mr, opts->variant, opts->tol_mode, opts->rtol, opts->max_iter,
                      pstar0);

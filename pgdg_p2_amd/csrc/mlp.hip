@@ -46,14 +46,16 @@ __host__ __device__ inline bool param_live(int p) {
   return true;
 }
 
-// FP32 -> BF16, round to nearest even (NaN kept quiet)
-__device__ inline unsigned short f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (the conversion instruction of gfx950)
+__device__ inline unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal
+// (relative error ~1e-6, far below the BF16 rounding the result takes).
+__device__ inline float tanh_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(fminf(x, 40.f) * 2.8853900817779268f);  // 2x / ln 2
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
 
 // One 16x16 tile of C = A * B over K (a multiple of 16) from BF16 LDS
 // operands with element strides: A(m, k) = A[m * ars + k * acs],
@@ -121,7 +123,7 @@ __device__ inline void hidden(const unsigned short* Hin, int kin, const unsigned
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
-      Hout[row * kH + col] = f2bf(tanhf(acc[i] + b[col]));
+      Hout[row * kH + col] = f2bf(tanh_fast(acc[i] + b[col]));
     }
   }
 }
@@ -151,11 +153,14 @@ __device__ inline void face_features(const double* __restrict__ in, int64_t n, i
                                      Eos mr, const pgdg_mlp_norm& nm, float* f) {
   const double ul = in[i], ur = in[n + i], pl = in[2 * n + i], pr = in[3 * n + i];
   const double rl = in[4 * n + i], rr = in[5 * n + i];
-  double raw[kIn] = {log(fmax(pl, 1e-30)), log(fmax(pr, 1e-30)), log(fmax(rl, 1e-30)),
-                     log(fmax(rr, 1e-30)), ul, ur, ml.gamma, log1p(fmax(ml.pinf, 0.0)),
-                     mr.gamma, log1p(fmax(mr.pinf, 0.0))};
+  // the features enter the network as BF16: FP32 logarithms are ample
+  const float raw[kIn] = {logf(fmaxf((float)pl, 1e-30f)), logf(fmaxf((float)pr, 1e-30f)),
+                          logf(fmaxf((float)rl, 1e-30f)), logf(fmaxf((float)rr, 1e-30f)),
+                          (float)ul, (float)ur, (float)ml.gamma,
+                          (float)log1p(fmax(ml.pinf, 0.0)), (float)mr.gamma,
+                          (float)log1p(fmax(mr.pinf, 0.0))};
 #pragma unroll
-  for (int k = 0; k < kIn; k++) f[k] = (float)((raw[k] - nm.feat_mu[k]) / nm.feat_sd[k]);
+  for (int k = 0; k < kIn; k++) f[k] = (raw[k] - nm.feat_mu[k]) / nm.feat_sd[k];
 #pragma unroll
   for (int k = kIn; k < kInPad; k++) f[k] = 0.f;
 }
@@ -163,33 +168,168 @@ __device__ inline void face_features(const double* __restrict__ in, int64_t n, i
 // ---------------------------------------------------------------------------
 // inference: p*0 for every face
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads) k_mlp_guess(const float* __restrict__ P, pgdg_mlp_norm nm,
-                                                        const double* __restrict__ in, int64_t n,
-                                                        Eos ml, Eos mr, double* __restrict__ pstar0) {
-  __shared__ FwdLds s;
-  load_params(P, s);
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  if (threadIdx.x < kTile) {
-    float f[kInPad];
-    const int64_t i = base + threadIdx.x;
-    if (i < n) face_features(in, n, i, ml, mr, nm, f);
-    else
-      for (int k = 0; k < kInPad; k++) f[k] = 0.f;
+// Inference.  Each layer is computed transposed, H^T = W^T X^T: the MFMA
+// result tile of a lane (column = face, rows = 4 consecutive units) is then
+// the B-operand layout the next layer needs (k = those units, n = face), so
+// activations stay in registers from the features to the output.  Hidden
+// layers use the K = 32 MFMA of gfx950: a lane's 8 k values are the 4 units
+// it holds of two result tiles, and the weights (A operand, W^T) are loaded
+// with the same k order, so no lane exchange is needed.  Weights and biases
+// are loaded once per wavefront; a wavefront runs 16-face tiles with no LDS
+// and no barrier, prefetching the next tile's inputs.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short s16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ inline bf16x4_t pack4(float a, float b, float c, float d) {
+  bf16x4_t r;
+  r[0] = (short)f2bf(a);
+  r[1] = (short)f2bf(b);
+  r[2] = (short)f2bf(c);
+  r[3] = (short)f2bf(d);
+  return r;
+}
+__device__ inline bf16x8_t cat8(bf16x4_t lo, bf16x4_t hi) {
+  s16x8_t r;
 #pragma unroll
-    for (int k = 0; k < kInPad; k++) s.X[threadIdx.x * kInPad + k] = f2bf(f[k]);
+  for (int q = 0; q < 4; q++) {
+    r[q] = lo[q];
+    r[4 + q] = hi[q];
   }
-  __syncthreads();
-  forward(s, [&](int row, int col, float y) {
-    if (col != 0) return;
-    const int64_t i = base + row;
-    if (i >= n) return;
-    const double pl = in[2 * n + i], pr = in[3 * n + i];
-    const double pm = pl >= pr ? pl : pr;
-    const double lr = (double)y * nm.out_sd[0] + nm.out_mu[0];
-    double g = pm * exp(lr);
-    if (!(g > 0) || !(g < 1e300)) g = pm;
-    pstar0[i] = g;
-  });
+  return __builtin_bit_cast(bf16x8_t, r);
+}
+
+// A(m = unit, k) = W[k][unit] of a [K][ldw] weight for m-tile mt, 16-wide
+// k-step ks (K = 16 MFMA layout: lane group g holds k = 4g .. 4g+3)
+__device__ inline bf16x4_t load_wt(const float* __restrict__ W, int ldw, int mt, int ks) {
+  const int l = threadIdx.x & 63;
+  const int unit = mt * 16 + (l & 15), k = ks * 16 + (l >> 4) * 4;
+  return pack4(W[(k + 0) * ldw + unit], W[(k + 1) * ldw + unit], W[(k + 2) * ldw + unit],
+               W[(k + 3) * ldw + unit]);
+}
+// K = 32 step covering input tiles 2j and 2j+1: k values 0..3 of the lane
+// are units 4g.. of tile 2j, 4..7 are units 4g.. of tile 2j+1
+__device__ inline bf16x8_t load_wt32(const float* __restrict__ W, int ldw, int mt, int j) {
+  return cat8(load_wt(W, ldw, mt, 2 * j), load_wt(W, ldw, mt, 2 * j + 1));
+}
+
+// fx: the standardised EOS features 6..9 (the same for every face)
+__global__ void __launch_bounds__(kThreads) k_mlp_guess(const float* __restrict__ P, pgdg_mlp_norm nm,
+                                                        float4 fx, const double* __restrict__ in,
+                                                        int64_t n, double* __restrict__ pstar0) {
+  const int l = threadIdx.x & 63, g = l >> 4, f = l & 15;
+  bf16x4_t A0[4];
+  bf16x8_t A1[4][2], A2[4][2], A3[2];
+  float bb0[4][4], bb1[4][4], bb2[4][4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    A0[mt] = load_wt(P + oW0, kH, mt, 0);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      A1[mt][j] = load_wt32(P + oW1, kH, mt, j);
+      A2[mt][j] = load_wt32(P + oW2, kH, mt, j);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int unit = mt * 16 + g * 4 + q;
+      bb0[mt][q] = P[oB0 + unit];
+      bb1[mt][q] = P[oB1 + unit];
+      bb2[mt][q] = P[oB2 + unit];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; j++) A3[j] = load_wt32(P + oW3, kOutPad, 0, j);
+  const float b3 = P[oB3];
+  const int64_t ntile = (n + 15) / 16;
+  const int64_t wstride = (int64_t)gridDim.x * (kThreads / 64);
+  // inputs of a lane: group 0 (pl, pr, rhol, rhor), group 1 (ul, ur); the
+  // next tile's are loaded while the current one is computed
+  auto load_in = [&](int64_t t, double* v) {
+    const int64_t i = t * 16 + f;
+    v[0] = v[1] = v[2] = v[3] = 1.0;
+    if (t >= ntile || i >= n) return;
+    if (g == 0) {
+      v[0] = in[2 * n + i];
+      v[1] = in[3 * n + i];
+      v[2] = in[4 * n + i];
+      v[3] = in[5 * n + i];
+    } else if (g == 1) {
+      v[0] = in[i];
+      v[1] = in[n + i];
+    }
+  };
+  int64_t tile = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  double cur[4], nxt[4];
+  load_in(tile, cur);
+  for (; tile < ntile; tile += wstride) {
+    load_in(tile + wstride, nxt);
+    const int64_t i = tile * 16 + f;
+    const bool live = i < n;
+    // features 4g .. 4g+3 of face f
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (live) {
+      if (g == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          x[q] = (logf(fmaxf((float)cur[q], 1e-30f)) - nm.feat_mu[q]) / nm.feat_sd[q];
+      } else if (g == 1) {
+        x[0] = ((float)cur[0] - nm.feat_mu[4]) / nm.feat_sd[4];
+        x[1] = ((float)cur[1] - nm.feat_mu[5]) / nm.feat_sd[5];
+        x[2] = fx.x;
+        x[3] = fx.y;
+      } else if (g == 2) {
+        x[0] = fx.z;
+        x[1] = fx.w;
+      }
+    }
+    const bf16x4_t bx = pack4(x[0], x[1], x[2], x[3]);
+    bf16x4_t h[4];
+    {
+      f32x4_t acc[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        acc[mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0[mt], bx, acc[mt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+        h[mt] = pack4(tanh_fast(acc[mt][0] + bb0[mt][0]), tanh_fast(acc[mt][1] + bb0[mt][1]),
+                      tanh_fast(acc[mt][2] + bb0[mt][2]), tanh_fast(acc[mt][3] + bb0[mt][3]));
+    }
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+      const bf16x8_t b01 = cat8(h[0], h[1]), b23 = cat8(h[2], h[3]);
+      f32x4_t acc[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        acc[mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer ? A2[mt][0] : A1[mt][0], b01,
+                                                          acc[mt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++)
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer ? A2[mt][1] : A1[mt][1], b23,
+                                                          acc[mt], 0, 0, 0);
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const float* bb = layer ? bb2[mt] : bb1[mt];
+        h[mt] = pack4(tanh_fast(acc[mt][0] + bb[0]), tanh_fast(acc[mt][1] + bb[1]),
+                      tanh_fast(acc[mt][2] + bb[2]), tanh_fast(acc[mt][3] + bb[3]));
+      }
+    }
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[0], cat8(h[0], h[1]), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[1], cat8(h[2], h[3]), acc, 0, 0, 0);
+    // lanes of group 0 hold output unit 0 (ln p* ratio) of face f in acc[0]
+    if (g == 0 && live) {
+      const double pm = cur[0] >= cur[1] ? cur[0] : cur[1];
+      const float lr = (acc[0] + b3) * nm.out_sd[0] + nm.out_mu[0];
+      double gs = pm * (double)expf(lr);
+      if (!(gs > 0) || !(gs < 1e300)) gs = pm;
+      pstar0[i] = gs;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) cur[q] = nxt[q];
+  }
 }
 
 // forward on standardised inputs (y[n][2] standardised)
@@ -684,9 +824,16 @@ int pgdg_mlp_guess(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
   if (!m || !eos || n < 0 || (n > 0 && (!in || !pstar0))) return PGDG_ERR_ARG;
   if (!m->ready) return PGDG_ERR_STATE;
   if (n == 0) return PGDG_OK;
-  const Eos ml{eos->gamma_l, eos->pinf_l}, mr{eos->gamma_r, eos->pinf_r};
-  hipLaunchKernelGGL(k_mlp_guess, dim3((unsigned)((n + kTile - 1) / kTile)), dim3(kThreads), 0,
-                     as_stream(stream), m->params, m->norm, in, n, ml, mr, pstar0);
+  const pgdg_mlp_norm& nm = m->norm;
+  const double c[4] = {eos->gamma_l, std::log1p(std::max(eos->pinf_l, 0.0)), eos->gamma_r,
+                       std::log1p(std::max(eos->pinf_r, 0.0))};
+  float fx4[4];
+  for (int k = 0; k < 4; k++) fx4[k] = ((float)c[k] - nm.feat_mu[6 + k]) / nm.feat_sd[6 + k];
+  const float4 fx = make_float4(fx4[0], fx4[1], fx4[2], fx4[3]);
+  const int64_t ntile = (n + 15) / 16;
+  const unsigned grid = (unsigned)std::min<int64_t>((ntile + 3) / 4, 256 * 8);
+  hipLaunchKernelGGL(k_mlp_guess, dim3(grid), dim3(kThreads), 0, as_stream(stream), m->params, nm, fx,
+                     in, n, pstar0);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

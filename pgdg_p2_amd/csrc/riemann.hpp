@@ -57,12 +57,19 @@ struct MgfmOut {
 enum { MGFM_S3 = 0, MGFM_T2 = 1 };
 enum { TOL_REF = 0, TOL_REL = 1 };
 
-// One MGFM solve.  `guess` < 0 selects the variant's own initial p*.
-// S3 variant: S3:mpm.cpp:882-926.  T2 variant: T2:mpm.cpp:1373-1416.
-PGDG_HD MgfmOut mgfm_solve(double ul, double ur, double pl, double pr, double rhol, double rhor,
-                           Eos ml, Eos mr, int variant, int tol_mode, double rtol, int max_iter,
-                           double guess) {
-  MgfmOut o;
+// One MGFM solve, split into begin / iterate / end so that a batch kernel
+// can advance many faces one Newton iteration at a time; mgfm_solve runs the
+// three in sequence.  S3 variant: S3:mpm.cpp:882-926.  T2 variant:
+// T2:mpm.cpp:1373-1416.
+struct MgfmState {
+  double ul, ur, pl, pr, rhol, rhor;
+  double tol, ps, rls, wl, wr, usl, f;
+  int nn;
+};
+
+// `guess` < 0 selects the variant's own initial p*.
+PGDG_HD void mgfm_begin(MgfmState& S, double ul, double ur, double pl, double pr, double rhol,
+                        double rhor, int variant, double guess) {
   if (variant == MGFM_S3) {
     pl = pl >= 1e-10 ? pl : 1e-10;   // AMAX1(pl, 1e-10)
     pr = pr >= 1e-10 ? pr : 1e-10;
@@ -72,64 +79,82 @@ PGDG_HD MgfmOut mgfm_solve(double ul, double ur, double pl, double pr, double rh
     if (pl < 0) pl = 0;
     if (pr < 0) pr = 0;
   }
+  S.ul = ul;
+  S.ur = ur;
+  S.pl = pl;
+  S.pr = pr;
+  S.rhol = rhol;
+  S.rhor = rhor;
   double du = fabs(ul - ur);
-  double tol = 1e-8 >= 1e-3 * du ? 1e-8 : 1e-3 * du;
-  double ps;
+  S.tol = 1e-8 >= 1e-3 * du ? 1e-8 : 1e-3 * du;
   if (guess >= 0)
-    ps = guess;
+    S.ps = guess;
   else if (variant == MGFM_S3)
-    ps = pl >= pr ? pl : pr;
+    S.ps = pl >= pr ? pl : pr;
   else
-    ps = (pl + pr) * 0.5;
+    S.ps = (pl + pr) * 0.5;
+  S.rls = S.wl = S.wr = S.usl = 0;
+  S.f = 1.;
+  S.nn = 0;
+}
 
-  double rls = 0, wl = 0, wr = 0, usl = 0, f = 1.;
-  int nn = 0;
+// One Newton iteration; true while another is needed.  TOL_REF: stop when
+// |f| (evaluated before the update) is within the reference tolerance;
+// TOL_REL (extension): stop when the update is below rtol * p*.
+PGDG_HD bool mgfm_iterate(MgfmState& S, Eos ml, Eos mr, int variant, int tol_mode, double rtol,
+                          int max_iter) {
+  const double ps = S.ps;
   if (tol_mode == TOL_REF) {
-    do {
-      if (variant == MGFM_S3) {
-        wl = shock_w(rhol, pl, ps, ml);
-        wr = shock_w(rhor, pr, ps, mr);
-        usl = ul - (ps - pl) / wl;
-        rls = shock_rho(rhol, pl, ps, ml);
-      } else {
-        rls = shock_rho(rhol, pl, ps, ml);
-        wl = shock_w(rhol, pl, ps, ml);
-        wr = shock_w(rhor, pr, ps, mr);
-      }
-      f = ul - ur - (ps - pl) / wl - (ps - pr) / wr;
-      double jac = -shock_dk(wl, pl, rhol, ps, ml) - shock_dk(wr, pr, rhor, ps, mr);
-      double dp = -f / jac;
-      if (fabs(dp) > 0.6 * ps) dp = copysign(0.6 * ps, dp);
-      ps += dp;
-      ps = ps >= 1e-8 ? ps : 1e-8;
-      nn++;
-    } while (fabs(f) > tol && nn < max_iter);
-    if (variant == MGFM_T2) usl = ul - (ps - pl) / wl;
+    if (variant == MGFM_S3) {
+      S.wl = shock_w(S.rhol, S.pl, ps, ml);
+      S.wr = shock_w(S.rhor, S.pr, ps, mr);
+      S.usl = S.ul - (ps - S.pl) / S.wl;
+      S.rls = shock_rho(S.rhol, S.pl, ps, ml);
+    } else {
+      S.rls = shock_rho(S.rhol, S.pl, ps, ml);
+      S.wl = shock_w(S.rhol, S.pl, ps, ml);
+      S.wr = shock_w(S.rhor, S.pr, ps, mr);
+    }
   } else {
-    // Relative-tolerance extension: iterate the same Newton map until the
-    // update is below rtol*p*, then evaluate rho*_L and u*_L at the result.
-    double ps_old;
-    do {
-      wl = shock_w(rhol, pl, ps, ml);
-      wr = shock_w(rhor, pr, ps, mr);
-      f = ul - ur - (ps - pl) / wl - (ps - pr) / wr;
-      double jac = -shock_dk(wl, pl, rhol, ps, ml) - shock_dk(wr, pr, rhor, ps, mr);
-      double dp = -f / jac;
-      if (fabs(dp) > 0.6 * ps) dp = copysign(0.6 * ps, dp);
-      ps_old = ps;
-      ps += dp;
-      ps = ps >= 1e-8 ? ps : 1e-8;
-      nn++;
-    } while (fabs(ps - ps_old) > rtol * ps && nn < max_iter);
-    wl = shock_w(rhol, pl, ps, ml);
-    rls = shock_rho(rhol, pl, ps, ml);
-    usl = ul - (ps - pl) / wl;
+    S.wl = shock_w(S.rhol, S.pl, ps, ml);
+    S.wr = shock_w(S.rhor, S.pr, ps, mr);
   }
-  o.rho_l = rls;
-  o.pstar = ps;
-  o.u_l = usl;
-  o.iters = nn;
+  S.f = S.ul - S.ur - (ps - S.pl) / S.wl - (ps - S.pr) / S.wr;
+  double jac = -shock_dk(S.wl, S.pl, S.rhol, ps, ml) - shock_dk(S.wr, S.pr, S.rhor, ps, mr);
+  double dp = -S.f / jac;
+  if (fabs(dp) > 0.6 * ps) dp = copysign(0.6 * ps, dp);
+  double pn = ps + dp;
+  pn = pn >= 1e-8 ? pn : 1e-8;
+  S.ps = pn;
+  S.nn++;
+  if (tol_mode == TOL_REF) return fabs(S.f) > S.tol && S.nn < max_iter;
+  return fabs(pn - ps) > rtol * pn && S.nn < max_iter;
+}
+
+PGDG_HD MgfmOut mgfm_end(MgfmState& S, Eos ml, int variant, int tol_mode) {
+  if (tol_mode == TOL_REF) {
+    if (variant == MGFM_T2) S.usl = S.ul - (S.ps - S.pl) / S.wl;
+  } else {
+    S.wl = shock_w(S.rhol, S.pl, S.ps, ml);
+    S.rls = shock_rho(S.rhol, S.pl, S.ps, ml);
+    S.usl = S.ul - (S.ps - S.pl) / S.wl;
+  }
+  MgfmOut o;
+  o.rho_l = S.rls;
+  o.pstar = S.ps;
+  o.u_l = S.usl;
+  o.iters = S.nn;
   return o;
+}
+
+PGDG_HD MgfmOut mgfm_solve(double ul, double ur, double pl, double pr, double rhol, double rhor,
+                           Eos ml, Eos mr, int variant, int tol_mode, double rtol, int max_iter,
+                           double guess) {
+  MgfmState S;
+  mgfm_begin(S, ul, ur, pl, pr, rhol, rhor, variant, guess);
+  while (mgfm_iterate(S, ml, mr, variant, tol_mode, rtol, max_iter)) {
+  }
+  return mgfm_end(S, ml, variant, tol_mode);
 }
 
 }  // namespace pgdg

@@ -85,3 +85,45 @@ def test_learned_guess_reduces_newton_iterations(cuda):
     assert rel.max() <= 1e-10
     assert it1.mean() < it0.mean() - 2.0
     m.close()
+
+
+@pytest.mark.gpu
+def test_guess_kernel_matches_restatement(cuda):
+    # the inference kernel (transposed weights, persistent tiles) against the
+    # restatement's forward pass on the same standardised features
+    import ctypes as C
+    import pgdg_p2_amd as P
+    from pgdg_p2_amd.mlp import Mlp, MlpNorm
+    eos = workloads.MGFM_EOS
+    n = 5000
+    inp = workloads.mgfm_faces(n, seed=21)
+    m = Mlp()
+    Pm = R.init_params(9).astype(np.float32)
+    nm = MlpNorm()
+    mu = [15.0, 15.0, 3.0, -1.0, 0.0, 0.0, 4.4, 20.0, 1.4, 0.0]
+    sd = [4.0, 4.0, 1.0, 2.0, 300.0, 300.0, 1.0, 1.0, 1.0, 1.0]
+    nm.feat_mu[:], nm.feat_sd[:] = mu, sd
+    nm.out_mu[:], nm.out_sd[:] = [-1.0, 0.0], [2.0, 100.0]
+    m.set_params(Pm, nm)
+    d_in = P.DeviceArray.from_numpy(inp)
+    d_g = P.DeviceArray((n,), np.float64)
+    m.guess_dev(d_in, n, eos, d_g)
+    P.device_sync()
+    g = d_g.numpy()
+    f32 = np.float32
+    raw = np.stack([np.log(np.maximum(inp[2].astype(f32), f32(1e-30))),
+                    np.log(np.maximum(inp[3].astype(f32), f32(1e-30))),
+                    np.log(np.maximum(inp[4].astype(f32), f32(1e-30))),
+                    np.log(np.maximum(inp[5].astype(f32), f32(1e-30))),
+                    inp[0].astype(f32), inp[1].astype(f32),
+                    np.full(n, eos[0], f32), np.full(n, np.log1p(eos[1]), f32),
+                    np.full(n, eos[2], f32), np.full(n, np.log1p(eos[3]), f32)], axis=1)
+    X = np.zeros((n, R.NPAD))
+    X[:, :R.NIN] = (raw - np.array(mu, f32)) / np.array(sd, f32)
+    y, _ = R.forward(Pm.astype(np.float64), X, emulate=True)
+    pm = np.maximum(inp[2], inp[3])
+    gr = pm * np.exp(y[:, 0] * 2.0 - 1.0)
+    rel = np.abs(g - gr) / gr
+    print("guess rel err max", rel.max(), "mean", rel.mean())
+    assert np.median(rel) <= 1e-5 and rel.max() <= 0.05
+    m.close()

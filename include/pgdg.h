@@ -212,7 +212,6 @@ typedef struct pgdg_mls_problem {
  * solved: a singular moment matrix falls back to P1, then P0) or NULL.    */
 int pgdg_mls_ls_fit(const pgdg_mls_problem* p, int precision, double* coef, int32_t* order_used,
                     void* stream);
-
 /* ======================================================================== *
  *  PG-DG time step: case description and host state layout                *
  * ======================================================================== */

@@ -68,21 +68,15 @@ __device__ inline void accumulate(T* acc, T w, const T* phi, const T* v, int nf)
       }
 }
 
-template <bool MIXED>
-__global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __restrict__ coef,
-                                                int32_t* __restrict__ ord) {
-  using T = typename std::conditional<MIXED, float, double>::type;
+// Candidate walk and moment sums of cell c (acc[kNR] in the accumulator
+// type of the form).
+template <bool MIXED, typename T>
+__device__ inline void mls_assemble(const pgdg_mls_problem& p, int64_t c, T* acc) {
   const int n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
-  const int64_t nc = (int64_t)n0 * n1 * n2;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
   const int nf = p.nfields;
   const int ck = (int)(c % n2), cj = (int)((c / n2) % n1), ci = (int)(c / ((int64_t)n1 * n2));
-
-  T acc[kNR];
 #pragma unroll
   for (int q = 0; q < kNR; q++) acc[q] = T(0);
-
   const double inv_dx = 1.0 / p.dx;
   const double Ck = 1.0 / kPiRef / ((p.h * p.h) * p.h) * 21. / 16.;
   const double xc0 = p.dx * (ci + 0.5), xc1 = p.dx * (cj + 0.5), xc2 = p.dx * (ck + 0.5);
@@ -141,10 +135,18 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
     k++;
   }
 
+}
+
+// FP64 factorisation and solves of cell c from its sums (sum(q) returns
+// entry q as FP64); writes coef and the order used.
+template <typename Sum>
+__device__ inline void mls_solve(const pgdg_mls_problem& p, int64_t c, int64_t nc, Sum sum,
+                                 double* __restrict__ coef, int32_t* __restrict__ ord) {
+  const int nf = p.nfields;
   // FP64 Cholesky of the packed matrix in place: L(i, j), i >= j, at tri(j, i)
   double A[kNA];
 #pragma unroll
-  for (int q = 0; q < kNA; q++) A[q] = (double)acc[q];
+  for (int q = 0; q < kNA; q++) A[q] = sum(q);
   double amax = 0;
 #pragma unroll
   for (int a = 0; a < 10; a++) amax = fmax(amax, fabs(A[tri(a, a)]));
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
     double b[10];
 #pragma unroll
     for (int i = 0; i < 10; i++) {
-      double s = (double)acc[kNA + i * 6 + f];
+      double s = sum(kNA + i * 6 + f);
 #pragma unroll
       for (int q = 0; q < i; q++) s -= A[tri(q, i)] * b[q];
       b[i] = i < m ? s / A[tri(i, i)] : 0.0;
@@ -194,11 +196,22 @@ __global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __re
   }
 }
 
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_mls_ls(pgdg_mls_problem p, double* __restrict__ coef,
+                                                int32_t* __restrict__ ord) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  const int64_t nc = (int64_t)p.n[0] * p.n[1] * p.n[2];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  T acc[kNR];
+  mls_assemble<MIXED>(p, c, acc);
+  mls_solve(p, c, nc, [&](int q) { return (double)acc[q]; }, coef, ord);
+}
+
 }  // namespace pgdg
 
-extern "C" int pgdg_mls_ls_fit(const pgdg_mls_problem* p, int precision, double* coef,
-                               int32_t* order_used, void* stream) {
-  using namespace pgdg;
+namespace {
+int mls_check(const pgdg_mls_problem* p, int precision, double* coef) {
   if (!p || !coef) return PGDG_ERR_ARG;
   if (p->nfields < 1 || p->nfields > 6 || p->order < 0 || p->order > 2) return PGDG_ERR_ARG;
   if (p->n[0] < 1 || p->n[1] < 1 || p->n[2] < 1 || !(p->dx > 0) || !(p->h > 0) || !(p->cut > 0))
@@ -206,18 +219,29 @@ extern "C" int pgdg_mls_ls_fit(const pgdg_mls_problem* p, int precision, double*
   if (!p->off || p->np < 0) return PGDG_ERR_ARG;
   const int64_t nc = (int64_t)p->n[0] * p->n[1] * p->n[2];
   if (nc > (int64_t)INT32_MAX * 64) return PGDG_ERR_ARG;
-  const unsigned grid = (unsigned)((nc + 255) / 256);
   if (precision == PGDG_PREC_FP64) {
     if (!p->x || !p->vol || !p->val) return PGDG_ERR_ARG;
-    hipLaunchKernelGGL(k_mls_ls<false>, dim3(grid), dim3(256), 0, as_stream(stream), *p, coef,
-                       order_used);
   } else if (precision == PGDG_PREC_MIXED) {
     if (!p->u || !p->volf || !p->valf) return PGDG_ERR_ARG;
-    hipLaunchKernelGGL(k_mls_ls<true>, dim3(grid), dim3(256), 0, as_stream(stream), *p, coef,
-                       order_used);
   } else {
     return PGDG_ERR_ARG;
   }
+  return PGDG_OK;
+}
+}  // namespace
+
+extern "C" int pgdg_mls_ls_fit(const pgdg_mls_problem* p, int precision, double* coef,
+                               int32_t* order_used, void* stream) {
+  using namespace pgdg;
+  PGDG_TRY(mls_check(p, precision, coef));
+  const int64_t nc = (int64_t)p->n[0] * p->n[1] * p->n[2];
+  const unsigned grid = (unsigned)((nc + 255) / 256);
+  if (precision == PGDG_PREC_FP64)
+    hipLaunchKernelGGL(k_mls_ls<false>, dim3(grid), dim3(256), 0, as_stream(stream), *p, coef,
+                       order_used);
+  else
+    hipLaunchKernelGGL(k_mls_ls<true>, dim3(grid), dim3(256), 0, as_stream(stream), *p, coef,
+                       order_used);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

@@ -126,3 +126,4 @@ def test_mls_empty_neighbourhood(cuda):
     np.testing.assert_array_equal(om, ordr)
     np.testing.assert_array_equal(coef, ref)
     np.testing.assert_allclose(cm, ref, rtol=1e-6, atol=0)
+

@@ -50,3 +50,11 @@ def test_step_parity_triple_point_lattice_1ppc(cuda):
 def test_step_parity_shock_bubble_3d(cuda):
     case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25)
     run_pair(case, st, 6, "sb3d")
+
+
+@pytest.mark.gpu
+def test_step_parity_triple_point_three_phase(cuda):
+    # FldNum = 3 (the reference's Triple_point_3phase: the 2-phase routines
+    # with a third material, T3:mpm_ini.cpp)
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25, nphase=3)
+    run_pair(case, st, 12, "tp2d-3phase")

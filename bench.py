@@ -58,7 +58,26 @@ def stage_bytes(stage: str, case, n: int, nwall: int) -> float:
     return 0.0
 
 
-def cpu_baseline(sample=(48, 24, 24), steps=2):
+TRAFFIC_PROFILE = ROOT / "profiles" / "step_shock_bubble_256x128x128_kernels_and_traffic.txt"
+
+
+def profiled_traffic(kernel: str, size: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
+    calibration), for the full-size workload only; None if absent."""
+    if size != "full" or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 4 and cols[0] == f"`{kernel}`":
+            try:
+                return (float(cols[2]) + float(cols[3])) * 1e9
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(sample=(48, 24, 24), steps=3):
     """The oracle (scalar C++ restatement of the reference step) on a reduced
     grid of the same workload, timed on this host."""
     from oracle.pyoracle_step import OracleSim
@@ -287,9 +306,14 @@ def main():
     per_launch_ms = stages[dom]["ms_total"] / stages[dom]["launches"]
     alg = stage_bytes(dom, case, R["n_mid"], walls)
     achieved = alg / (per_launch_ms * 1e-3) / 1e9 if alg > 0 else None
+    kname = {"pst": "pgdg::k_pst_mixed<3>", "rhs": "pgdg::k_rhs<3>", "p2g": "pgdg::k_p2g<3>",
+             "g2p": "pgdg::k_g2p<3>"}.get(dom, "")
+    traffic = profiled_traffic(kname, args.size)
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None, "alg_bytes_per_launch": alg, "avg_launch_ms": per_launch_ms}
+            "traffic": traffic, "alg_bytes_per_launch": alg, "avg_launch_ms": per_launch_ms}
+    if traffic is not None:
+        roof["traffic_source"] = str(TRAFFIC_PROFILE.relative_to(ROOT))
 
     line = {
         "metric": "P2 DG cell-steps/s + MGFM Newton its/face, mixed vs FP64 plain at equal L1 err",

@@ -557,16 +557,27 @@ __global__ void __launch_bounds__(128) k_limiter(Geo g, CellDev C, const int32_t
 // ---------------------------------------------------------------------------
 // DG right-hand side (S3:particle_flow.cpp:16-405)
 // ---------------------------------------------------------------------------
-// Flux column b of DF = U v^T + P I (+ v_b P in the energy row).
+// Flux column b of DF = U v^T + P I (+ v_b P in the energy row).  b is a
+// runtime axis: it selects through comparisons rather than array indexing,
+// so the operands stay in registers (a runtime index into a local array
+// would place it in scratch memory).
+template <int D>
+__device__ inline double pick_axis(const double* v, int b) {
+  return b == 0 ? v[0] : (b == 1 || D == 2 ? v[1] : v[2]);
+}
+
 template <int D>
 __device__ inline void flux_col(double rho, const double* rv, double E, const double* v, double P,
                                 int b, double* out) {
   constexpr int U = D + 2;
-  out[0] = rho * v[b];
+  const double vb = pick_axis<D>(v, b);
+  out[0] = rho * vb;
 #pragma unroll
-  for (int a = 0; a < D; a++) out[1 + a] = rv[a] * v[b];
-  out[1 + b] += P;
-  out[U - 1] = E * v[b] + v[b] * P;
+  for (int a = 0; a < D; a++) {
+    const double f = rv[a] * vb;
+    out[1 + a] = a == b ? f + P : f;
+  }
+  out[U - 1] = E * vb + vb * P;
 }
 
 template <int D>
@@ -643,7 +654,9 @@ __global__ void __launch_bounds__(128) k_rhs(Geo g, CellDev C, const int32_t* __
       const int ax = fg.axis;
       const double sgn = fg.sgn;
       double nrm[3] = {0, 0, 0};
-      nrm[ax] = sgn;
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        if (a == ax) nrm[a] = sgn;
       double DFn[M][U];
 #pragma unroll
       for (int m = 0; m < M; m++)
@@ -800,7 +813,12 @@ __global__ void __launch_bounds__(128) k_rhs(Geo g, CellDev C, const int32_t* __
             }
           } else if (wall_like) {
 #pragma unroll
-            for (int bi = 0; bi < M; bi++) DFn[bi][1 + ax] += P_i * Nb[bi] * gw;
+            for (int bi = 0; bi < M; bi++) {
+              const double t = P_i * Nb[bi] * gw;
+#pragma unroll
+              for (int a = 0; a < D; a++)
+                if (a == ax) DFn[bi][1 + a] += t;
+            }
           } else {  // S3 inlet / outlet / symmetry: HLL against a ghost state
             double rho_j = rho_i;
             double vj[3] = {0, 0, 0}, rvj[3] = {0, 0, 0};

@@ -420,14 +420,26 @@ __global__ void __launch_bounds__(256) k_timestep(Geo g, PtlDev P, CellDev C,
     ts = amin1(t1, t2);
     vm = Uv;
   }
-  // wavefront reductions (min/max are order independent)
+  // wavefront, then workgroup reductions (min/max are order independent),
+  // one pair of integer atomics per workgroup
   for (int off = 32; off > 0; off >>= 1) {
     double o1 = __shfl_down(ts, off, 64);
     double o2 = __shfl_down(vm, off, 64);
     ts = amin1(ts, o1);
     vm = amax1(vm, o2);
   }
+  __shared__ double red_ts[4], red_vm[4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    red_ts[w] = ts;
+    red_vm[w] = vm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+      ts = amin1(ts, red_ts[q]);
+      vm = amax1(vm, red_vm[q]);
+    }
     if (ts > 0) atomicMin(&sc->red_dt, dbits(ts));
     if (vm >= 0) atomicMax(&sc->red_maxv, dbits(vm));
   }

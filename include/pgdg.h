@@ -367,6 +367,9 @@ int pgdg_get_stream(pgdg_ctx* ctx, void** stream);
 int pgdg_particle_count(pgdg_ctx* ctx, int64_t* n);
 int pgdg_memory_bytes(pgdg_ctx* ctx, int64_t* bytes);
 int pgdg_profile_enable(pgdg_ctx* ctx, int on);
+/* RK stages of pgdg_step / pgdg_run_simulation as HIP graphs (default on;
+ * profiling runs the stages one launch at a time).                       */
+int pgdg_graphs_enable(pgdg_ctx* ctx, int on);
 int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 
 #ifdef __cplusplus

@@ -24,6 +24,14 @@
 
 namespace pgdg {
 
+// counts read by the stage kernels (see step_particles.hip)
+__global__ void k_set_counts(DevScalars* sc, int64_t np, int64_t nent) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->np_dev = np;
+    sc->nent_dev = nent;
+  }
+}
+
 __global__ void k_used_clear(int32_t* used, int nc) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < nc) used[c] = 0;
@@ -119,6 +127,29 @@ struct Ctx {
   int64_t n_split = 0, n_inlet = 0, n_outlet = 0, n_merge = 0;
   bool loaded = false;
   size_t bytes = 0;
+  // RK-stage graphs: one per stage index and per pair of buffers the stage
+  // reads through pointers that alternate (particle slab, bin entries)
+  struct StageGraph {
+    int md;
+    const void *k1, *k2;
+    hipGraphExec_t exec;
+  };
+  static constexpr int kMaxGraphs = 8;
+  StageGraph graphs[kMaxGraphs];
+  int ngraphs = 0;
+  bool use_graphs = true;
+  bool capturing = false;
+  int sync_counts() {
+    hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, sc, np, T.n_ent);
+    PGDG_HIP_TRY(hipGetLastError());
+    return PGDG_OK;
+  }
+  void drop_graphs() {
+    for (int q = 0; q < ngraphs; q++) (void)hipGraphExecDestroy(graphs[q].exec);
+    ngraphs = 0;
+  }
+  template <int D_>
+  int rk_stage(int md);
 
   int init(const pgdg_case* c, int64_t cap, int64_t nwall);
   void release();
@@ -269,6 +300,7 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
 
 void Ctx::release() {
   if (st) (void)hipStreamSynchronize(st);
+  drop_graphs();
   PA.release();
   T.release();
   void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light, mb_slab};
@@ -498,7 +530,8 @@ int Ctx::stage_impl(int fn, int arg) {
     case FN_MEMBERSHIP:
       return launch_in_ele<D_>(g, P, S, np, st);
     case FN_NEIGHBORS:
-      return launch_stage_save<D_>(g, P, S, np, arg, st);
+      if (!capturing) PGDG_TRY(sync_counts());
+      return launch_stage_save<D_>(g, P, S, capturing ? PA.cap : np, arg, sc, st);
     case FN_P2G:
       return launch_p2g<D_>(g, C, P, T.bin_off, T.bin_ent, np + W.n, used, PB, st);
     case FN_LIMITER:
@@ -508,17 +541,19 @@ int Ctx::stage_impl(int fn, int arg) {
     case FN_UPDATE_GRID:
       return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, arg, st);
     case FN_UPDATE_PARTICLE: {
+      if (!capturing) PGDG_TRY(sync_counts());
+      const int64_t n_p = capturing ? PA.cap : np, n_e = capturing ? B.ne_cap : T.n_ent;
       Profiler::Rec r;
       if (prof.on) PGDG_TRY(prof.begin(PS_G2P, st, r));
-      PGDG_TRY(launch_g2p<D_>(g, P, C, S, np, sc, arg, st));
+      PGDG_TRY(launch_g2p<D_>(g, P, C, S, n_p, sc, arg, st));
       if (prof.on) {
         PGDG_TRY(prof.end(r, st));
         PGDG_TRY(prof.begin(PS_PST, st, r));
       }
       if (cs.precision == PGDG_PREC_MIXED)
-        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, T.n_ent, T.bin_off, T.bin_ent, sc, st));
+        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st));
       else
-        PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, T.n_ent, T.bin_off, T.bin_ent, sc, st));
+        PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st));
       if (prof.on) PGDG_TRY(prof.end(r, st));
       return PGDG_OK;
     }
@@ -537,13 +572,18 @@ int Ctx::run_simulation(int64_t n) {
   PGDG_TRY(stage<D_>(FN_SPLIT, 0));
   for (int it = 0; it < 4; it++) PGDG_TRY(stage<D_>(FN_MERGE, 0));
   PGDG_TRY(stage<D_>(FN_MEMBERSHIP, 0));
-  for (int md = 0; md < 2; md++) {
-    PGDG_TRY(stage<D_>(FN_NEIGHBORS, md));
-    if (md == 0) PGDG_TRY(stage<D_>(FN_P2G, 0));
-    PGDG_TRY(stage<D_>(FN_LIMITER, 0));
-    PGDG_TRY(stage<D_>(FN_RHS, 0));
-    PGDG_TRY(stage<D_>(FN_UPDATE_GRID, md));
-    PGDG_TRY(stage<D_>(FN_UPDATE_PARTICLE, md));
+  if (use_graphs && !prof.on) {
+    PGDG_TRY(sync_counts());
+    for (int md = 0; md < 2; md++) PGDG_TRY(rk_stage<D_>(md));
+  } else {
+    for (int md = 0; md < 2; md++) {
+      PGDG_TRY(stage<D_>(FN_NEIGHBORS, md));
+      if (md == 0) PGDG_TRY(stage<D_>(FN_P2G, 0));
+      PGDG_TRY(stage<D_>(FN_LIMITER, 0));
+      PGDG_TRY(stage<D_>(FN_RHS, 0));
+      PGDG_TRY(stage<D_>(FN_UPDATE_GRID, md));
+      PGDG_TRY(stage<D_>(FN_UPDATE_PARTICLE, md));
+    }
   }
   // lists the next step's split/merge passes read (built from the last
   // stage's neighbour-list inputs)
@@ -551,6 +591,44 @@ int Ctx::run_simulation(int64_t n) {
   if (prof.on) PGDG_TRY(prof.begin(PS_LISTS, st, r));
   PGDG_TRY(T.end_of_step_lists<D_>(g, PA.view(), S.xb, S.hb, W, np));
   if (prof.on) PGDG_TRY(prof.end(r, st));
+  return PGDG_OK;
+}
+
+// One RK stage as a HIP graph: captured on first use for the current pair
+// of alternating buffers (grids cover the capacities; the kernels read the
+// counts from the device scalars), then replayed.
+template <int D_>
+int Ctx::rk_stage(int md) {
+  const void* k1 = PA.view().x;
+  const void* k2 = T.bin_ent;
+  for (int q = 0; q < ngraphs; q++)
+    if (graphs[q].md == md && graphs[q].k1 == k1 && graphs[q].k2 == k2) {
+      PGDG_HIP_TRY(hipGraphLaunch(graphs[q].exec, st));
+      return PGDG_OK;
+    }
+  if (ngraphs == kMaxGraphs) drop_graphs();
+  PGDG_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  capturing = true;
+  int rc = stage_impl<D_>(FN_NEIGHBORS, md);
+  if (rc == PGDG_OK && md == 0) rc = stage_impl<D_>(FN_P2G, 0);
+  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_LIMITER, 0);
+  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, 0);
+  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_GRID, md);
+  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, md);
+  capturing = false;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc != PGDG_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return PGDG_ERR_HIP;
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return PGDG_ERR_HIP;
+  graphs[ngraphs++] = StageGraph{md, k1, k2, exec};
+  PGDG_HIP_TRY(hipGraphLaunch(exec, st));
   return PGDG_OK;
 }
 
@@ -646,6 +724,14 @@ int pgdg_get_stream(pgdg_ctx* x, void** stream) {
 int pgdg_particle_count(pgdg_ctx* x, int64_t* n) {
   if (!x || !n) return PGDG_ERR_ARG;
   *n = x->c.np;
+  return PGDG_OK;
+}
+
+int pgdg_graphs_enable(pgdg_ctx* x, int on) {
+  if (!x) return PGDG_ERR_ARG;
+  PGDG_HIP_TRY(hipStreamSynchronize(x->c.st));
+  x->c.use_graphs = on != 0;
+  if (!on) x->c.drop_graphs();
   return PGDG_OK;
 }
 

@@ -74,7 +74,8 @@ struct DevScalars {
   long long iterate_num;
   unsigned long long red_dt;    // min-reduction slot (IEEE bits of a positive double)
   unsigned long long red_maxv;  // max-reduction slot
-  long long pad;
+  long long np_dev;             // live fluid particles (read by the stage kernels,
+  long long nent_dev;           //  bin entries          so a captured stage replays)
 };
 
 // get_cell_id (S3:glbfunc.h:161-173) of coord - box_l

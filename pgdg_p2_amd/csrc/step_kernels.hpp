@@ -39,7 +39,7 @@ struct StageBufs {
 
 template <int D>
 int launch_stage_save(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, int md,
-                      hipStream_t st);
+                      const DevScalars* sc, hipStream_t st);
 template <int D>
 int launch_in_ele(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, hipStream_t st);
 template <int D>

@@ -20,9 +20,9 @@ namespace pgdg {
 template <int D>
 __global__ void k_bin_gather_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B, MixedBins MB,
                                    const int* __restrict__ bin_off, const int* __restrict__ bin_ent,
-                                   int64_t ne) {
+                                   const DevScalars* __restrict__ sc) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  if (e >= sc->nent_dev) return;
   int lo = 0, hi = g.nc - 1;  // last cell with bin_off[c] <= e
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -91,11 +91,10 @@ __global__ void k_cell_bbox_mixed(Geo g, MixedBins MB, const int* __restrict__ b
 
 template <int D>
 __global__ void __launch_bounds__(256) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
-                                                   MixedBins MB, int64_t ne,
-                                                   const int* __restrict__ bin_off,
+                                                   MixedBins MB, const int* __restrict__ bin_off,
                                                    const DevScalars* __restrict__ sc) {
   const int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  if (e >= sc->nent_dev) return;
   const int i = B.ref[e];
   if (i < 0) return;  // wall or removed entry
   const int64_t cap = P.cap;
@@ -279,10 +278,10 @@ int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const Stag
   if (n_ent <= 0) return PGDG_OK;
   if (n_ent > MB.ne_cap) return PGDG_ERR_CAPACITY;
   hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
-                     P, W, S, B, MB, bin_off, bin_ent, n_ent);
+                     P, W, S, B, MB, bin_off, bin_ent, sc);
   hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
   hipLaunchKernelGGL(k_pst_mixed<D>, dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S, B, MB,
-                     n_ent, bin_off, sc);
+                     bin_off, sc);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

@@ -21,10 +21,13 @@
 
 namespace pgdg {
 
+// The stage kernels take their particle / bin-entry counts from the device
+// scalars, so their launches can be captured once and replayed while the
+// counts change from step to step (the grid then covers the capacity).
 template <int D>
-__global__ void k_stage_save(PtlDev P, StageBufs S, int64_t n) {
+__global__ void k_stage_save(PtlDev P, StageBufs S, const DevScalars* __restrict__ sc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= sc->np_dev) return;
   const int64_t cap = P.cap;
 #pragma unroll
   for (int a = 0; a < D; a++) S.xb[a * cap + i] = P.x[a * cap + i];
@@ -45,11 +48,11 @@ __global__ void k_in_ele(Geo g, PtlDev P, StageBufs S, int64_t n) {
 // Update_state_onestep (S3:particle_flow.cpp:413-477)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S, int64_t n,
+__global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S,
                                              const DevScalars* __restrict__ sc, int md) {
   constexpr int U = D + 2, M = D + 1;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= sc->np_dev) return;
   const int64_t cap = P.cap;
   const int nc = g.nc;
   const double dt = sc->dt;
@@ -142,9 +145,9 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
 // Gather the data PST reads from its neighbours into bin order.
 template <int D>
 __global__ void k_bin_gather(PtlDev P, WallDev W, StageBufs S, BinData B, const int* __restrict__ bin_ent,
-                             int64_t ne) {
+                             const DevScalars* __restrict__ sc) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  if (e >= sc->nent_dev) return;
   const int ref = bin_ent[e];
   const int64_t cap = P.cap;
   B.ref[e] = ref;
@@ -206,10 +209,10 @@ __global__ void k_cell_bbox(Geo g, BinData B, const int* __restrict__ bin_off) {
 // only candidates inside the margin band take the exact sqrt test.
 template <int D>
 __global__ void __launch_bounds__(256) k_pst_binned(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
-                                                    int64_t ne, const int* __restrict__ bin_off,
+                                                    const int* __restrict__ bin_off,
                                                     const DevScalars* __restrict__ sc) {
   int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+  if (e >= sc->nent_dev) return;
   const int i = B.ref[e];
   if (i < 0) return;  // wall entry
   const int64_t cap = P.cap;
@@ -366,10 +369,10 @@ int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
                       const DevScalars* sc, hipStream_t st) {
   if (n_ent <= 0) return PGDG_OK;
   hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S,
-                     B, bin_ent, n_ent);
+                     B, bin_ent, sc);
   hipLaunchKernelGGL(k_cell_bbox<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, B, bin_off);
   hipLaunchKernelGGL(k_pst_binned<D>, dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S, B,
-                     n_ent, bin_off, sc);
+                     bin_off, sc);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
@@ -464,11 +467,11 @@ __global__ void k_ts_finalize(DevScalars* sc, double output_dt) {
 // ---------------------------------------------------------------------------
 template <int D>
 int launch_stage_save(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, int md,
-                      hipStream_t st) {
+                      const DevScalars* sc, hipStream_t st) {
   (void)g;
   (void)md;
   if (n <= 0) return PGDG_OK;
-  hipLaunchKernelGGL(k_stage_save<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, P, S, n);
+  hipLaunchKernelGGL(k_stage_save<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, P, S, sc);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
@@ -485,8 +488,8 @@ template <int D>
 int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
                const DevScalars* sc, int md, hipStream_t st) {
   if (n <= 0) return PGDG_OK;
-  hipLaunchKernelGGL(k_g2p<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, n,
-                     sc, md);
+  hipLaunchKernelGGL(k_g2p<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
+                     md);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
@@ -510,7 +513,7 @@ int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32
 
 #define PGDG_INST(D)                                                                              \
   template int launch_stage_save<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, int,     \
-                                    hipStream_t);                                                 \
+                                    const DevScalars*, hipStream_t);                              \
   template int launch_in_ele<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, hipStream_t); \
   template int launch_g2p<D>(const Geo&, const PtlDev&, const CellDev&, const StageBufs&, int64_t, \
                              const DevScalars*, int, hipStream_t);                                \

@@ -45,6 +45,10 @@ class GpuSim:
               "pgdg_download_state")
         return st
 
+    def graphs(self, on: bool):
+        """Run the RK stages as captured HIP graphs (default) or launch by launch."""
+        check(self.L.pgdg_graphs_enable(self.h, 1 if on else 0), "pgdg_graphs_enable")
+
     def step(self, n: int = 1):
         check(self.L.pgdg_step(self.h, n), "pgdg_step")
 

@@ -12,10 +12,11 @@ from pgdg_p2_amd import cases
 from pgdg_p2_amd.compare import compare_states
 
 
-def run_pair(case, st, steps, label):
+def run_pair(case, st, steps, label, graphs=True):
     from pgdg_p2_amd.step import GpuSim
     ref = OracleSim(case, st)
     gpu = GpuSim(case, st)
+    gpu.graphs(graphs)
     a, b = st.copy(), st.copy()
     events = 0
     for k in range(steps):
@@ -58,3 +59,11 @@ def test_step_parity_triple_point_three_phase(cuda):
     # with a third material, T3:mpm_ini.cpp)
     case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25, nphase=3)
     run_pair(case, st, 12, "tp2d-3phase")
+
+
+@pytest.mark.gpu
+def test_step_parity_shock_bubble_3d_without_graphs(cuda):
+    # the RK stages launched one kernel at a time (the other tests replay
+    # the captured stage graphs, with split/merge work between replays)
+    case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25)
+    run_pair(case, st, 6, "sb3d-eager", graphs=False)

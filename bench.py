@@ -285,11 +285,12 @@ def main():
         gpu.profile(False)
         stages = {k: {"ms_total": v[0], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
         n_mid = gpu.count()
+        tuning = gpu.tuning()
         gpu.save_cells(st)
         rho = (st.c["alph"] * st.c["rho"]).sum(axis=0)
         gpu.close()
         return {"case": case, "el": el, "n_start": n_start, "n_end": n_end, "n_mid": n_mid,
-                "stages": stages, "rho": rho}
+                "stages": stages, "rho": rho, "tuning": tuning}
 
     plain = run_form(0)
     mixed = run_form(1)
@@ -346,6 +347,8 @@ def main():
             "after_steps": args.warmup + args.steps + args.profile_steps,
             "same_particle_count": plain["n_end"] == R["n_end"]},
         "roofline": roof,
+        "autotuned_workgroups": {"mixed": R["tuning"], "fp64": plain["tuning"]},
+        "stage_graphs": "RK stages replayed as HIP graphs in the timed steps",
     }
     if not args.no_mgfm:
         line["mgfm"] = mgfm_bench()

@@ -370,6 +370,14 @@ int pgdg_profile_enable(pgdg_ctx* ctx, int on);
 /* RK stages of pgdg_step / pgdg_run_simulation as HIP graphs (default on;
  * profiling runs the stages one launch at a time).                       */
 int pgdg_graphs_enable(pgdg_ctx* ctx, int on);
+/* Autotuner (default on): during the first step after it is enabled, the
+ * workgroup sizes (64, 128 or 256) of the particle-shifting, RHS and
+ * limiter kernels are chosen by timing each candidate in place (these
+ * kernels overwrite their outputs, so the extra launches change nothing).
+ * pgdg_tuning_read returns {shifting, RHS, limiter}; PGDG_ERR_STATE before
+ * the tuning step has run.                                                */
+int pgdg_autotune_enable(pgdg_ctx* ctx, int on);
+int pgdg_tuning_read(pgdg_ctx* ctx, int* blocks);
 int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 
 #ifdef __cplusplus

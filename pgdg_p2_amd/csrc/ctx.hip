@@ -139,6 +139,39 @@ struct Ctx {
   int ngraphs = 0;
   bool use_graphs = true;
   bool capturing = false;
+  // autotuner: workgroup sizes of the particle-shifting, RHS and limiter
+  // kernels, chosen by timing the candidates in place during the first step
+  Tuning tune;
+  bool autotune = true, tuned = false, tuning_now = false;
+  template <typename F>
+  int pick_block(F launch, int* best) {
+    static const int cand[3] = {64, 128, 256};
+    hipEvent_t a = nullptr, b = nullptr;
+    PGDG_HIP_TRY(hipEventCreate(&a));
+    PGDG_HIP_TRY(hipEventCreate(&b));
+    float best_ms = 1e30f;
+    int rc = PGDG_OK;
+    for (int q = 0; q < 3 && rc == PGDG_OK; q++) {
+      rc = launch(cand[q]);  // warm
+      if (rc != PGDG_OK) break;
+      (void)hipEventRecord(a, st);
+      rc = launch(cand[q]);
+      (void)hipEventRecord(b, st);
+      if (rc != PGDG_OK || hipEventSynchronize(b) != hipSuccess) {
+        rc = rc != PGDG_OK ? rc : PGDG_ERR_HIP;
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, a, b);
+      if (ms < best_ms) {
+        best_ms = ms;
+        *best = cand[q];
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+  }
   int sync_counts() {
     hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, sc, np, T.n_ent);
     PGDG_HIP_TRY(hipGetLastError());
@@ -535,9 +568,18 @@ int Ctx::stage_impl(int fn, int arg) {
     case FN_P2G:
       return launch_p2g<D_>(g, C, P, T.bin_off, T.bin_ent, np + W.n, used, PB, st);
     case FN_LIMITER:
-      return launch_cells<D_>(CELL_LIMITER, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
+      if (tuning_now)
+        PGDG_TRY(pick_block([&](int b) {
+          return launch_cells<D_>(CELL_LIMITER, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st, b);
+        }, &tune.limiter));
+      return launch_cells<D_>(CELL_LIMITER, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st,
+                              tune.limiter);
     case FN_RHS:
-      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st);
+      if (tuning_now)
+        PGDG_TRY(pick_block([&](int b) {
+          return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st, b);
+        }, &tune.rhs));
+      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st, tune.rhs);
     case FN_UPDATE_GRID:
       return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, arg, st);
     case FN_UPDATE_PARTICLE: {
@@ -550,10 +592,23 @@ int Ctx::stage_impl(int fn, int arg) {
         PGDG_TRY(prof.end(r, st));
         PGDG_TRY(prof.begin(PS_PST, st, r));
       }
-      if (cs.precision == PGDG_PREC_MIXED)
-        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st));
-      else
-        PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st));
+      if (cs.precision == PGDG_PREC_MIXED) {
+        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst));
+        if (tuning_now) {  // the gathered records are in place: time the shifting kernel alone
+          PGDG_TRY(pick_block([&](int b) {
+            return launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, b);
+          }, &tune.pst));
+          PGDG_TRY(launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, tune.pst));
+        }
+      } else {
+        PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst));
+        if (tuning_now) {
+          PGDG_TRY(pick_block([&](int b) {
+            return launch_pst_kernel<D_>(g, P, W, S, B, n_e, T.bin_off, sc, st, b);
+          }, &tune.pst));
+          PGDG_TRY(launch_pst_kernel<D_>(g, P, W, S, B, n_e, T.bin_off, sc, st, tune.pst));
+        }
+      }
       if (prof.on) PGDG_TRY(prof.end(r, st));
       return PGDG_OK;
     }
@@ -572,7 +627,21 @@ int Ctx::run_simulation(int64_t n) {
   PGDG_TRY(stage<D_>(FN_SPLIT, 0));
   for (int it = 0; it < 4; it++) PGDG_TRY(stage<D_>(FN_MERGE, 0));
   PGDG_TRY(stage<D_>(FN_MEMBERSHIP, 0));
-  if (use_graphs && !prof.on) {
+  if (autotune && !tuned && !prof.on) {
+    // first step: eager stages, timing the candidate workgroup sizes in place
+    for (int md = 0; md < 2; md++) {
+      tuning_now = md == 0;
+      int rc = stage_impl<D_>(FN_NEIGHBORS, md);
+      if (rc == PGDG_OK && md == 0) rc = stage_impl<D_>(FN_P2G, 0);
+      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_LIMITER, 0);
+      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, 0);
+      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_GRID, md);
+      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, md);
+      tuning_now = false;
+      PGDG_TRY(rc);
+    }
+    tuned = true;
+  } else if (use_graphs && !prof.on) {
     PGDG_TRY(sync_counts());
     for (int md = 0; md < 2; md++) PGDG_TRY(rk_stage<D_>(md));
   } else {
@@ -733,6 +802,22 @@ int pgdg_graphs_enable(pgdg_ctx* x, int on) {
   x->c.use_graphs = on != 0;
   if (!on) x->c.drop_graphs();
   return PGDG_OK;
+}
+
+int pgdg_autotune_enable(pgdg_ctx* x, int on) {
+  if (!x) return PGDG_ERR_ARG;
+  x->c.autotune = on != 0;
+  x->c.tuned = false;
+  x->c.drop_graphs();
+  return PGDG_OK;
+}
+
+int pgdg_tuning_read(pgdg_ctx* x, int* blocks) {
+  if (!x || !blocks) return PGDG_ERR_ARG;
+  blocks[0] = x->c.tune.pst;
+  blocks[1] = x->c.tune.rhs;
+  blocks[2] = x->c.tune.limiter;
+  return x->c.tuned ? PGDG_OK : PGDG_ERR_STATE;
 }
 
 int pgdg_profile_enable(pgdg_ctx* x, int on) {

@@ -308,8 +308,8 @@ __device__ inline void face_geo(const Geo& g, int i, int j, int k, const double*
   fg.b = in ? 0 : g.bc[f];
 }
 
-template <int D>
-__global__ void __launch_bounds__(128) k_limiter(Geo g, CellDev C, const int32_t* __restrict__ used) {
+template <int D, int BS>
+__global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = D + 1, NF = 2 * D;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
@@ -580,8 +580,8 @@ __device__ inline void flux_col(double rho, const double* rv, double E, const do
   out[U - 1] = E * vb + vb * P;
 }
 
-template <int D>
-__global__ void __launch_bounds__(128) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
+template <int D, int BS>
+__global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = D + 1, B = M - 1, NF = 2 * D;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
@@ -1014,16 +1014,26 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
 template <int D>
 int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
-                 hipStream_t st) {
+                 hipStream_t st, int block) {
   const int nc = g.nc;
   switch (which) {
     case CELL_P2G:
       return PGDG_ERR_ARG;  // launch_p2g
     case CELL_LIMITER:
-      hipLaunchKernelGGL(k_limiter<D>, dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used);
+      switch (block) {
+        case 64: hipLaunchKernelGGL((k_limiter<D, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+        case 128: hipLaunchKernelGGL((k_limiter<D, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+        case 256: hipLaunchKernelGGL((k_limiter<D, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+        default: return PGDG_ERR_ARG;
+      }
       break;
     case CELL_RHS:
-      hipLaunchKernelGGL(k_rhs<D>, dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used);
+      switch (block) {
+        case 64: hipLaunchKernelGGL((k_rhs<D, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+        case 128: hipLaunchKernelGGL((k_rhs<D, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+        case 256: hipLaunchKernelGGL((k_rhs<D, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+        default: return PGDG_ERR_ARG;
+      }
       break;
     case CELL_UPDATE:
       hipLaunchKernelGGL(k_update_grid<D>, dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
@@ -1054,8 +1064,8 @@ template int launch_p2g<3>(const Geo&, const CellDev&, const PtlDev&, const int*
                            const int32_t*, const P2gBuf&, hipStream_t);
 
 template int launch_cells<2>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
-                             const int32_t*, const DevScalars*, int, hipStream_t);
+                             const int32_t*, const DevScalars*, int, hipStream_t, int);
 template int launch_cells<3>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
-                             const int32_t*, const DevScalars*, int, hipStream_t);
+                             const int32_t*, const DevScalars*, int, hipStream_t, int);
 
 }  // namespace pgdg

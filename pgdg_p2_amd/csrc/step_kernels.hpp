@@ -7,10 +7,17 @@ namespace pgdg {
 
 enum { CELL_P2G = 0, CELL_LIMITER = 1, CELL_RHS = 2, CELL_UPDATE = 3 };
 
+// Workgroup sizes of the tuned kernels (64, 128 or 256).
+struct Tuning {
+  int pst = 256;
+  int rhs = 128;
+  int limiter = 128;
+};
+
 template <int D>
 int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
-                 hipStream_t st);
+                 hipStream_t st, int block = 256);
 
 // Particle data MLS_P2G reads, gathered into bin-entry order.  Every cell
 // needs the kernel weights (position, volume, phase: the 8-double `light`
@@ -60,7 +67,11 @@ struct BinData {
 template <int D>
 int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                       const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
-                      const DevScalars* sc, hipStream_t st);
+                      const DevScalars* sc, hipStream_t st, int block);
+template <int D>
+int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                      const BinData& B, int64_t n_ent, const int* bin_off, const DevScalars* sc,
+                      hipStream_t st, int block);
 
 // Mixed-precision particle shifting: FP32 records in bin-entry order with
 // positions relative to the lower corner of the entry's bin cell.
@@ -78,7 +89,11 @@ struct MixedBins {
 template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                     const int* bin_ent, const DevScalars* sc, hipStream_t st);
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block);
+template <int D>
+int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                            const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
+                            const DevScalars* sc, hipStream_t st, int block);
 
 template <int D>
 int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,

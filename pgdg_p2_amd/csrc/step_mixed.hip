@@ -89,8 +89,8 @@ __global__ void k_cell_bbox_mixed(Geo g, MixedBins MB, const int* __restrict__ b
   }
 }
 
-template <int D>
-__global__ void __launch_bounds__(256) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
+template <int D, int BS>
+__global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                    MixedBins MB, const int* __restrict__ bin_off,
                                                    const DevScalars* __restrict__ sc) {
   const int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
@@ -272,25 +272,51 @@ __global__ void __launch_bounds__(256) k_pst_mixed(Geo g, PtlDev P, WallDev W, S
 }
 
 template <int D>
+int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                            const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
+                            const DevScalars* sc, hipStream_t st, int block) {
+  switch (block) {
+    case 64:
+      hipLaunchKernelGGL((k_pst_mixed<D, 64>), dim3(grid_xcd(n_ent, 64)), dim3(64), 0, st, g, P, W, S, B,
+                         MB, bin_off, sc);
+      break;
+    case 128:
+      hipLaunchKernelGGL((k_pst_mixed<D, 128>), dim3(grid_xcd(n_ent, 128)), dim3(128), 0, st, g, P, W, S,
+                         B, MB, bin_off, sc);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_pst_mixed<D, 256>), dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S,
+                         B, MB, bin_off, sc);
+      break;
+    default:
+      return PGDG_ERR_ARG;
+  }
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
+template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                     const int* bin_ent, const DevScalars* sc, hipStream_t st) {
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block) {
   if (n_ent <= 0) return PGDG_OK;
   if (n_ent > MB.ne_cap) return PGDG_ERR_CAPACITY;
   hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
                      P, W, S, B, MB, bin_off, bin_ent, sc);
   hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
-  hipLaunchKernelGGL(k_pst_mixed<D>, dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S, B, MB,
-                     bin_off, sc);
-  PGDG_HIP_TRY(hipGetLastError());
+  PGDG_TRY(launch_pst_mixed_kernel<D>(g, P, W, S, B, MB, n_ent, bin_off, sc, st, block));
   return PGDG_OK;
 }
 
-template int launch_pst_mixed<2>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,
-                                 const BinData&, const MixedBins&, int64_t, const int*, const int*,
-                                 const DevScalars*, hipStream_t);
-template int launch_pst_mixed<3>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,
-                                 const BinData&, const MixedBins&, int64_t, const int*, const int*,
-                                 const DevScalars*, hipStream_t);
+#define PGDG_INST_MIXED(D)                                                                         \
+  template int launch_pst_mixed<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,     \
+                                   const BinData&, const MixedBins&, int64_t, const int*,          \
+                                   const int*, const DevScalars*, hipStream_t, int);               \
+  template int launch_pst_mixed_kernel<D>(const Geo&, const PtlDev&, const WallDev&,                \
+                                          const StageBufs&, const BinData&, const MixedBins&,       \
+                                          int64_t, const int*, const DevScalars*, hipStream_t, int);
+PGDG_INST_MIXED(2)
+PGDG_INST_MIXED(3)
+#undef PGDG_INST_MIXED
 
 }  // namespace pgdg

@@ -207,8 +207,8 @@ __global__ void k_cell_bbox(Geo g, BinData B, const int* __restrict__ bin_off) {
 // radius by a relative margin far above rounding; a candidate's squared
 // distance is compared with the squared radius with the same margin, and
 // only candidates inside the margin band take the exact sqrt test.
-template <int D>
-__global__ void __launch_bounds__(256) k_pst_binned(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
+template <int D, int BS>
+__global__ void __launch_bounds__(BS) k_pst_binned(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                     const int* __restrict__ bin_off,
                                                     const DevScalars* __restrict__ sc) {
   int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
@@ -363,16 +363,40 @@ __global__ void __launch_bounds__(256) k_pst_binned(Geo g, PtlDev P, WallDev W, 
   }
 }
 
+// the particle-shifting kernel alone, at a workgroup size of 64, 128 or 256
+template <int D>
+int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                      const BinData& B, int64_t n_ent, const int* bin_off, const DevScalars* sc,
+                      hipStream_t st, int block) {
+  switch (block) {
+    case 64:
+      hipLaunchKernelGGL((k_pst_binned<D, 64>), dim3(grid_xcd(n_ent, 64)), dim3(64), 0, st, g, P, W, S,
+                         B, bin_off, sc);
+      break;
+    case 128:
+      hipLaunchKernelGGL((k_pst_binned<D, 128>), dim3(grid_xcd(n_ent, 128)), dim3(128), 0, st, g, P, W,
+                         S, B, bin_off, sc);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_pst_binned<D, 256>), dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W,
+                         S, B, bin_off, sc);
+      break;
+    default:
+      return PGDG_ERR_ARG;
+  }
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
 template <int D>
 int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                       const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
-                      const DevScalars* sc, hipStream_t st) {
+                      const DevScalars* sc, hipStream_t st, int block) {
   if (n_ent <= 0) return PGDG_OK;
   hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S,
                      B, bin_ent, sc);
   hipLaunchKernelGGL(k_cell_bbox<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, B, bin_off);
-  hipLaunchKernelGGL(k_pst_binned<D>, dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S, B,
-                     bin_off, sc);
+  PGDG_TRY(launch_pst_kernel<D>(g, P, W, S, B, n_ent, bin_off, sc, st, block));
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
@@ -519,7 +543,10 @@ int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32
                              const DevScalars*, int, hipStream_t);                                \
   template int launch_pst_binned<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \
                                     const BinData&, int64_t, const int*, const int*,              \
-                                    const DevScalars*, hipStream_t);                              \
+                                    const DevScalars*, hipStream_t, int);                         \
+  template int launch_pst_kernel<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \
+                                    const BinData&, int64_t, const int*, const DevScalars*,       \
+                                    hipStream_t, int);                                            \
   template int launch_timestep<D>(const Geo&, const PtlDev&, const CellDev&, const int32_t*,       \
                                   int64_t, bool, DevScalars*, double, hipStream_t);
 PGDG_INST(2)

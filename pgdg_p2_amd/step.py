@@ -49,6 +49,15 @@ class GpuSim:
         """Run the RK stages as captured HIP graphs (default) or launch by launch."""
         check(self.L.pgdg_graphs_enable(self.h, 1 if on else 0), "pgdg_graphs_enable")
 
+    def autotune(self, on: bool):
+        check(self.L.pgdg_autotune_enable(self.h, 1 if on else 0), "pgdg_autotune_enable")
+
+    def tuning(self):
+        """Chosen workgroup sizes {pst, rhs, limiter}, or None before tuning."""
+        b = (C.c_int * 3)()
+        rc = self.L.pgdg_tuning_read(self.h, b)
+        return {"pst": b[0], "rhs": b[1], "limiter": b[2]} if rc == 0 else None
+
     def step(self, n: int = 1):
         check(self.L.pgdg_step(self.h, n), "pgdg_step")
 

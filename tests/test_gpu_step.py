@@ -27,6 +27,9 @@ def run_pair(case, st, steps, label, graphs=True):
         gpu.save(b)
         bad = compare_states(a, b)
         assert not bad, f"{label} step {k}: {bad[:6]}"
+        if k == 0:
+            tuned = gpu.tuning()  # the autotuner ran during the first step
+            assert tuned is not None and all(v in (64, 128, 256) for v in tuned.values()), tuned
         events += a.s.n_split + a.s.n_merge + a.s.n_inlet + a.s.n_outlet
     gpu.close()
     return a, events

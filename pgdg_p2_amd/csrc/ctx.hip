@@ -309,16 +309,15 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PB.heavy = PB.light + kP2gLight * (size_t)ne;
   size_t mb_bytes = 0;
   if (cs.precision == PGDG_PREC_MIXED) {
-    // r1, r2 (16 B), r3 (8 B), ecell (4 B) per entry; boxes 6 floats per cell
-    mb_bytes = (size_t)ne * (16 + 16 + 8 + 4) + 6 * sizeof(float) * (size_t)nc + 64;
+    // boxes 32 B per cell; r1, r2 (16 B), r3 (8 B), ecell (4 B) per entry
+    mb_bytes = 32 * (size_t)nc + (size_t)ne * (16 + 16 + 8 + 4) + 64;
     PGDG_HIP_TRY(hipMalloc(&mb_slab, mb_bytes));
     char* q = (char*)mb_slab;
+    MB.box = (float4*)q; q += 32 * (size_t)nc;
     MB.r1 = (float4*)q; q += 16 * (size_t)ne;
     MB.r2 = (float4*)q; q += 16 * (size_t)ne;
     MB.r3 = (float2*)q; q += 8 * (size_t)ne;
-    MB.ecell = (int32_t*)q; q += 4 * (size_t)ne;
-    MB.bmin = (float*)q; q += 3 * sizeof(float) * (size_t)nc;
-    MB.bmax = (float*)q;
+    MB.ecell = (int32_t*)q;
     MB.ne_cap = ne;
   }
   PGDG_HIP_TRY(hipMalloc((void**)&sc, sizeof(DevScalars)));

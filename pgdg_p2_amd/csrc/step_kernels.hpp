@@ -76,13 +76,14 @@ int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
 // Mixed-precision particle shifting: FP32 records in bin-entry order with
 // positions relative to the lower corner of the entry's bin cell.
 //   r1 = (stage-start position, phase bits), r2 = (current position, vol),
-//   r3 = (rho, c); ecell = bin cell of the entry; bmin/bmax = per-cell box
-//   of the stage-start positions in the cell's own frame.
+//   r3 = (rho, c); ecell = bin cell of the entry; box[2c], box[2c+1] = lower
+//   and upper corner of the box of cell c's stage-start positions in the
+//   cell's own frame (x, y, z; w unused).
 struct MixedBins {
   float4 *r1, *r2;
   float2* r3;
   int32_t* ecell;
-  float *bmin, *bmax;  // [3][nc]
+  float4* box;  // [nc][2]
   int64_t ne_cap;
 };
 

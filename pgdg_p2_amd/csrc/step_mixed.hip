@@ -72,21 +72,21 @@ template <int D>
 __global__ void k_cell_bbox_mixed(Geo g, MixedBins MB, const int* __restrict__ bin_off) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+  float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
   for (int e = bin_off[c]; e < bin_off[c + 1]; e++) {
     const float4 r = MB.r1[e];
-    const float v[3] = {r.x, r.y, r.z};
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-      lo[a] = fminf(lo[a], v[a]);
-      hi[a] = fmaxf(hi[a], v[a]);
+    lo.x = fminf(lo.x, r.x);
+    lo.y = fminf(lo.y, r.y);
+    hi.x = fmaxf(hi.x, r.x);
+    hi.y = fmaxf(hi.y, r.y);
+    if (D == 3) {
+      lo.z = fminf(lo.z, r.z);
+      hi.z = fmaxf(hi.z, r.z);
     }
   }
-#pragma unroll
-  for (int a = 0; a < D; a++) {
-    MB.bmin[a * g.nc + c] = lo[a];
-    MB.bmax[a * g.nc + c] = hi[a];
-  }
+  MB.box[2 * c] = lo;
+  MB.box[2 * c + 1] = hi;
 }
 
 template <int D, int BS>
@@ -140,15 +140,23 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
     for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
       for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
         const int cc = (t * g.N[1] + s) * g.N[2] + m;
-        // candidate cell frame relative to the target's: off = (cc - ct) * ps
+        // the target's positions in the candidate cell's frame:
+        // (target offset) - (cell offset) * cell size
         const float off[3] = {(float)(t - cti) * psf[0], (float)(s - ctj) * psf[1],
                               D == 3 ? (float)(m - ctk) * psf[2] : 0.f};
+        float vb[3], vu[3];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+          vb[a] = ubt[a] - off[a];
+          vu[a] = ut[a] - off[a];
+        }
+        const float4 blo = MB.box[2 * cc], bhi = MB.box[2 * cc + 1];
+        const float lo[3] = {blo.x, blo.y, blo.z}, hb_[3] = {bhi.x, bhi.y, bhi.z};
         float bd = 0;
 #pragma unroll
         for (int a = 0; a < D; a++) {
-          const float lo = off[a] + MB.bmin[a * g.nc + cc], hb_ = off[a] + MB.bmax[a * g.nc + cc];
-          const float d = ubt[a] < lo ? lo - ubt[a] : (ubt[a] > hb_ ? ubt[a] - hb_ : 0.f);
-          bd += d * d;
+          const float d = fmaxf(fmaxf(lo[a] - vb[a], vb[a] - hb_[a]), 0.f);
+          bd = fmaf(d, d, bd);
         }
         if (!(bd <= r2_box)) continue;  // empty cell or wholly outside
         const int e1 = bin_off[cc + 1];
@@ -158,8 +166,8 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
           float sqb = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) {
-            const float d = ubt[a] - (off[a] + cb[a]);
-            sqb += d * d;
+            const float d = vb[a] - cb[a];
+            sqb = fmaf(d, d, sqb);
           }
           if (!(sqb <= r2)) continue;
           const float4 c2 = MB.r2[f];
@@ -168,22 +176,21 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
           float sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) {
-            dr[a] = ut[a] - (off[a] + cu[a]);
-            sq += dr[a] * dr[a];
+            dr[a] = vu[a] - cu[a];
+            sq = fmaf(dr[a], dr[a], sq);
           }
           if (sq == 0.f) continue;
-          const float dist = sqrtf(sq);
+          // Wendland C2: W = C q^4 (1 + 2 s), dW/r = -5 C q^3 / h^2 with
+          // q = max(1 - s/2, 0), s = r / h
+          const float dist = __builtin_amdgcn_sqrtf(sq);
           const float sk = dist * inv_h;
-          float W_ = 0.f, aw = 0.f;
-          if (sk < 2.f) {
-            const float q = 1.f - 0.5f * sk;
-            const float q3 = (q * q) * q;
-            W_ = Cf * (q3 * q) * (1.f + 2.f * sk);
-            aw = dWc * q3;
-          }
+          const float q = fmaxf(fmaf(-0.5f, sk, 1.f), 0.f);
+          const float q3 = (q * q) * q;
+          const float W_ = Cf * (q3 * q) * fmaf(2.f, sk, 1.f);
+          const float aw = dWc * q3;
           const float rr = W_ * inv_Wdp;
           const float q2 = rr * rr;
-          const float tmp1 = tmp1_base * (1.f + 0.2f * (q2 * q2));
+          const float tmp1 = tmp1_base * fmaf(0.2f, q2 * q2, 1.f);
           const int phj = __float_as_int(c1.w);
           const float vol_j = c2.w;
           if (phj != 10) {
@@ -192,7 +199,8 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
             for (int a = 0; a < D; a++) pst[a] += (double)(dr[a] * fac);
             if (phj != phi) {
               const float2 c3 = MB.r3[f];
-              const float kk = -0.1f * (cri + c3.y * c3.y * c3.x) / (rhoi_f + c3.x);
+              const float kk = -0.1f * fmaf(c3.y * c3.y, c3.x, cri) *
+                               __builtin_amdgcn_rcpf(rhoi_f + c3.x);
               const float fs = kk * aw * vol_j;
 #pragma unroll
               for (int a = 0; a < D; a++) sharp[a] += (double)(dr[a] * fs);

@@ -255,46 +255,57 @@ __global__ void __launch_bounds__(BS) k_pst_binned(Geo g, PtlDev P, WallDev W, S
         }
         if (!(bd <= r2_out)) continue;  // empty cell or wholly outside
         const int e1 = bin_off[cc + 1];
-        for (int f = bin_off[cc]; f < e1; f++) {
-          double sqb = 0;
+        for (int f0 = bin_off[cc]; f0 < e1; f0 += 32) {
+          // list membership of the cell's candidates as a bit mask (loads
+          // issued back to back), then the pair terms once per set bit in
+          // entry order
+          const int nf = min(32, e1 - f0);
+          uint32_t msk = 0;
+#pragma unroll 4
+          for (int k = 0; k < nf; k++) {
+            double sqb = 0;
 #pragma unroll
-          for (int a = 0; a < D; a++) {
-            double d = xbi[a] - B.xb[a * nb + f];
-            sqb += d * d;
+            for (int a = 0; a < D; a++) {
+              double d = xbi[a] - B.xb[a * nb + f0 + k];
+              sqb += d * d;
+            }
+            if (sqb <= r2_in || (sqb <= r2_out && sqrt(sqb) <= rlim)) msk |= 1u << k;
           }
-          if (sqb > r2_out) continue;
-          if (!(sqb <= r2_in) && !(sqrt(sqb) <= rlim)) continue;
-          double dr[3];
+          while (msk) {
+            const int f = f0 + __builtin_ctz(msk);
+            msk &= msk - 1;
+            double dr[3];
 #pragma unroll
-          for (int a = 0; a < D; a++) dr[a] = xi[a] - B.x[a * nb + f];
-          double sq = 0;
+            for (int a = 0; a < D; a++) dr[a] = xi[a] - B.x[a * nb + f];
+            double sq = 0;
 #pragma unroll
-          for (int a = 0; a < D; a++) sq += dr[a] * dr[a];
-          double dist = sqrt(sq);
-          if (dist != 0) {
-            const int phj = B.ph[f];
-            const bool wl = phj == 10;
-            double vol_j = B.vol[f];
-            double aw = kern_dW<D>(dist, hi);
-            double Wij = kern_W<D>(dist, hi);
-            double rr = Wij / W_dp;
-            double q2 = rr * rr;
-            double R = 0.2 * (q2 * q2);
-            double tmp1 = tmp1_base * (1 + R);
-            if (!wl) {
+            for (int a = 0; a < D; a++) sq += dr[a] * dr[a];
+            double dist = sqrt(sq);
+            if (dist != 0) {
+              const int phj = B.ph[f];
+              const bool wl = phj == 10;
+              double vol_j = B.vol[f];
+              double aw = kern_dW<D>(dist, hi);
+              double Wij = kern_W<D>(dist, hi);
+              double rr = Wij / W_dp;
+              double q2 = rr * rr;
+              double R = 0.2 * (q2 * q2);
+              double tmp1 = tmp1_base * (1 + R);
+              if (!wl) {
 #pragma unroll
-              for (int a = 0; a < D; a++) pst[a] += dr[a] * aw * tmp1 * vol_j;
-              if (phj != phi) {
-                double rhoj = B.rho[f], cj = B.c[f];
-                double tmp2 = 1. / (rhoi + rhoj);
-                double kk = -0.1 * tmp2 * (ci * ci * rhoi + cj * cj * rhoj);
+                for (int a = 0; a < D; a++) pst[a] += dr[a] * aw * tmp1 * vol_j;
+                if (phj != phi) {
+                  double rhoj = B.rho[f], cj = B.c[f];
+                  double tmp2 = 1. / (rhoi + rhoj);
+                  double kk = -0.1 * tmp2 * (ci * ci * rhoi + cj * cj * rhoj);
 #pragma unroll
-                for (int a = 0; a < D; a++) sharp[a] += kk * (dr[a] * aw) * vol_j;
+                  for (int a = 0; a < D; a++) sharp[a] += kk * (dr[a] * aw) * vol_j;
+                }
+              } else {
+                const int w = ref_wall(B.ref[f]);
+#pragma unroll
+                for (int a = 0; a < D; a++) pst[a] += W.norm[a * W.n + w] * Wij * tmp1 * vol_j;
               }
-            } else {
-              const int w = ref_wall(B.ref[f]);
-#pragma unroll
-              for (int a = 0; a < D; a++) pst[a] += W.norm[a * W.n + w] * Wij * tmp1 * vol_j;
             }
           }
         }

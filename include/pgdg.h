@@ -243,8 +243,12 @@ enum { PGDG_SIDE_YL = 0, PGDG_SIDE_XR, PGDG_SIDE_YR, PGDG_SIDE_XL, PGDG_SIDE_ZL,
 typedef struct pgdg_case {
   int    dim;            /* D: 2 or 3                                      */
   int    nphase;         /* F: 2 or 3 (reference FldNum)                   */
-  int    order;          /* 1: P1 basis (1,x,y[,z]) of the reference        */
-  int    rk_stages;      /* 2: the reference's two-stage Heun update        */
+  int    order;          /* 1: P1 basis (1,x,y[,z]) of the reference;
+                            2: zero-mean P2 basis (UX:glbfunc.h:210-222;
+                            3D: 1,x,y,z,xy,xz,yz,x2,y2,z2) with 3-point
+                            Gauss rules (no reference counterpart)        */
+  int    rk_stages;      /* 2: the reference's two-stage Heun update;
+                            3: SSP-RK3 (no reference counterpart)          */
   int    semantics;      /* PGDG_SEM_T2 or PGDG_SEM_S3                     */
   int    n[3];           /* cells per axis (n[2] = 1 when dim == 2)         */
   int    bc[6];          /* tag of each domain side, PGDG_SIDE_* order      */
@@ -262,12 +266,18 @@ typedef struct pgdg_case {
                             and order (bit-identical to the CPU restatement);
                             PGDG_PREC_MIXED: particle shifting evaluated in
                             FP32 in cell-local coordinates with FP64 sums   */
+  double cfl;            /* time-step factor; 0 selects the reference 0.3
+                            (S3:particle.cpp:560)                          */
+  int    ppc_min, ppc_max; /* per-cell particle-count bounds of split and
+                            merge (0, 0: the reference's volume criteria
+                            only)                                          */
 } pgdg_case;
 
 /* Host SoA views.  Arrays are sized by capacity `cap` (particles) or by the
  * cell count nc = n0*n1*n2 (cells, index (i*n1 + j)*n2 + k); vector fields
  * are stored component-major ([3][cap]); coef is [U*M][cap] with entry
- * u*M + m.  U = dim+2, M = dim+1 for order 1.                              */
+ * u*M + m.  U = dim+2; M = dim+1 for order 1, 6 (2D) or 10 (3D) for
+ * order 2.                                                                 */
 typedef struct pgdg_particles {
   int64_t n, cap;
   double *x, *v, *tv, *rhov;                 /* [3][cap]                 */

@@ -30,6 +30,20 @@
 //   * mu = 0 in every reference case, so the viscous stress tau is dropped.
 //   * Eigen reductions (norm, dot, small matrix-vector products) are
 //     evaluated as left-to-right sums.
+// Extensions with no reference routine (the same definitions are in the HIP
+// library; each is pinned by the tests' closed-form checks):
+//   * order 2: the zero-mean P2 basis of UX:glbfunc.h:210-222 (2D: 1, x, y,
+//     xy, x^2 - dx^2/12, y^2 - dy^2/12), in 3D 1, x, y, z, xy, xz, yz,
+//     x^2 - dx^2/12, y^2 - dy^2/12, z^2 - dz^2/12; 3-point Gauss rules; the
+//     diagonal mass of UX:particle_flow.cpp:28-30 extended to 3D.  A
+//     particle's coefficient copy holds the gradient at the particle (not at
+//     the cell centre) and the second-order coefficients, so the particle's
+//     polynomial is U_p + sum_m c_m psi_m(x - x_p) with psi_m = phi_m -
+//     phi_m(0).
+//   * rk_stages 3: the SSP-RK3 (Shu-Osher) stages, stage codes 0, 2, 3:
+//     U1 = U0 + dt L(U0); U2 = 3/4 U0 + 1/4 (U1 + dt L(U1));
+//     U = 1/3 U0 + 2/3 (U2 + dt L(U2)); particle positions and volumes the same.
+//   * cfl: the time-step factor (0 selects the reference 0.3).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -58,6 +72,77 @@ static inline double AMIN1(double a, double b) { return a <= b ? a : b; }
 static inline int IMAX(int a, int b) { return a >= b ? a : b; }
 static inline int IMIN(int a, int b) { return a <= b ? a : b; }
 
+// Number of modes and of Gauss points per axis.
+template <int D, int ORD>
+struct Basis {
+  static const int M = ORD == 1 ? D + 1 : (D == 2 ? 6 : 10);
+  static const int NQ = ORD + 1;
+};
+
+// Modes 1..M-1 of the cell polynomial at offset r from the cell centre
+// (q12[a] = dx_a^2 / 12).
+template <int D, int ORD>
+static inline void basis_tail(const double* r, const double* q12, double* phi) {
+  for (int a = 0; a < D; a++) phi[a] = r[a];
+  if (ORD == 2) {
+    if (D == 2) {
+      phi[2] = r[0] * r[1];
+      phi[3] = r[0] * r[0] - q12[0];
+      phi[4] = r[1] * r[1] - q12[1];
+    } else {
+      phi[3] = r[0] * r[1];
+      phi[4] = r[0] * r[2];
+      phi[5] = r[1] * r[2];
+      phi[6] = r[0] * r[0] - q12[0];
+      phi[7] = r[1] * r[1] - q12[1];
+      phi[8] = r[2] * r[2] - q12[2];
+    }
+  }
+}
+
+// The same modes without their constants: psi_m(r) = phi_m(r) - phi_m(0).
+template <int D, int ORD>
+static inline void incr_tail(const double* r, double* psi) {
+  const double z[3] = {0, 0, 0};
+  basis_tail<D, ORD>(r, z, psi);
+}
+
+// Gradient of a polynomial with tail coefficients c[1..M-1] at offset r:
+// g[a] = c[1+a] + (second-order terms at r).
+template <int D, int ORD>
+static inline void grad_at(const double* c, const double* r, double* g) {
+  if (ORD == 1) {
+    for (int a = 0; a < D; a++) g[a] = c[1 + a];
+  } else if (D == 2) {
+    g[0] = c[1] + c[3] * r[1] + 2. * c[4] * r[0];
+    g[1] = c[2] + c[3] * r[0] + 2. * c[5] * r[1];
+  } else {
+    g[0] = c[1] + c[4] * r[1] + c[5] * r[2] + 2. * c[7] * r[0];
+    g[1] = c[2] + c[4] * r[0] + c[6] * r[2] + 2. * c[8] * r[1];
+    g[2] = c[3] + c[5] * r[0] + c[6] * r[1] + 2. * c[9] * r[2];
+  }
+}
+
+// Volume-integral weight of mode m (1..M-1) given the flux columns col[b]
+// (component u) at Gauss offset r: sum_b col[b][u] * d(phi_m)/dx_b.
+template <int D, int ORD>
+static inline double vol_weight(int m, const double (*col)[D + 2], int u, const double* r) {
+  if (m <= D) return col[m - 1][u];
+  if (D == 2) {
+    if (m == 3) return col[0][u] * r[1] + col[1][u] * r[0];
+    if (m == 4) return col[0][u] * (2. * r[0]);
+    return col[1][u] * (2. * r[1]);
+  }
+  switch (m) {
+    case 4: return col[0][u] * r[1] + col[1][u] * r[0];
+    case 5: return col[0][u] * r[2] + col[2][u] * r[0];
+    case 6: return col[1][u] * r[2] + col[2][u] * r[1];
+    case 7: return col[0][u] * (2. * r[0]);
+    case 8: return col[1][u] * (2. * r[1]);
+    default: return col[2][u] * (2. * r[2]);
+  }
+}
+
 struct Base {
   virtual ~Base() {}
   virtual int load(const pgdg_particles*, const pgdg_walls*, const pgdg_cells*,
@@ -72,10 +157,11 @@ struct Base {
 static inline bool is_wall(int r) { return r <= -2; }
 static inline int wall_of(int r) { return -r - 2; }
 
-template <int D>
+template <int D, int ORD>
 struct Sim : Base {
   static const int U = D + 2;
-  static const int M = D + 1;  // P1 basis (1, x, y[, z])
+  static const int M = Basis<D, ORD>::M;  // P1 (1, x, y[, z]) or P2
+  static const int NQ = Basis<D, ORD>::NQ;
   static const int NF = 2 * D;
 
   struct Ptl {
@@ -139,7 +225,9 @@ struct Sim : Base {
   int64_t iterate_num = 0;
   int64_t n_split = 0, n_inlet = 0, n_outlet = 0, n_merge = 0;
   int radius = 0;
-  double Gp[2], Gw[2];
+  double Gp[3], Gw[3];
+  double q12[3];  // dx_a^2 / 12
+  double cfl;
   double probe_c[4], probe_s[4], sqrt2;
 
   explicit Sim(const pgdg_case& c) : cs(c) {
@@ -152,9 +240,20 @@ struct Sim : Base {
     }
     double g_p_rt = cs.p_g / cs.p_sz;
     radius = (int)std::ceil(CELL_RATIO / g_p_rt);
-    Gp[0] = -1. / std::sqrt(3.);
-    Gp[1] = 1. / std::sqrt(3.);
-    Gw[0] = Gw[1] = 0.5;
+    if (ORD == 1) {
+      Gp[0] = -1. / std::sqrt(3.);
+      Gp[1] = 1. / std::sqrt(3.);
+      Gw[0] = Gw[1] = 0.5;
+      Gp[2] = Gw[2] = 0;
+    } else {
+      Gp[0] = -std::sqrt(3. / 5.);
+      Gp[1] = 0.;
+      Gp[2] = std::sqrt(3. / 5.);
+      Gw[0] = Gw[2] = 5. / 18.;
+      Gw[1] = 8. / 18.;
+    }
+    for (int a = 0; a < 3; a++) q12[a] = ps[a] * ps[a] / 12.;
+    cfl = cs.cfl > 0 ? cs.cfl : 0.3;
     for (int jj = 0; jj < 4; jj++) {
       double theta = (jj + 0.5) * PI_REF * 0.5;
       probe_c[jj] = std::cos(theta);
@@ -407,7 +506,6 @@ struct Sim : Base {
     double lt = 1.e20, mv = -1.e20;
     auto visit = [&](const double* v, double h, const double* c, const double* ac) {
       double Uv = norm_d(v);
-      double cfl = 0.3;
       double ds = h / CELL_RATIO * CUT_OFF;
       double c12 = AMAX1(c[0], c[1]);
       double a12 = AMAX1(ac[0], ac[1]);
@@ -454,7 +552,10 @@ struct Sim : Base {
     split_fast();
     for (int it = 0; it < 4; it++) merge_fast();
     membership();
-    for (int md = 0; md < 2; md++) {
+    const int codes3[3] = {0, 2, 3}, codes2[2] = {0, 1};
+    const int ns = cs.rk_stages == 3 ? 3 : 2;
+    for (int st = 0; st < ns; st++) {
+      const int md = ns == 3 ? codes3[st] : codes2[st];
       for (int id : fluid) refresh_neighbors(id, 2);
       if (md == 0) mls_p2g();
       calculate_limiter();
@@ -532,12 +633,20 @@ struct Sim : Base {
     ch.mass = cp.mass / div;
     ch.h = cp.h / 2.0;
     std::memcpy(ch.coef, cp.coef, sizeof(ch.coef));
-    std::memcpy(ch.coef0, ch.coef, sizeof(ch.coef));
+    double psi[M - 1];
+    incr_tail<D, ORD>(dr, psi);
     auto slope = [&](int u) {
       double s = 0;
-      for (int k = 0; k < D; k++) s += ch.coef[u][1 + k] * dr[k];
+      for (int k = 0; k < M - 1; k++) s += cp.coef[u][1 + k] * psi[k];
       return s;
     };
+    if (ORD == 2)  // the child's copy carries the gradient at the child
+      for (int u = 0; u < U; u++) {
+        double g[3];
+        grad_at<D, ORD>(cp.coef[u], dr, g);
+        for (int a = 0; a < D; a++) ch.coef[u][1 + a] = g[a];
+      }
+    std::memcpy(ch.coef0, ch.coef, sizeof(ch.coef));
     ch.rho = cp.rho + slope(0);
     for (int a = 0; a < D; a++) ch.rhov[a] = cp.rhov[a] + slope(1 + a);
     ch.E = cp.E + slope(U - 1);
@@ -895,9 +1004,13 @@ struct Sim : Base {
                 pj[0] = nb.rho;
                 for (int a = 0; a < D; a++) pj[1 + a] = nb.rhov[a];
                 pj[U - 1] = nb.E;
+                // the particle's polynomial at the cell centre
+                const double nx[3] = {-xij[0], -xij[1], -xij[2]};
+                double psi[M - 1];
+                incr_tail<D, ORD>(nx, psi);
                 for (int u = 0; u < U; u++) {
                   double sacc = 0;
-                  for (int kk = 0; kk < D; kk++) sacc += nb.coef[u][1 + kk] * (-xij[kk]);
+                  for (int kk = 0; kk < M - 1; kk++) sacc += nb.coef[u][1 + kk] * psi[kk];
                   pj[u] += sacc;
                 }
                 int ph = nb.phase - 1;
@@ -920,6 +1033,9 @@ struct Sim : Base {
           for (int u = 0; u < U; u++) p_avg[ss][u] /= (mass_t[ss] + 1e-20);
           for (int u = 0; u < U; u++)
             for (int m = 0; m < M - 1; m++) Cf[ss][u][m] /= (mass_t[ss] + 1e-20);
+          if (ORD == 2)  // centre value -> mean of the zero-mean basis
+            for (int u = 0; u < U; u++)
+              for (int a = 0; a < D; a++) p_avg[ss][u] += Cf[ss][u][(D == 2 ? 3 : 6) + a] * q12[a];
           cp.coef[ss][0][0] = cp.rho[ss] = p_avg[ss][0];
           for (int a = 0; a < D; a++) cp.coef[ss][1 + a][0] = cp.rhov[ss][a] = p_avg[ss][1 + a];
           cp.coef[ss][U - 1][0] = cp.E[ss] = p_avg[ss][U - 1];
@@ -1090,9 +1206,11 @@ struct Sim : Base {
         for (int s = 0; s < F; s++) {
           int lim_fg = bd_fg;
           if (cp.alph[s] != 1) lim_fg = 1;
+          double phi[M - 1];
+          basis_tail<D, ORD>(dr, q12, phi);
           for (int u = 0; u < U; u++) {
             double dU = 0;
-            for (int kk = 0; kk < D; kk++) dU += cp.coef[s][u][1 + kk] * dr[kk];
+            for (int kk = 0; kk < M - 1; kk++) dU += cp.coef[s][u][1 + kk] * phi[kk];
             double sd = 1;
             if (dU >= 0) {
               sd = u_max[s][u] / (dU + 1e-20);
@@ -1114,10 +1232,12 @@ struct Sim : Base {
             dr[0] = sqrt2 * cs.p_g * probe_c[jj];
             dr[1] = sqrt2 * cs.p_g * probe_s[jj];
             if (D == 3) dr[2] = (-0.5 + k1) * cs.p_g;
+            double phi[M - 1];
+            basis_tail<D, ORD>(dr, q12, phi);
             double Ui[U];
             for (int u = 0; u < U; u++) {
               double acc = 0;
-              for (int kk = 0; kk < D; kk++) acc += (cp.lim[s][u] * cp.coef[s][u][1 + kk]) * dr[kk];
+              for (int kk = 0; kk < M - 1; kk++) acc += (cp.lim[s][u] * cp.coef[s][u][1 + kk]) * phi[kk];
               Ui[u] = cp.coef[s][u][0] + acc;
             }
             double rho_p = Ui[0], E_p = Ui[U - 1];
@@ -1140,10 +1260,12 @@ struct Sim : Base {
   };
   // value of phase s of cell C at offset dr, slopes scaled by lim
   void trace(const Cell& C, int s, const double* dr, Trace& t) const {
+    double phi[M - 1];
+    basis_tail<D, ORD>(dr, q12, phi);
     double dU[U];
     for (int u = 0; u < U; u++) {
       double acc = 0;
-      for (int kk = 0; kk < D; kk++) acc += C.coef[s][u][1 + kk] * dr[kk];
+      for (int kk = 0; kk < M - 1; kk++) acc += C.coef[s][u][1 + kk] * phi[kk];
       dU[u] = C.coef[s][u][0] + C.lim[s][u] * acc;
     }
     t.rho = dU[0];
@@ -1177,12 +1299,23 @@ struct Sim : Base {
     double mij[M];
     mij[0] = 1.;
     for (int a = 0; a < D; a++) mij[1 + a] = hp[a] * hp[a] / 3.;
+    if (ORD == 2) {  // UX:particle_flow.cpp:28-29: (hx hy)^2/9 and 4 hx^4/45
+      const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+      for (int q = 0; q < (D == 2 ? 1 : 3); q++) {
+        const double t = hp[pa[q]] * hp[pb[q]];
+        mij[1 + D + q] = t * t / 9.;
+      }
+      for (int a = 0; a < D; a++) {
+        const double t = hp[a] * hp[a];
+        mij[M - D + a] = t * t * 4. / 45.;
+      }
+    }
     for (int m = 0; m < M; m++) mij[m] *= voli;
     const double g = cs.gamma[ss], pinf = cs.pinf[ss];
     // volume integral: Gauss points in g1, g2[, g3] order
-    int ng3 = D == 3 ? 2 : 1;
-    for (int g1 = 0; g1 < 2; g1++)
-      for (int g2 = 0; g2 < 2; g2++)
+    int ng3 = D == 3 ? NQ : 1;
+    for (int g1 = 0; g1 < NQ; g1++)
+      for (int g2 = 0; g2 < NQ; g2++)
         for (int g3 = 0; g3 < ng3; g3++) {
           double dp[3] = {0, 0, 0};
           dp[0] = Gp[g1] * hp[0];
@@ -1200,8 +1333,16 @@ struct Sim : Base {
           double P_p = eos_p(e_p, ss);
           double DF[U][D];
           flux(t.rho, t.rv, t.E, v, P_p, DF);
-          for (int bi = 1; bi < M; bi++)
+          for (int bi = 1; bi <= D; bi++)
             for (int u = 0; u < U; u++) cp.dcoef[ss][u][bi] += DF[u][bi - 1] * gw * voli * 1;
+          if (ORD == 2) {
+            double col[D][U];
+            for (int b2 = 0; b2 < D; b2++)
+              for (int u = 0; u < U; u++) col[b2][u] = DF[u][b2];
+            for (int bi = D + 1; bi < M; bi++)
+              for (int u = 0; u < U; u++)
+                cp.dcoef[ss][u][bi] += vol_weight<D, ORD>(bi, col, u, dp) * gw * voli;
+          }
         }
     // face integral
     for (int j = 0; j < NF; j++) {
@@ -1210,8 +1351,8 @@ struct Sim : Base {
       int b = fc.b;
       double DFij[M][U][D];
       std::memset(DFij, 0, sizeof(DFij));
-      int ng2 = D == 3 ? 2 : 1;
-      for (int g1 = 0; g1 < 2; g1++)
+      int ng2 = D == 3 ? NQ : 1;
+      for (int g1 = 0; g1 < NQ; g1++)
         for (int g2 = 0; g2 < ng2; g2++) {
           double dp[3], gw;
           if (D == 3) {
@@ -1225,7 +1366,7 @@ struct Sim : Base {
           for (int a = 0; a < 3; a++) dri[a] = dp[a] + fc.cen[a] - cp.x[a];
           double Nb[M];
           Nb[0] = 1.;
-          for (int a = 0; a < D; a++) Nb[1 + a] = dri[a];
+          basis_tail<D, ORD>(dri, q12, Nb + 1);
           Trace ti;
           trace(cp, ss, dri, ti);
           double vi[3] = {0, 0, 0};
@@ -1404,10 +1545,18 @@ struct Sim : Base {
             std::memcpy(cp.coef0[s], cp.coef[s], sizeof(cp.coef[s]));
             for (int u = 0; u < U; u++)
               for (int m = 0; m < M; m++) cp.coef[s][u][m] += cp.dcoef[s][u][m] * dt;
-          } else {
+          } else if (md == 1) {
             for (int u = 0; u < U; u++)
               for (int m = 0; m < M; m++)
                 cp.coef[s][u][m] = 0.5 * (cp.dcoef[s][u][m] * dt + cp.coef[s][u][m] + cp.coef0[s][u][m]);
+          } else if (md == 2) {  // SSP-RK3 stage 2
+            for (int u = 0; u < U; u++)
+              for (int m = 0; m < M; m++)
+                cp.coef[s][u][m] = 0.75 * cp.coef0[s][u][m] + 0.25 * (cp.coef[s][u][m] + cp.dcoef[s][u][m] * dt);
+          } else {  // SSP-RK3 stage 3
+            for (int u = 0; u < U; u++)
+              for (int m = 0; m < M; m++)
+                cp.coef[s][u][m] = (1. / 3.) * cp.coef0[s][u][m] + (2. / 3.) * (cp.coef[s][u][m] + cp.dcoef[s][u][m] * dt);
           }
           cp.rho[s] = cp.coef[s][0][0];
           for (int a = 0; a < D; a++) cp.rhov[s][a] = cp.coef[s][1 + a][0];
@@ -1449,19 +1598,33 @@ struct Sim : Base {
       q.vol0 = q.vol;
       for (int a = 0; a < D; a++) q.x[a] += q.tv[a] * dt;
       q.vol = (1. + divu * dt) * q.vol;
-    } else {
+    } else if (md == 1) {
       for (int a = 0; a < D; a++) q.x[a] = 0.5 * ((q.x0[a] + q.x[a]) + q.tv[a] * dt);
       q.vol = 0.5 * (q.vol0 + q.vol + divu * dt * q.vol);
+    } else if (md == 2) {
+      for (int a = 0; a < D; a++) q.x[a] = 0.75 * q.x0[a] + 0.25 * (q.x[a] + q.tv[a] * dt);
+      q.vol = 0.75 * q.vol0 + 0.25 * (q.vol + divu * dt * q.vol);
+    } else {
+      for (int a = 0; a < D; a++) q.x[a] = (1. / 3.) * q.x0[a] + (2. / 3.) * (q.x[a] + q.tv[a] * dt);
+      q.vol = (1. / 3.) * q.vol0 + (2. / 3.) * (q.vol + divu * dt * q.vol);
     }
     double dri[3];
     for (int a = 0; a < 3; a++) dri[a] = q.x[a] - C.x[a];
+    double phi[M - 1];
+    basis_tail<D, ORD>(dri, q12, phi);
     double Ui[U];
     for (int t = 0; t < U; t++) {
       double acc = 0;
-      for (int kk = 0; kk < D; kk++) acc += C.coef[ss][t][1 + kk] * dri[kk];
+      for (int kk = 0; kk < M - 1; kk++) acc += C.coef[ss][t][1 + kk] * phi[kk];
       Ui[t] = C.coef[ss][t][0] + acc;
     }
     std::memcpy(q.coef, C.coef[ss], sizeof(q.coef));
+    if (ORD == 2)  // the copy carries the gradient at the particle
+      for (int t = 0; t < U; t++) {
+        double g[3];
+        grad_at<D, ORD>(C.coef[ss][t], dri, g);
+        for (int a = 0; a < D; a++) q.coef[t][1 + a] = g[a];
+      }
     q.rho = Ui[0];
     for (int a = 0; a < 3; a++) q.rhov[a] = 0;
     for (int a = 0; a < D; a++) q.rhov[a] = Ui[1 + a];
@@ -1592,9 +1755,10 @@ extern "C" {
 void* oracle_create(const pgdg_case* c) {
   if (!c) return nullptr;
   if (c->nphase < 2 || c->nphase > PGDG_MAX_PHASE) return nullptr;
-  if (c->order != 1) return nullptr;
-  if (c->dim == 2) return new oracle::Sim<2>(*c);
-  if (c->dim == 3) return new oracle::Sim<3>(*c);
+  if (c->order != 1 && c->order != 2) return nullptr;
+  if (c->rk_stages != 2 && c->rk_stages != 3) return nullptr;
+  if (c->dim == 2) return c->order == 1 ? (void*)new oracle::Sim<2, 1>(*c) : (void*)new oracle::Sim<2, 2>(*c);
+  if (c->dim == 3) return c->order == 1 ? (void*)new oracle::Sim<3, 1>(*c) : (void*)new oracle::Sim<3, 2>(*c);
   return nullptr;
 }
 void oracle_destroy(void* h) { delete static_cast<oracle::Base*>(h); }

@@ -21,9 +21,15 @@ from .state import (BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_WALL, Case, HostState, 
 CELL_RATIO, CUT_OFF = 3.2, 2.0
 
 
-def _base_case(dim, nphase, n, box_l, box_r, semantics, bc, p_g, ppc_axis):
+def _base_case(dim, nphase, n, box_l, box_r, semantics, bc, p_g, ppc_axis, scheme=None):
+    """`scheme` = dict(order=1|2, rk_stages=2|3, cfl=0 (reference 0.3),
+    ppc_min/ppc_max=0 (reference volume criteria only))."""
+    sch = dict(order=1, rk_stages=2, cfl=0.0, ppc_min=0, ppc_max=0)
+    sch.update(scheme or {})
     c = Case()
-    c.dim, c.nphase, c.order, c.rk_stages, c.semantics = dim, nphase, 1, 2, semantics
+    c.dim, c.nphase, c.order, c.rk_stages, c.semantics = (dim, nphase, sch["order"],
+                                                          sch["rk_stages"], semantics)
+    c.cfl, c.ppc_min, c.ppc_max = sch["cfl"], sch["ppc_min"], sch["ppc_max"]
     for a in range(3):
         c.n[a] = n[a] if a < dim else 1
         c.box_l[a] = box_l[a] if a < dim else 0.0
@@ -124,7 +130,7 @@ def _walls_nodes(case: Case, spacing, vol):
 
 
 def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nphase=2,
-                    cap_factor=3.0, gammas=None):
+                    cap_factor=3.0, gammas=None, scheme=None):
     """2-D triple point on [0,7]x[0,3] with T2 semantics (all sides walls).
 
     Two phases (default): phase 1 gamma=1.5 holds the left block x<=1
@@ -136,7 +142,8 @@ def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nph
     """
     box_l, box_r = (0.0, 0.0, 0.0), (7.0, 3.0, 1.0)
     p_g = 3.0 / ny
-    case = _base_case(2, nphase, (nx, ny, 1), box_l, box_r, SEM_T2, [BC_WALL] * 6, p_g, ppc_axis)
+    case = _base_case(2, nphase, (nx, ny, 1), box_l, box_r, SEM_T2, [BC_WALL] * 6, p_g, ppc_axis,
+                      scheme)
     if nphase == 2:
         g = gammas or (1.5, 1.4)
         rho0 = (1.0, 0.125)
@@ -189,7 +196,7 @@ def rankine_hugoniot(mach, gamma, rho1, p1):
 
 
 def shock_bubble_3d(nx=256, ny=128, nz=128, ppc_axis=2, jitter=0.25, seed=0x5EED0002,
-                    x_shock=0.2, cap_factor=2.5, length=2.0):
+                    x_shock=0.2, cap_factor=2.5, length=2.0, scheme=None):
     """3-D shock-bubble channel [0,L]x[0,1]x[0,1] with S3 semantics.
 
     Air (gamma 1.4, rho 1, p 1) holds a helium sphere (gamma 1.648,
@@ -201,7 +208,7 @@ def shock_bubble_3d(nx=256, ny=128, nz=128, ppc_axis=2, jitter=0.25, seed=0x5EED
     box_l, box_r = (0.0, 0.0, 0.0), (length, 1.0, 1.0)
     p_g = 1.0 / ny
     bc = [BC_SYMMETRY, BC_OUTLET, BC_WALL, BC_INLET, BC_SYMMETRY, BC_WALL]
-    case = _base_case(3, 2, (nx, ny, nz), box_l, box_r, SEM_S3, bc, p_g, ppc_axis)
+    case = _base_case(3, 2, (nx, ny, nz), box_l, box_r, SEM_S3, bc, p_g, ppc_axis, scheme)
     case.gamma[0], case.pinf[0], case.ini_rho[0] = 1.4, 0.0, 1.0
     case.gamma[1], case.pinf[1], case.ini_rho[1] = 1.648, 0.0, 0.138
     rho2, u2, p2 = rankine_hugoniot(1.22, 1.4, 1.0, 1.0)

@@ -225,9 +225,22 @@ static void fill_geo(const pgdg_case& cs, Geo& g) {
   for (int a = 0; a < 3; a++) g.inlet_u[a] = cs.inlet_u[a];
   g.inlet_p = cs.inlet_p;
   g.radius = (int)std::ceil(kCellRatio / (cs.p_g / cs.p_sz));  // S3:level_infor.cpp:155
-  g.Gp[0] = -1. / std::sqrt(3.);  // S3:mpm_ini.cpp:37-41
-  g.Gp[1] = 1. / std::sqrt(3.);
-  g.Gw[0] = g.Gw[1] = 0.5;
+  g.ord = cs.order;
+  if (cs.order == 1) {
+    g.Gp[0] = -1. / std::sqrt(3.);  // S3:mpm_ini.cpp:37-41
+    g.Gp[1] = 1. / std::sqrt(3.);
+    g.Gw[0] = g.Gw[1] = 0.5;
+  } else {  // 3-point Gauss-Legendre, weights normalised to sum 1
+    g.Gp[0] = -std::sqrt(3. / 5.);
+    g.Gp[1] = 0.;
+    g.Gp[2] = std::sqrt(3. / 5.);
+    g.Gw[0] = g.Gw[2] = 5. / 18.;
+    g.Gw[1] = 8. / 18.;
+  }
+  for (int a = 0; a < 3; a++) g.q12[a] = g.ps[a] * g.ps[a] / 12.;
+  g.cfl = cs.cfl > 0 ? cs.cfl : 0.3;  // S3:particle.cpp:558
+  g.ppc_min = cs.ppc_min;
+  g.ppc_max = cs.ppc_max;
   for (int jj = 0; jj < 4; jj++) {  // limiter probe angles, S3:mpm.cpp:697-698
     double theta = (jj + 0.5) * kPiRef * 0.5;
     g.probe_c[jj] = std::cos(theta);
@@ -244,7 +257,7 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   cs = *c;
   D = cs.dim;
   U = D + 2;
-  M = D + 1;
+  M = cs.order == 1 ? D + 1 : (D == 2 ? 6 : 10);
   F = cs.nphase;
   fill_geo(cs, g);
   nc = g.nc;
@@ -303,7 +316,9 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   B.bmax = B.bmin + 3 * (size_t)nc;
   B.ph = bi_slab;
   B.ref = bi_slab + ne;
-  const int p2g_rows_n = kP2gLight + (D == 3 ? p2g_heavy<3>() : p2g_heavy<2>());
+  const int p2g_rows_n =
+      kP2gLight + (D == 3 ? (cs.order == 2 ? p2g_heavy<3, 2>() : p2g_heavy<3, 1>())
+                          : (cs.order == 2 ? p2g_heavy<2, 2>() : p2g_heavy<2, 1>()));
   PB.ne = ne;
   PGDG_HIP_TRY(hipMalloc((void**)&PB.light, sizeof(double) * p2g_rows_n * (size_t)ne));
   PB.heavy = PB.light + kP2gLight * (size_t)ne;
@@ -626,9 +641,15 @@ int Ctx::run_simulation(int64_t n) {
   PGDG_TRY(stage<D_>(FN_SPLIT, 0));
   for (int it = 0; it < 4; it++) PGDG_TRY(stage<D_>(FN_MERGE, 0));
   PGDG_TRY(stage<D_>(FN_MEMBERSHIP, 0));
+  // stage codes of the cell and particle updates: Heun (reference) 0, 1;
+  // SSP-RK3 0, 2, 3
+  const int codes3[3] = {0, 2, 3}, codes2[2] = {0, 1};
+  const int ns = cs.rk_stages == 3 ? 3 : 2;
+  const int* codes = ns == 3 ? codes3 : codes2;
   if (autotune && !tuned && !prof.on) {
     // first step: eager stages, timing the candidate workgroup sizes in place
-    for (int md = 0; md < 2; md++) {
+    for (int q = 0; q < ns; q++) {
+      const int md = codes[q];
       tuning_now = md == 0;
       int rc = stage_impl<D_>(FN_NEIGHBORS, md);
       if (rc == PGDG_OK && md == 0) rc = stage_impl<D_>(FN_P2G, 0);
@@ -642,9 +663,10 @@ int Ctx::run_simulation(int64_t n) {
     tuned = true;
   } else if (use_graphs && !prof.on) {
     PGDG_TRY(sync_counts());
-    for (int md = 0; md < 2; md++) PGDG_TRY(rk_stage<D_>(md));
+    for (int q = 0; q < ns; q++) PGDG_TRY(rk_stage<D_>(codes[q]));
   } else {
-    for (int md = 0; md < 2; md++) {
+    for (int q = 0; q < ns; q++) {
+      const int md = codes[q];
       PGDG_TRY(stage<D_>(FN_NEIGHBORS, md));
       if (md == 0) PGDG_TRY(stage<D_>(FN_P2G, 0));
       PGDG_TRY(stage<D_>(FN_LIMITER, 0));
@@ -729,7 +751,10 @@ int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_c
   if (!c || !out || particle_cap <= 0 || n_wall < 0) return PGDG_ERR_ARG;
   if (c->dim != 2 && c->dim != 3) return PGDG_ERR_ARG;
   if (c->nphase < 2 || c->nphase > PGDG_MAX_PHASE) return PGDG_ERR_ARG;
-  if (c->order != 1 || c->rk_stages != 2) return PGDG_ERR_ARG;
+  if (c->order != 1 && c->order != 2) return PGDG_ERR_ARG;
+  if (c->rk_stages != 2 && c->rk_stages != 3) return PGDG_ERR_ARG;
+  if (c->cfl < 0 || c->ppc_min < 0 || c->ppc_max < 0) return PGDG_ERR_ARG;
+  if (c->ppc_max > 0 && c->ppc_min > c->ppc_max) return PGDG_ERR_ARG;
   if (c->semantics != PGDG_SEM_T2 && c->semantics != PGDG_SEM_S3) return PGDG_ERR_ARG;
   if (c->precision != PGDG_PREC_FP64 && c->precision != PGDG_PREC_MIXED) return PGDG_ERR_ARG;
   if (particle_cap > (int64_t)0x7ffffff0) return PGDG_ERR_ARG;

@@ -41,10 +41,69 @@ struct Geo {
   double p_sz, p_g, p2g_cut;
   double gamma[PGDG_MAX_PHASE], pinf[PGDG_MAX_PHASE], ini_rho[PGDG_MAX_PHASE];
   double inlet_rho, inlet_u[3], inlet_p;
-  double Gp[2], Gw[2];
+  double Gp[3], Gw[3];  // Gauss rule per axis: 2 points (order 1), 3 (order 2)
+  double q12[3];        // ps[a]^2 / 12, the constants of the P2 square modes
+  double cfl;           // time-step factor
   double probe_c[4], probe_s[4], sqrt2;
   double vol_max, vol_min, dss0;
+  int ord;              // 1: P1 basis, 2: P2 basis
+  int ppc_min, ppc_max; // per-cell particle-count bounds of split/merge (0: off)
 };
+
+// Modes per cell and Gauss points per axis.
+template <int D, int ORD>
+struct Basis {
+  static constexpr int M = ORD == 1 ? D + 1 : (D == 2 ? 6 : 10);
+  static constexpr int NQ = ORD + 1;
+};
+
+// Modes 1..M-1 of the cell polynomial at offset r from the cell centre.
+// Order 2 follows the zero-mean P2 form of UX:glbfunc.h:210-222 (2D: x, y,
+// xy, x^2 - dx^2/12, y^2 - dy^2/12); 3D: x, y, z, xy, xz, yz, x^2 - dx^2/12,
+// y^2 - dy^2/12, z^2 - dz^2/12.
+template <int D, int ORD>
+__device__ inline void basis_tail(const double* r, const double* q12, double* phi) {
+#pragma unroll
+  for (int a = 0; a < D; a++) phi[a] = r[a];
+  if constexpr (ORD == 2) {
+    if constexpr (D == 2) {
+      phi[2] = r[0] * r[1];
+      phi[3] = r[0] * r[0] - q12[0];
+      phi[4] = r[1] * r[1] - q12[1];
+    } else {
+      phi[3] = r[0] * r[1];
+      phi[4] = r[0] * r[2];
+      phi[5] = r[1] * r[2];
+      phi[6] = r[0] * r[0] - q12[0];
+      phi[7] = r[1] * r[1] - q12[1];
+      phi[8] = r[2] * r[2] - q12[2];
+    }
+  }
+}
+
+// The same modes without their constants, psi_m(r) = phi_m(r) - phi_m(0):
+// a particle's copy is a polynomial about the particle's own position.
+template <int D, int ORD>
+__device__ inline void incr_tail(const double* r, double* psi) {
+  const double z[3] = {0, 0, 0};
+  basis_tail<D, ORD>(r, z, psi);
+}
+
+// Gradient at offset r of a polynomial whose tail coefficients are c[1..].
+template <int D, int ORD>
+__device__ inline void grad_at(const double* c, const double* r, double* g) {
+  if constexpr (ORD == 1) {
+#pragma unroll
+    for (int a = 0; a < D; a++) g[a] = c[1 + a];
+  } else if constexpr (D == 2) {
+    g[0] = c[1] + c[3] * r[1] + 2. * c[4] * r[0];
+    g[1] = c[2] + c[3] * r[0] + 2. * c[5] * r[1];
+  } else {
+    g[0] = c[1] + c[4] * r[1] + c[5] * r[2] + 2. * c[7] * r[0];
+    g[1] = c[2] + c[4] * r[0] + c[6] * r[2] + 2. * c[8] * r[1];
+    g[2] = c[3] + c[5] * r[0] + c[6] * r[1] + 2. * c[9] * r[2];
+  }
+}
 
 struct PtlDev {
   int64_t cap;

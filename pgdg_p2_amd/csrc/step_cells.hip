@@ -30,10 +30,10 @@ __device__ inline void cell_center(const Geo& g, int i, int j, int k, double* x)
 // ---------------------------------------------------------------------------
 // MLS P2G (S3:mpm.cpp:274-382)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, int ORD>
 __global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent,
                              const int* __restrict__ bin_total, P2gBuf pb) {
-  constexpr int U = D + 2, M = D + 1, B = M - 1, H = p2g_heavy<D>();
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, H = p2g_heavy<D, ORD>();
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= pb.ne || e >= *bin_total) return;
   const int64_t cap = P.cap;
@@ -70,10 +70,10 @@ __global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent,
 // the phase masses; a second walk, in the same order and only for cells
 // where a phase appears (ph_state 0 -> 1), forms the weighted state and
 // slope averages from the heavy records.
-template <int D>
+template <int D, int ORD>
 __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const int* __restrict__ bin_off,
                                              const int32_t* __restrict__ used) {
-  constexpr int U = D + 2, M = D + 1, B = M - 1, H = p2g_heavy<D>();
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, H = p2g_heavy<D, ORD>();
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
@@ -167,6 +167,10 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const 
               double pj[U];
 #pragma unroll
               for (int u = 0; u < U; u++) pj[u] = h[u];
+              // the particle's polynomial at the cell centre
+              const double nx[3] = {-xij[0], -xij[1], D == 3 ? -xij[2] : 0.0};
+              double psi[B];
+              incr_tail<D, ORD>(nx, psi);
               double sl[U][B];
 #pragma unroll
               for (int u = 0; u < U; u++) {
@@ -174,7 +178,7 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const 
 #pragma unroll
                 for (int b = 0; b < B; b++) {
                   sl[u][b] = h[U + u * B + b];
-                  acc += sl[u][b] * (-xij[b]);
+                  acc += sl[u][b] * psi[b];
                 }
                 pj[u] += acc;
               }
@@ -210,6 +214,12 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const 
       double pa[U];
 #pragma unroll
       for (int u = 0; u < U; u++) pa[u] = pav[ss][u] / (mass_t[ss] + 1e-20);
+      if constexpr (ORD == 2)  // centre value -> mean of the zero-mean basis
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int a = 0; a < D; a++)
+            pa[u] += cf[ss][u][(D == 2 ? 3 : 6) + a] / (mass_t[ss] + 1e-20) * g.q12[a];
       double rho = pa[0];
       C.coef[CIDX(ss, 0, 0)] = rho;
       C.rho[ss * nc + c] = rho;
@@ -308,9 +318,9 @@ __device__ inline void face_geo(const Geo& g, int i, int j, int k, const double*
   fg.b = in ? 0 : g.bc[f];
 }
 
-template <int D, int BS>
+template <int D, int ORD, int BS>
 __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t* __restrict__ used) {
-  constexpr int U = D + 2, M = D + 1, NF = 2 * D;
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, NF = 2 * D;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
@@ -506,11 +516,13 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
 #pragma unroll
         for (int a = 0; a < 3; a++) dr[a] = fg.cen[a] - xc[a];
       }
+      double phi[M - 1];
+      basis_tail<D, ORD>(dr, g.q12, phi);
 #pragma unroll
       for (int u = 0; u < U; u++) {
         double dU = 0;
 #pragma unroll
-        for (int b = 0; b < M - 1; b++) dU += sl[u][b] * dr[b];
+        for (int b = 0; b < M - 1; b++) dU += sl[u][b] * phi[b];
         double sd = 1;
         if (dU >= 0) {
           sd = umax[s][u] / (dU + 1e-20);
@@ -533,12 +545,14 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
         dr[0] = g.sqrt2 * g.p_g * g.probe_c[jj];
         dr[1] = g.sqrt2 * g.p_g * g.probe_s[jj];
         if (D == 3) dr[2] = (-0.5 + k1) * g.p_g;
+        double phi[M - 1];
+        basis_tail<D, ORD>(dr, g.q12, phi);
         double Ui[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
           double acc = 0;
 #pragma unroll
-          for (int b = 0; b < M - 1; b++) acc += (lim[u] * sl[u][b]) * dr[b];
+          for (int b = 0; b < M - 1; b++) acc += (lim[u] * sl[u][b]) * phi[b];
           Ui[u] = c0[u] + acc;
         }
         double sq = 0;
@@ -580,9 +594,9 @@ __device__ inline void flux_col(double rho, const double* rv, double E, const do
   out[U - 1] = E * vb + vb * P;
 }
 
-template <int D, int BS>
+template <int D, int ORD, int BS>
 __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
-  constexpr int U = D + 2, M = D + 1, B = M - 1, NF = 2 * D;
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NF = 2 * D, NQ = Basis<D, ORD>::NQ;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
@@ -610,9 +624,9 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
       for (int m = 0; m < M; m++) dc[u][m] = 0;
     // volume integral, Gauss points in g1, g2[, g3] order
-    for (int g1 = 0; g1 < 2; g1++)
-      for (int g2 = 0; g2 < 2; g2++)
-        for (int g3 = 0; g3 < (D == 3 ? 2 : 1); g3++) {
+    for (int g1 = 0; g1 < NQ; g1++)
+      for (int g2 = 0; g2 < NQ; g2++)
+        for (int g3 = 0; g3 < (D == 3 ? NQ : 1); g3++) {
           double dp[3] = {0, 0, 0};
           dp[0] = g.Gp[g1] * hp[0];
           dp[1] = g.Gp[g2] * hp[1];
@@ -621,12 +635,14 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
             dp[2] = g.Gp[g3] * hp[2];
             gw = g.Gw[g1] * g.Gw[g2] * g.Gw[g3];
           }
+          double phv[B];
+          basis_tail<D, ORD>(dp, g.q12, phv);
           double Up[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {
             double acc = 0;
 #pragma unroll
-            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * dp[b];
+            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * phv[b];
             Up[u] = cf[u][0] + lm[u] * acc;
           }
           double rv[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -639,12 +655,30 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
           for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
           double e_p = Up[U - 1] - 0.5 * sq / Up[0];
           double P_p = eos_p(e_p, me);
+          double col[D][U];
 #pragma unroll
-          for (int bi = 1; bi < M; bi++) {
-            double col[U];
-            flux_col<D>(Up[0], rv, Up[U - 1], v, P_p, bi - 1, col);
+          for (int b = 0; b < D; b++) flux_col<D>(Up[0], rv, Up[U - 1], v, P_p, b, col[b]);
 #pragma unroll
-            for (int u = 0; u < U; u++) dc[u][bi] += col[u] * gw * voli * 1;
+          for (int bi = 1; bi <= D; bi++)
+#pragma unroll
+            for (int u = 0; u < U; u++) dc[u][bi] += col[bi - 1][u] * gw * voli * 1;
+          if constexpr (ORD == 2) {
+            // second-order modes: sum_b col[b] d(phi_m)/dx_b at dp
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+              if constexpr (D == 2) {
+                dc[u][3] += (col[0][u] * dp[1] + col[1][u] * dp[0]) * gw * voli;
+                dc[u][4] += (col[0][u] * (2. * dp[0])) * gw * voli;
+                dc[u][5] += (col[1][u] * (2. * dp[1])) * gw * voli;
+              } else {
+                dc[u][4] += (col[0][u] * dp[1] + col[1][u] * dp[0]) * gw * voli;
+                dc[u][5] += (col[0][u] * dp[2] + col[2][u] * dp[0]) * gw * voli;
+                dc[u][6] += (col[1][u] * dp[2] + col[2][u] * dp[1]) * gw * voli;
+                dc[u][7] += (col[0][u] * (2. * dp[0])) * gw * voli;
+                dc[u][8] += (col[1][u] * (2. * dp[1])) * gw * voli;
+                dc[u][9] += (col[2][u] * (2. * dp[2])) * gw * voli;
+              }
+            }
           }
         }
     // face integral
@@ -663,8 +697,8 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
         for (int u = 0; u < U; u++) DFn[m][u] = 0;
       const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
-      for (int g1 = 0; g1 < 2; g1++)
-        for (int g2 = 0; g2 < (D == 3 ? 2 : 1); g2++) {
+      for (int g1 = 0; g1 < NQ; g1++)
+        for (int g2 = 0; g2 < (D == 3 ? NQ : 1); g2++) {
           double dp[3], gw;
           if (D == 3) {
 #pragma unroll
@@ -680,14 +714,13 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
           for (int a = 0; a < 3; a++) dri[a] = dp[a] + fg.cen[a] - xc[a];
           double Nb[M];
           Nb[0] = 1.;
-#pragma unroll
-          for (int a = 0; a < D; a++) Nb[1 + a] = dri[a];
+          basis_tail<D, ORD>(dri, g.q12, Nb + 1);
           double Ui[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {
             double acc = 0;
 #pragma unroll
-            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * dri[b];
+            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * Nb[1 + b];
             Ui[u] = cf[u][0] + lm[u] * acc;
           }
           const double rho_i = Ui[0], E_i = Ui[U - 1];
@@ -716,6 +749,8 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
             double drj[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) drj[a] = dp[a] + fg.cen[a] - xn[a];
+            double phj[B];
+            basis_tail<D, ORD>(drj, g.q12, phj);
             for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
               if (s1 >= g.F) break;
               double alph_j = C.alph[s1 * nc + nb];
@@ -725,7 +760,7 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
               for (int u = 0; u < U; u++) {
                 double acc = 0;
 #pragma unroll
-                for (int b = 0; b < B; b++) acc += C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * drj[b];
+                for (int b = 0; b < B; b++) acc += C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * phj[b];
                 Uj[u] = C.coef[(((size_t)s1 * U + u) * M) * nc + nb] + C.lim[(s1 * U + u) * nc + nb] * acc;
               }
               double rho_j = Uj[0], E_j = Uj[U - 1];
@@ -893,6 +928,18 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
     mij[0] = 1.;
 #pragma unroll
     for (int a = 0; a < D; a++) mij[1 + a] = hp[a] * hp[a] / 3.;
+    if constexpr (ORD == 2) {  // UX:particle_flow.cpp:28-29: (hx hy)^2/9 and 4 hx^4/45
+#pragma unroll
+      for (int q = 0; q < (D == 2 ? 1 : 3); q++) {
+        const double t = q == 0 ? hp[0] * hp[1] : (q == 1 ? hp[0] * hp[2] : hp[1] * hp[2]);
+        mij[1 + D + q] = t * t / 9.;
+      }
+#pragma unroll
+      for (int a = 0; a < D; a++) {
+        const double t = hp[a] * hp[a];
+        mij[M - D + a] = t * t * 4. / 45.;
+      }
+    }
 #pragma unroll
     for (int m = 0; m < M; m++) mij[m] *= voli;
 #pragma unroll
@@ -907,10 +954,12 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 // ---------------------------------------------------------------------------
 // Update_grid (S3:mpm.cpp:768-830; T2:mpm.cpp:1254-1320)
 // ---------------------------------------------------------------------------
-template <int D>
+// md: 0 first stage (Euler), 1 Heun second stage (reference), 2 and 3 the
+// second and third SSP-RK3 stages.
+template <int D, int ORD>
 __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int32_t* __restrict__ used,
                                                      const DevScalars* __restrict__ sc, int md) {
-  constexpr int U = D + 2, M = D + 1;
+  constexpr int U = D + 2, M = Basis<D, ORD>::M;
   const int nc = g.nc;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
@@ -951,11 +1000,22 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
             C.coef0[CIDX(s, u, m)] = cf[u][m];
             cf[u][m] += dcf[u][m] * dt;
           }
-      } else {
+      } else if (md == 1) {
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
           for (int m = 0; m < M; m++) cf[u][m] = 0.5 * (dcf[u][m] * dt + cf[u][m] + C.coef0[CIDX(s, u, m)]);
+      } else if (md == 2) {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int m = 0; m < M; m++) cf[u][m] = 0.75 * C.coef0[CIDX(s, u, m)] + 0.25 * (cf[u][m] + dcf[u][m] * dt);
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int m = 0; m < M; m++)
+            cf[u][m] = (1. / 3.) * C.coef0[CIDX(s, u, m)] + (2. / 3.) * (cf[u][m] + dcf[u][m] * dt);
       }
       rho = cf[0][0];
       rv[2] = C.rhov[(s * 3 + 2) * nc + c];
@@ -1011,39 +1071,47 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int D>
-int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
-                 const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
-                 hipStream_t st, int block) {
+template <int D, int ORD>
+static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32_t* used,
+                          const DevScalars* sc, int arg, hipStream_t st, int block) {
   const int nc = g.nc;
   switch (which) {
-    case CELL_P2G:
-      return PGDG_ERR_ARG;  // launch_p2g
     case CELL_LIMITER:
       switch (block) {
-        case 64: hipLaunchKernelGGL((k_limiter<D, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
-        case 128: hipLaunchKernelGGL((k_limiter<D, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
-        case 256: hipLaunchKernelGGL((k_limiter<D, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+        case 64: hipLaunchKernelGGL((k_limiter<D, ORD, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+        case 128: hipLaunchKernelGGL((k_limiter<D, ORD, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+        case 256: hipLaunchKernelGGL((k_limiter<D, ORD, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
         default: return PGDG_ERR_ARG;
       }
       break;
     case CELL_RHS:
       switch (block) {
-        case 64: hipLaunchKernelGGL((k_rhs<D, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
-        case 128: hipLaunchKernelGGL((k_rhs<D, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
-        case 256: hipLaunchKernelGGL((k_rhs<D, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+        case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+        case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+        case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
         default: return PGDG_ERR_ARG;
       }
       break;
     case CELL_UPDATE:
-      hipLaunchKernelGGL(k_update_grid<D>, dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
+      hipLaunchKernelGGL((k_update_grid<D, ORD>), dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
                          sc, arg);
       break;
     default:
-      return PGDG_ERR_ARG;
+      return PGDG_ERR_ARG;  // P2G: launch_p2g
   }
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
+}
+
+template <int D>
+int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
+                 const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
+                 hipStream_t st, int block) {
+  (void)P;
+  (void)bin_off;
+  (void)bin_ent;
+  if (g.ord == 2) return launch_cells_o<D, 2>(which, g, C, used, sc, arg, st, block);
+  return launch_cells_o<D, 1>(which, g, C, used, sc, arg, st, block);
 }
 
 template <int D>
@@ -1052,9 +1120,15 @@ int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_o
   // every bin entry (the device-side total bin_off[nc]) is gathered; the
   // grid covers the buffer's capacity
   (void)n_ent;
-  hipLaunchKernelGGL(k_p2g_gather<D>, dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P,
-                     bin_ent, bin_off + g.nc, pb);
-  hipLaunchKernelGGL(k_p2g<D>, dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, pb, bin_off, used);
+  if (g.ord == 2) {
+    hipLaunchKernelGGL((k_p2g_gather<D, 2>), dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P,
+                       bin_ent, bin_off + g.nc, pb);
+    hipLaunchKernelGGL((k_p2g<D, 2>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, pb, bin_off, used);
+  } else {
+    hipLaunchKernelGGL((k_p2g_gather<D, 1>), dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P,
+                       bin_ent, bin_off + g.nc, pb);
+    hipLaunchKernelGGL((k_p2g<D, 1>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, pb, bin_off, used);
+  }
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

@@ -29,8 +29,8 @@ struct P2gBuf {
   int64_t ne;
 };
 constexpr int kP2gLight = 6;  // x[3] (or x[2], 0), vol, phase, pad
-template <int D>
-constexpr int p2g_heavy() { return ((D + 2) * (D + 1) + 1) / 2 * 2; }
+template <int D, int ORD = 1>
+constexpr int p2g_heavy() { return ((D + 2) * Basis<D, ORD>::M + 1) / 2 * 2; }
 
 template <int D>
 int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off, const int* bin_ent,

@@ -47,10 +47,14 @@ __global__ void k_in_ele(Geo g, PtlDev P, StageBufs S, int64_t n) {
 // ---------------------------------------------------------------------------
 // Update_state_onestep (S3:particle_flow.cpp:413-477)
 // ---------------------------------------------------------------------------
-template <int D>
+// md: 0 first stage, 1 Heun second stage, 2 and 3 the SSP-RK3 stages (the
+// cell update's stage codes).  Order 2: the particle's copy carries the
+// gradient of the cell polynomial at the particle and its second-order
+// coefficients.
+template <int D, int ORD>
 __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S,
                                              const DevScalars* __restrict__ sc, int md) {
-  constexpr int U = D + 2, M = D + 1;
+  constexpr int U = D + 2, M = Basis<D, ORD>::M;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= sc->np_dev) return;
   const int64_t cap = P.cap;
@@ -73,30 +77,48 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
     }
     S.vol0[i] = vol;
     vol = (1. + divu * dt) * vol;
-  } else {
+  } else if (md == 1) {
 #pragma unroll
     for (int a = 0; a < D; a++) x[a] = 0.5 * ((S.x0[a * cap + i] + x[a]) + P.tv[a * cap + i] * dt);
     vol = 0.5 * (S.vol0[i] + vol + divu * dt * vol);
+  } else if (md == 2) {
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = 0.75 * S.x0[a * cap + i] + 0.25 * (x[a] + P.tv[a * cap + i] * dt);
+    vol = 0.75 * S.vol0[i] + 0.25 * (vol + divu * dt * vol);
+  } else {
+#pragma unroll
+    for (int a = 0; a < D; a++)
+      x[a] = (1. / 3.) * S.x0[a * cap + i] + (2. / 3.) * (x[a] + P.tv[a * cap + i] * dt);
+    vol = (1. / 3.) * S.vol0[i] + (2. / 3.) * (vol + divu * dt * vol);
   }
   int ci, cj, ck;
   cell_ijk(g, c, ci, cj, ck);
   double xc[3] = {g.ps[0] * (ci + 0.5), g.ps[1] * (cj + 0.5), D == 3 ? g.ps[2] * (ck + 0.5) : 0.0};
-  double dri[3];
+  double dri[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; a++) dri[a] = x[a] - xc[a];
+  double phi[M - 1];
+  basis_tail<D, ORD>(dri, g.q12, phi);
   double Ui[U];
 #pragma unroll
   for (int t = 0; t < U; t++) {
     double acc = 0;
-    double c0 = C.coef[(((size_t)ss * U + t) * M) * nc + c];
-    P.coef[(size_t)(t * M) * cap + i] = c0;
+    double cw[M];
+    cw[0] = C.coef[(((size_t)ss * U + t) * M) * nc + c];
 #pragma unroll
     for (int b = 0; b < M - 1; b++) {
-      double cb = C.coef[(((size_t)ss * U + t) * M + 1 + b) * nc + c];
-      P.coef[(size_t)(t * M + 1 + b) * cap + i] = cb;
-      acc += cb * dri[b];
+      cw[1 + b] = C.coef[(((size_t)ss * U + t) * M + 1 + b) * nc + c];
+      acc += cw[1 + b] * phi[b];
     }
-    Ui[t] = c0 + acc;
+    Ui[t] = cw[0] + acc;
+    if constexpr (ORD == 2) {
+      double gr[3];
+      grad_at<D, ORD>(cw, dri, gr);
+#pragma unroll
+      for (int a = 0; a < D; a++) cw[1 + a] = gr[a];
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) P.coef[(size_t)(t * M + m) * cap + i] = cw[m];
   }
   double rho = Ui[0], E = Ui[U - 1];
   double rv[3] = {0, 0, 0};
@@ -451,7 +473,7 @@ __global__ void __launch_bounds__(256) k_timestep(Geo g, PtlDev P, CellDev C,
 #pragma unroll
     for (int a = 0; a < D; a++) sq += v[a] * v[a];
     double Uv = sqrt(sq);
-    double cfl = 0.3;
+    const double cfl = g.cfl;
     double ds = h / kCellRatio * kCutOff;
     double t1 = cfl * ds / (c12 + Uv);
     double t2 = cfl * sqrt(ds / a12);
@@ -523,7 +545,11 @@ template <int D>
 int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
                const DevScalars* sc, int md, hipStream_t st) {
   if (n <= 0) return PGDG_OK;
-  hipLaunchKernelGGL(k_g2p<D>, dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
+  if (g.ord == 2)
+    hipLaunchKernelGGL((k_g2p<D, 2>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
+                       md);
+  else
+  hipLaunchKernelGGL((k_g2p<D, 1>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
                      md);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;

@@ -293,25 +293,50 @@ __global__ void k_split_classify(Geo g, PtlDev P, int64_t n, uint8_t* type, int*
   outcnt[i] = t == T_OUTLET ? 1 : 0;
 }
 
+// Increments of the parent's polynomial at offset dr (order 1: slopes * dr;
+// order 2: the particle-centred P2 form) and, for order 2, the copy moved to
+// the new position (gradient re-evaluated at dr).
+template <int D, int ORD>
+__device__ inline void split_increment(const double* pcoef, const double* dr, double* sl) {
+  constexpr int U = D + 2, M = Basis<D, ORD>::M;
+  double psi[M - 1];
+  incr_tail<D, ORD>(dr, psi);
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < M - 1; k++) s += pcoef[u * M + 1 + k] * psi[k];
+    sl[u] = s;
+  }
+}
+template <int D, int ORD>
+__device__ inline void write_moved_copy(const PtlDev& P, int64_t dst, const double* pcoef,
+                                        const double* dr) {
+  constexpr int U = D + 2, M = Basis<D, ORD>::M;
+  const int64_t cap = P.cap;
+  for (int q = 0; q < U * M; q++) P.coef[(size_t)q * cap + dst] = pcoef[q];
+  if constexpr (ORD == 2)
+    for (int u = 0; u < U; u++) {
+      double gr[3];
+      grad_at<D, ORD>(pcoef + u * M, dr, gr);
+#pragma unroll
+      for (int a = 0; a < D; a++) P.coef[(size_t)(u * M + 1 + a) * cap + dst] = gr[a];
+    }
+}
+
 // Child of parent p at coordinate xc (Set_split_ptcl, S3:particle.cpp:575-609)
-template <int D>
+template <int D, int ORD>
 __device__ void write_split_child(const Geo& g, const PtlDev& P, int64_t p, int64_t dst,
                                   const double* xc, const double* pcoef) {
-  constexpr int U = D + 2, M = D + 1;
+  constexpr int U = D + 2;
   const int64_t cap = P.cap;
-  double dr[3];
+  double dr[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; a++) dr[a] = xc[a] - P.x[a * cap + p];
   const int ph = P.phase[p];
   const double div = D == 3 ? 8.0 : 4.0;
   double sl[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    double s = 0;
-#pragma unroll
-    for (int k = 0; k < D; k++) s += pcoef[u * M + 1 + k] * dr[k];
-    sl[u] = s;
-  }
+  split_increment<D, ORD>(pcoef, dr, sl);
   double rho = P.rho[p] + sl[0];
   double rv[3] = {0, 0, 0};
 #pragma unroll
@@ -338,7 +363,7 @@ __device__ void write_split_child(const Geo& g, const PtlDev& P, int64_t p, int6
   P.vol[dst] = P.vol[p] / div;
   P.mass[dst] = P.mass[p] / div;
   P.h[dst] = P.h[p] / 2.0;
-  for (int q = 0; q < U * M; q++) P.coef[(size_t)q * cap + dst] = pcoef[q];
+  write_moved_copy<D, ORD>(P, dst, pcoef, dr);
   P.rho[dst] = rho;
   P.E[dst] = E;
   P.P[dst] = Pp;
@@ -346,12 +371,12 @@ __device__ void write_split_child(const Geo& g, const PtlDev& P, int64_t p, int6
   P.mark_in[dst] = 0;
 }
 
-template <int D>
+template <int D, int ORD>
 __global__ void k_split_generate(Geo g, PtlDev P, int64_t n, const uint8_t* __restrict__ type,
                                  const int* __restrict__ newbase, const int* __restrict__ seqbase,
                                  const int* __restrict__ inbase, int s_total, uint8_t* rm,
                                  uint8_t* removed, int* app_ref, uint32_t* app_cell, int* lcnt) {
-  constexpr int U = D + 2, M = D + 1;
+  constexpr int U = D + 2, M = Basis<D, ORD>::M;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint8_t t = type[i];
@@ -427,7 +452,7 @@ __global__ void k_split_generate(Geo g, PtlDev P, int64_t n, const uint8_t* __re
     if (D == 3) xc[2] = pref[2] + dp / 2. * cc;
     if (ch < nch - 1) {
       int64_t dst = base + ch;
-      write_split_child<D>(g, P, i, dst, xc, pcoef);
+      write_split_child<D, ORD>(g, P, i, dst, xc, pcoef);
       lcnt[dst] = 0;
       app_ref[seq + 1 + ch] = (int)dst;
       app_cell[seq + 1 + ch] = (uint32_t)cell_of<D>(g, xc);
@@ -438,19 +463,14 @@ __global__ void k_split_generate(Geo g, PtlDev P, int64_t n, const uint8_t* __re
   }
   // parent takes the last child's state (Copy_split_state, v not copied)
   {
-    double dr[3];
+    double dr[3] = {0, 0, 0};
 #pragma unroll
     for (int a = 0; a < D; a++) dr[a] = xlast[a] - P.x[a * cap + i];
     const int ph = P.phase[i];
     const double div = D == 3 ? 8.0 : 4.0;
     double sl[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      double s = 0;
-#pragma unroll
-      for (int k = 0; k < D; k++) s += pcoef[u * M + 1 + k] * dr[k];
-      sl[u] = s;
-    }
+    split_increment<D, ORD>(pcoef, dr, sl);
+    if constexpr (ORD == 2) write_moved_copy<D, ORD>(P, i, pcoef, dr);
     double rho = P.rho[i] + sl[0];
     double rv[3] = {0, 0, 0};
 #pragma unroll
@@ -578,7 +598,8 @@ template <int D>
 __global__ void k_merge_apply(Geo g, PtlDev P, int64_t n, const int* cand, const int* surv,
                               const int* sbase, uint8_t* rm, uint8_t* removed, uint8_t* src,
                               int* app_ref, uint32_t* app_cell) {
-  constexpr int U = D + 2, M = D + 1;
+  constexpr int U = D + 2;
+  const int M = g.ord == 2 ? (D == 2 ? 6 : 10) : D + 1;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !surv[i]) return;
   const int64_t cap = P.cap;
@@ -870,8 +891,12 @@ int Topo::split(const Geo& g, ParticleArrays& PA, const WallDev& W, int64_t* n_i
   // old lists of split parents and outlet particles -> affected
   hipLaunchKernelGGL(k_split_src, dim3(g1(n)), dim3(256), 0, stream, type, n, src);
   hipLaunchKernelGGL(k_scatter_lists, dim3(g1(n)), dim3(256), 0, stream, n, src, lcnt, loff, pool, aff);
-  hipLaunchKernelGGL(k_split_generate<D>, dim3(g1(n)), dim3(256), 0, stream, g, P, n, type, sa, sb,
-                     sc_, (int)n_seq, rm, removed, app_ref, app_cell, lcnt);
+  if (g.ord == 2)
+    hipLaunchKernelGGL((k_split_generate<D, 2>), dim3(g1(n)), dim3(256), 0, stream, g, P, n, type, sa, sb,
+                       sc_, (int)n_seq, rm, removed, app_ref, app_cell, lcnt);
+  else
+    hipLaunchKernelGGL((k_split_generate<D, 1>), dim3(g1(n)), dim3(256), 0, stream, g, P, n, type, sa, sb,
+                       sc_, (int)n_seq, rm, removed, app_ref, app_cell, lcnt);
   int64_t k_app = n_seq + n_in;
   // centre cells of the windows are the cells of every appended entry
   if (k_app > 0) {

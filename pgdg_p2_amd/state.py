@@ -21,6 +21,7 @@ class Case(C.Structure):
         ("inlet_rho", C.c_double), ("inlet_u", C.c_double * 3), ("inlet_p", C.c_double),
         ("p2g_cut", C.c_double), ("output_dt", C.c_double), ("max_v0", C.c_double),
         ("precision", C.c_int),
+        ("cfl", C.c_double), ("ppc_min", C.c_int), ("ppc_max", C.c_int),
     ]
 
     @property

@@ -70,3 +70,28 @@ def test_step_parity_shock_bubble_3d_without_graphs(cuda):
     # the captured stage graphs, with split/merge work between replays)
     case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25)
     run_pair(case, st, 6, "sb3d-eager", graphs=False)
+
+
+# P2 basis (3-point rules) and SSP-RK3: no reference routine; the oracle's
+# restatement of the same definitions is the checker (bit-exact).
+P2_RK3 = dict(order=2, rk_stages=3, cfl=0.2)
+
+
+@pytest.mark.gpu
+def test_step_parity_p2_rk3_triple_point_2d(cuda):
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25, scheme=P2_RK3)
+    run_pair(case, st, 10, "tp2d-p2rk3")
+
+
+@pytest.mark.gpu
+def test_step_parity_p2_rk3_shock_bubble_3d(cuda):
+    case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25,
+                                     scheme=dict(order=2, rk_stages=3))
+    run_pair(case, st, 4, "sb3d-p2rk3")
+
+
+@pytest.mark.gpu
+def test_step_parity_p1_rk3_shock_bubble_3d_without_graphs(cuda):
+    case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25,
+                                     scheme=dict(order=1, rk_stages=3))
+    run_pair(case, st, 4, "sb3d-p1rk3-eager", graphs=False)

@@ -44,6 +44,10 @@
 //     U1 = U0 + dt L(U0); U2 = 3/4 U0 + 1/4 (U1 + dt L(U1));
 //     U = 1/3 U0 + 2/3 (U2 + dt L(U2)); particle positions and volumes the same.
 //   * cfl: the time-step factor (0 selects the reference 0.3).
+//   * ppc_min / ppc_max (0: off): a split is skipped when it would take its
+//     cell above ppc_max fluid particles; a particle is a merge candidate
+//     only while its cell holds more than ppc_min (counts of the bins when
+//     the split or merge pass starts).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -733,12 +737,16 @@ struct Sim : Base {
     std::unordered_set<int> outlet_set;
     bool io = s3();
     // 1. split / inlet / outlet, single-thread order
+    std::vector<int> fcnt;
+    if (cs.ppc_max > 0) fluid_counts(fcnt);
     std::vector<int> snapshot = fluid;
     for (int id : snapshot) {
       pool[id].copy_p = -1;
       pool[id].mrg_mark = 0;
       double dp = root(pool[id].vol);
       bool f1 = pool[id].vol > vol_max;
+      // per-cell bound: no split that would take the cell above ppc_max
+      if (f1 && cs.ppc_max > 0) f1 = fcnt[cell_index_of(pool[id].x)] + ((1 << D) - 1) <= cs.ppc_max;
       bool f2 = io && pool[id].x[0] - cs.box_l[0] >= dp && pool[id].mark_in == 1;
       bool f3 = io && pool[id].x[0] > cs.box_r[0];
       if (f1) {
@@ -879,6 +887,14 @@ struct Sim : Base {
     cp.mrg = 0;
   }
 
+  // fluid particles in every cell's bin
+  void fluid_counts(std::vector<int>& fcnt) const {
+    fcnt.assign(nc, 0);
+    for (int c = 0; c < nc; c++)
+      for (int r : bins[c])
+        if (r >= 0) fcnt[c]++;
+  }
+
   void merge_fast() {
     double vol_min = (D == 3 ? (cs.p_sz * cs.p_sz) * cs.p_sz : cs.p_sz * cs.p_sz) * 2. / 3.;
     struct Rec {
@@ -886,12 +902,20 @@ struct Sim : Base {
       std::vector<int> old_nb;
     };
     std::vector<Rec> recs;
+    // per-cell bound: merge candidates only in cells holding more than
+    // ppc_min fluid particles when the pass starts
+    std::vector<int> fcnt;
+    if (cs.ppc_min > 0) fluid_counts(fcnt);
+    auto small = [&](const Ptl& q) {
+      if (!(q.vol < vol_min)) return false;
+      return cs.ppc_min <= 0 || fcnt[cell_index_of(q.x)] > cs.ppc_min;
+    };
     for (int id : fluid) {
       Ptl& cp = pool[id];
       cp.mrg = 0;
       cp.mrg_mark = 0;
       int cp1 = -1;
-      if (cp.vol < vol_min) {
+      if (small(cp)) {
         double dss = root(vol_min);
         for (int r : cp.nb) {
           if (r == id || r == -1) continue;
@@ -902,7 +926,7 @@ struct Sim : Base {
           double d[3];
           for (int a = 0; a < 3; a++) d[a] = cp.x[a] - nb.x[a];
           double dist = norm_d(d);
-          if (nb.vol < vol_min && dist < dss && cp.phase == nb.phase) {
+          if (nb.vol < vol_min && dist < dss && cp.phase == nb.phase && small(nb)) {
             dss = dist;
             cp1 = r;
             cp.mrg = 1;

@@ -60,6 +60,16 @@ __global__ void k_set_i32(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// Fluid entries (refs >= 0) of every cell's bin.
+__global__ void k_fluid_count(const int* __restrict__ bin_off, const int* __restrict__ bin_ent, int nc,
+                              int* fcnt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int k = 0;
+  for (int e = bin_off[c]; e < bin_off[c + 1]; e++) k += bin_ent[e] >= 0 ? 1 : 0;
+  fcnt[c] = k;
+}
+
 // Particles whose list the next step's split/merge passes read before any
 // rebuild: split candidates, outlet candidates, merge candidates.
 template <int D>
@@ -274,7 +284,7 @@ enum { T_NONE = 0, T_SPLIT = 1, T_INLET = 2, T_OUTLET = 3 };
 
 template <int D>
 __global__ void k_split_classify(Geo g, PtlDev P, int64_t n, uint8_t* type, int* newcnt,
-                                 int* seqcnt, int* incnt, int* outcnt) {
+                                 int* seqcnt, int* incnt, int* outcnt, const int* __restrict__ fcnt) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const bool io = g.sem == PGDG_SEM_S3;
@@ -282,6 +292,12 @@ __global__ void k_split_classify(Geo g, PtlDev P, int64_t n, uint8_t* type, int*
   double dp = root_d<D>(vol);
   double x0 = P.x[i];
   bool f1 = vol > g.vol_max;
+  if (f1 && g.ppc_max > 0) {  // no split that would take the cell above ppc_max
+    double x[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) x[a] = P.x[a * P.cap + i];
+    f1 = fcnt[cell_of<D>(g, x)] + ((1 << D) - 1) <= g.ppc_max;
+  }
   bool f2 = io && x0 - g.box_l[0] >= dp && P.mark_in[i] == 1;
   bool f3 = io && x0 > g.box_r[0];
   uint8_t t = f1 ? T_SPLIT : f2 ? T_INLET : f3 ? T_OUTLET : T_NONE;
@@ -552,16 +568,27 @@ __global__ void k_affected(Geo g, PtlDev P, int64_t n, const uint8_t* win, uint8
 // ---------------------------------------------------------------------------
 // merge
 // ---------------------------------------------------------------------------
+// Merge candidate: volume below vol_min and, with per-cell bounds, a cell
+// (of the current position) holding more than ppc_min fluid particles.
+template <int D>
+__device__ inline bool merge_small(const Geo& g, const PtlDev& P, int64_t i, const int* fcnt) {
+  if (!(P.vol[i] < g.vol_min)) return false;
+  if (g.ppc_min <= 0) return true;
+  double x[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) x[a] = P.x[a * P.cap + i];
+  return fcnt[cell_of<D>(g, x)] > g.ppc_min;
+}
+
 template <int D>
 __global__ void __launch_bounds__(128) k_merge_candidate(Geo g, PtlDev P, int64_t n, const int* lcnt,
                                                         const int64_t* loff, const int* pool,
-                                                        int* cand) {
+                                                        int* cand, const int* __restrict__ fcnt) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t cap = P.cap;
   int best = -1;
-  double vi = P.vol[i];
-  if (vi < g.vol_min) {
+  if (merge_small<D>(g, P, i, fcnt)) {
     double dss = g.dss0;
     double xi[3];
 #pragma unroll
@@ -578,7 +605,7 @@ __global__ void __launch_bounds__(128) k_merge_candidate(Geo g, PtlDev P, int64_
         sq += d * d;
       }
       double dist = sqrt(sq);
-      if (P.vol[r] < g.vol_min && dist < dss && ph == P.phase[r]) {
+      if (P.vol[r] < g.vol_min && dist < dss && ph == P.phase[r] && merge_small<D>(g, P, r, fcnt)) {
         dss = dist;
         best = r;
       }
@@ -678,6 +705,7 @@ int Topo::init(int64_t cap_, int64_t nwall_, int nc_, hipStream_t st) {
   PGDG_TRY(A((void**)&cnt, sizeof(int) * (nc + 1)));
   PGDG_TRY(A((void**)&cnt2, sizeof(int) * (nc + 1)));
   PGDG_TRY(A((void**)&off2, sizeof(int) * (nc + 1)));
+  PGDG_TRY(A((void**)&fcnt, sizeof(int) * (nc + 1)));
   PGDG_TRY(A((void**)&win, nc));
   PGDG_TRY(A((void**)&rm, cap));
   PGDG_TRY(A((void**)&removed, cap));
@@ -728,7 +756,7 @@ int Topo::init(int64_t cap_, int64_t nwall_, int nc_, hipStream_t st) {
 }
 
 void Topo::release() {
-  void* ptrs[] = {bin_off, bin_ent, bin_ent2, keys, keys2, vals, cnt, cnt2, off2, win, rm, removed,
+  void* ptrs[] = {bin_off, bin_ent, bin_ent2, keys, keys2, vals, cnt, cnt2, off2, fcnt, win, rm, removed,
                   src, aff, need_, type, ia, ib, ic, id, sa, sb, sc_, sd, app_ref, app_ref2,
                   app_cell, app_cell2, lcnt, lcnt2, loff, loff2, lscan, lwide, newidx, host_small, pool, tmp};
   for (void* p : ptrs)
@@ -872,8 +900,10 @@ int Topo::split(const Geo& g, ParticleArrays& PA, const WallDev& W, int64_t* n_i
   counts[0] = counts[1] = counts[2] = 0;
   if (n == 0) return PGDG_OK;
   PtlDev P = PA.view();
+  if (g.ppc_max > 0)
+    hipLaunchKernelGGL(k_fluid_count, dim3(g1(nc)), dim3(256), 0, stream, bin_off, bin_ent, nc, fcnt);
   hipLaunchKernelGGL(k_split_classify<D>, dim3(g1(n)), dim3(256), 0, stream, g, P, n, type, ia, ib,
-                     ic, id);
+                     ic, id, fcnt);
   int64_t n_new = 0, n_seq = 0, n_in = 0, n_out = 0;
   PGDG_TRY(scan_total(ia, sa, n, &n_new));
   PGDG_TRY(scan_total(ib, sb, n, &n_seq));
@@ -923,8 +953,10 @@ int Topo::merge(const Geo& g, ParticleArrays& PA, const WallDev& W, int64_t* n_i
   *merged = 0;
   if (n == 0) return PGDG_OK;
   PtlDev P = PA.view();
+  if (g.ppc_min > 0)
+    hipLaunchKernelGGL(k_fluid_count, dim3(g1(nc)), dim3(256), 0, stream, bin_off, bin_ent, nc, fcnt);
   hipLaunchKernelGGL(k_merge_candidate<D>, dim3(g1(n, 128)), dim3(128), 0, stream, g, P, n, lcnt, loff,
-                     pool, ia);
+                     pool, ia, fcnt);
   hipLaunchKernelGGL(k_merge_mark, dim3(g1(n)), dim3(256), 0, stream, n, ia, ib);
   int64_t ns = 0;
   PGDG_TRY(scan_total(ib, sa, n, &ns));

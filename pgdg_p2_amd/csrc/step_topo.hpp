@@ -74,6 +74,7 @@ struct Topo {
   uint32_t *keys = nullptr, *keys2 = nullptr;
   int* vals = nullptr;
   int *cnt = nullptr, *cnt2 = nullptr, *off2 = nullptr;
+  int* fcnt = nullptr;  // fluid entries per cell (per-cell split/merge bounds)
   uint8_t* win = nullptr;
   // per-particle scratch
   uint8_t *rm = nullptr, *removed = nullptr, *src = nullptr, *aff = nullptr, *need_ = nullptr,

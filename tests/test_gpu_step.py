@@ -95,3 +95,13 @@ def test_step_parity_p1_rk3_shock_bubble_3d_without_graphs(cuda):
     case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25,
                                      scheme=dict(order=1, rk_stages=3))
     run_pair(case, st, 4, "sb3d-p1rk3-eager", graphs=False)
+
+
+@pytest.mark.gpu
+def test_step_parity_per_cell_bounds_lattice_1ppc(cuda):
+    # per-cell split/merge bounds (no reference routine): merges only in cells
+    # above ppc_min, no split past ppc_max; counts and order bit-exact
+    case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=1, jitter=0.0,
+                                     scheme=dict(ppc_min=1, ppc_max=4))
+    a, events = run_pair(case, st, 40, "tp2d-1ppc-bounds")
+    assert events > 0

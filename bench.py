@@ -1,13 +1,16 @@
 """Benchmark of the PG-DG time step on one MI355X.
 
-Workload (default): the 3-D shock-bubble step, channel 256x128x128 P1 DG cells
-with 8 particles per cell (2x2x2 sub-lattice, seeded jitter), air with a
-helium sphere hit by a Mach 1.22 shock; the reference's two-stage step with
-split/merge, FP64.  `value` is whole-job cell-steps/s with the state resident
-in HBM.  A secondary measurement runs the batched MGFM solve on 1e7 interface
-face pairs (stiffened gas) and reports Newton iterations per face.
+Workload (default): the 3-D shock-bubble step, channel 256x128x128 P2 DG cells
+(10 modes, 3-point Gauss rules) with 8 particles per cell (2x2x2 sub-lattice,
+seeded jitter), air with a helium sphere hit by a Mach 1.22 shock; SSP-RK3
+stages, split/merge with per-cell bounds [4, 16].  Plain FP64 and mixed forms
+are both run; `value` is the mixed form's whole-job cell-steps/s with the state
+resident in HBM.  `--scheme p1` runs the reference's P1 basis and two-stage
+step instead.  Secondary measurements: the batched MGFM solve on 1e7
+interface face pairs (plain and learned initial guess) and the MLS fit.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size full|medium|small]
+                       [--scheme p2rk3|p1]
 """
 from __future__ import annotations
 
@@ -23,13 +26,19 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 SIZES = {"full": (256, 128, 128), "medium": (128, 64, 64), "small": (32, 16, 16)}
+SCHEMES = {
+    # the named workload: P2 basis, SSP-RK3, per-cell particle bounds [4, 16]
+    "p2rk3": dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16),
+    # the reference's own scheme: P1 basis, two-stage Heun, volume criteria
+    "p1": dict(order=1, rk_stages=2),
+}
 
 
 def stage_bytes(stage: str, case, n: int, nwall: int) -> float:
     """Algorithmic HBM bytes of one launch of a stage: every input element
     read once, every output element written once (FP64 = 8 B, int32 = 4 B)."""
     D, F = case.dim, case.nphase
-    U, M = D + 2, D + 1
+    U, M = D + 2, case.M
     nc = case.n[0] * case.n[1] * case.n[2]
     bins = (n + nwall) * 4 + (nc + 1) * 4
     if stage == "pst":
@@ -77,13 +86,13 @@ def profiled_traffic(kernel: str, size: str):
     return None
 
 
-def cpu_baseline(sample=(48, 24, 24), steps=3):
+def cpu_baseline(scheme, sample=(48, 24, 24), steps=2):
     """The oracle (scalar C++ restatement of the reference step) on a reduced
     grid of the same workload, timed on this host."""
     from oracle.pyoracle_step import OracleSim
     from pgdg_p2_amd import cases
     case, st = cases.shock_bubble_3d(nx=sample[0], ny=sample[1], nz=sample[2], ppc_axis=2,
-                                     jitter=0.25)
+                                     jitter=0.25, scheme=scheme)
     sim = OracleSim(case, st)
     sim.step(1)  # first step (particle dt path) excluded
     t0 = time.perf_counter()
@@ -213,6 +222,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default="full", choices=list(SIZES))
+    ap.add_argument("--scheme", default="p2rk3", choices=list(SCHEMES))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mgfm", action="store_true")
     ap.add_argument("--no-mls", action="store_true")
@@ -259,7 +269,7 @@ def main():
     def run_form(precision: int):
         t_setup = time.perf_counter()
         case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25,
-                                         cap_factor=1.0)
+                                         cap_factor=1.0, scheme=SCHEMES[args.scheme])
         case.precision = precision
         n0 = st.n
         gpu = GpuSim(case, st, cap=int(n0 * 2.2) + 4096)
@@ -329,9 +339,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 state and fluxes, f32 particle-shifting pair terms (f64 sums)",
         "data": "synthetic",
-        "config": {"workload": f"3-D shock-bubble step, {nx}x{ny}x{nz} cells, 8 particles/cell, "
-                               "P1 DG (reference basis), reference two-stage step with split/merge, "
-                               "mixed form",
+        "config": {"workload": f"3-D shock-bubble step, {nx}x{ny}x{nz} cells, 8 particles/cell, " + (
+                               "P2 DG (10 modes, 3-point Gauss), SSP-RK3, split/merge with per-cell "
+                               "bounds [4,16]" if args.scheme == "p2rk3" else
+                               "P1 DG (reference basis), reference two-stage step with split/merge")
+                               + ", mixed form",
+                   "scheme": args.scheme,
                    "cells": nc, "particles_start": R["n_start"], "particles_end": R["n_end"],
                    "parallelism": "single GPU"},
         "particles_per_s": (R["n_start"] + R["n_end"]) / 2 * args.steps / el,
@@ -355,7 +368,7 @@ def main():
     if not args.no_mls:
         line["mls"] = mls_bench(args.mls_n)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline(SCHEMES[args.scheme])
         line["cpu_baseline"]["host_cpus"] = os.cpu_count()
     print(json.dumps(line))
 

@@ -226,6 +226,7 @@ static void fill_geo(const pgdg_case& cs, Geo& g) {
   g.inlet_p = cs.inlet_p;
   g.radius = (int)std::ceil(kCellRatio / (cs.p_g / cs.p_sz));  // S3:level_infor.cpp:155
   g.ord = cs.order;
+  g.prec = cs.precision;
   if (cs.order == 1) {
     g.Gp[0] = -1. / std::sqrt(3.);  // S3:mpm_ini.cpp:37-41
     g.Gp[1] = 1. / std::sqrt(3.);

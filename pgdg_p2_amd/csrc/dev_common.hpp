@@ -47,6 +47,7 @@ struct Geo {
   double probe_c[4], probe_s[4], sqrt2;
   double vol_max, vol_min, dss0;
   int ord;              // 1: P1 basis, 2: P2 basis
+  int prec;             // PGDG_PREC_FP64 or PGDG_PREC_MIXED
   int ppc_min, ppc_max; // per-cell particle-count bounds of split/merge (0: off)
 };
 
@@ -227,5 +228,7 @@ __device__ inline double kern_dW(double dist, double h) {
 
 __device__ inline double amax1(double a, double b) { return a >= b ? a : b; }
 __device__ inline double amin1(double a, double b) { return a <= b ? a : b; }
+__device__ inline float amax1(float a, float b) { return a >= b ? a : b; }
+__device__ inline float amin1(float a, float b) { return a <= b ? a : b; }
 
 }  // namespace pgdg

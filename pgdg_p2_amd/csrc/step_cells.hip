@@ -575,26 +575,40 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
 // runtime axis: it selects through comparisons rather than array indexing,
 // so the operands stay in registers (a runtime index into a local array
 // would place it in scratch memory).
-template <int D>
-__device__ inline double pick_axis(const double* v, int b) {
+// Stiffened-gas EOS in arithmetic T (the same operations as eos_p and
+// eos_rhoe for T = double).
+template <typename T>
+struct EosT {
+  T g, pinf, gm1, gpinf;
+  __device__ explicit EosT(const Eos& m)
+      : g((T)m.gamma), pinf((T)m.pinf), gm1((T)(m.gamma - 1)), gpinf((T)(m.gamma * m.pinf)) {}
+  __device__ T p(T rhoe) const { return rhoe * gm1 - gpinf; }
+  __device__ T rhoe(T p_) const { return (p_ + gpinf) / gm1; }
+};
+
+template <int D, typename T>
+__device__ inline T pick_axis(const T* v, int b) {
   return b == 0 ? v[0] : (b == 1 || D == 2 ? v[1] : v[2]);
 }
 
-template <int D>
-__device__ inline void flux_col(double rho, const double* rv, double E, const double* v, double P,
-                                int b, double* out) {
+template <int D, typename T>
+__device__ inline void flux_col(T rho, const T* rv, T E, const T* v, T P, int b, T* out) {
   constexpr int U = D + 2;
-  const double vb = pick_axis<D>(v, b);
+  const T vb = pick_axis<D>(v, b);
   out[0] = rho * vb;
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    const double f = rv[a] * vb;
+    const T f = rv[a] * vb;
     out[1 + a] = a == b ? f + P : f;
   }
   out[U - 1] = E * vb + vb * P;
 }
 
-template <int D, int ORD, int BS>
+// T: the arithmetic of the quadrature (double: the reference's FP64 order,
+// bit-identical to the oracle; float: the mixed form, FP32 evaluation, fluxes
+// and sums with the MGFM Newton solve in FP64 and the mass division and
+// stored rates in FP64).
+template <int D, int ORD, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NF = 2 * D, NQ = Basis<D, ORD>::NQ;
   const int nc = g.nc;
@@ -602,23 +616,26 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
   if (c >= nc || !used[c]) return;
   int i, j, k;
   cell_ijk(g, c, i, j, k);
+  // geometry (offsets, weights) stays FP64; the state arithmetic is T
   double xc[3];
   cell_center<D>(g, i, j, k, xc);
   const bool s3 = g.sem == PGDG_SEM_S3;
   const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
-  const double voli = D == 3 ? g.ps[0] * g.ps[1] * g.ps[2] : g.ps[0] * g.ps[1];
+  const double vol64 = D == 3 ? g.ps[0] * g.ps[1] * g.ps[2] : g.ps[0] * g.ps[1];
+  const T voli = (T)vol64;
   for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
     if (ss >= g.F) break;
     if (C.ph_state[ss * nc + c] != 1) continue;
     const Eos me{g.gamma[ss], g.pinf[ss]};
-    double cf[U][M], lm[U];
+    const EosT<T> mt(me);
+    T cf[U][M], lm[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       lm[u] = C.lim[(ss * U + u) * nc + c];
 #pragma unroll
       for (int m = 0; m < M; m++) cf[u][m] = C.coef[CIDX(ss, u, m)];
     }
-    double dc[U][M];
+    T dc[U][M];
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
@@ -630,32 +647,36 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
           double dp[3] = {0, 0, 0};
           dp[0] = g.Gp[g1] * hp[0];
           dp[1] = g.Gp[g2] * hp[1];
-          double gw = g.Gw[g1] * g.Gw[g2];
+          T gw = g.Gw[g1] * g.Gw[g2];
           if (D == 3) {
             dp[2] = g.Gp[g3] * hp[2];
             gw = g.Gw[g1] * g.Gw[g2] * g.Gw[g3];
           }
-          double phv[B];
-          basis_tail<D, ORD>(dp, g.q12, phv);
-          double Up[U];
+          const T dpT[3] = {(T)dp[0], (T)dp[1], (T)dp[2]};
+          double ph64[B];
+          basis_tail<D, ORD>(dp, g.q12, ph64);
+          T phv[B];
+#pragma unroll
+          for (int b = 0; b < B; b++) phv[b] = (T)ph64[b];
+          T Up[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {
-            double acc = 0;
+            T acc = 0;
 #pragma unroll
             for (int b = 0; b < B; b++) acc += cf[u][1 + b] * phv[b];
             Up[u] = cf[u][0] + lm[u] * acc;
           }
-          double rv[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+          T rv[3] = {0, 0, 0}, v[3] = {0, 0, 0};
 #pragma unroll
           for (int a = 0; a < D; a++) rv[a] = Up[1 + a];
 #pragma unroll
           for (int a = 0; a < D; a++) v[a] = rv[a] / Up[0];
-          double sq = 0;
+          T sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
-          double e_p = Up[U - 1] - 0.5 * sq / Up[0];
-          double P_p = eos_p(e_p, me);
-          double col[D][U];
+          T e_p = Up[U - 1] - T(0.5) * sq / Up[0];
+          T P_p = mt.p(e_p);
+          T col[D][U];
 #pragma unroll
           for (int b = 0; b < D; b++) flux_col<D>(Up[0], rv, Up[U - 1], v, P_p, b, col[b]);
 #pragma unroll
@@ -667,16 +688,16 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
             for (int u = 0; u < U; u++) {
               if constexpr (D == 2) {
-                dc[u][3] += (col[0][u] * dp[1] + col[1][u] * dp[0]) * gw * voli;
-                dc[u][4] += (col[0][u] * (2. * dp[0])) * gw * voli;
-                dc[u][5] += (col[1][u] * (2. * dp[1])) * gw * voli;
+                dc[u][3] += (col[0][u] * dpT[1] + col[1][u] * dpT[0]) * gw * voli;
+                dc[u][4] += (col[0][u] * (T(2.) * dpT[0])) * gw * voli;
+                dc[u][5] += (col[1][u] * (T(2.) * dpT[1])) * gw * voli;
               } else {
-                dc[u][4] += (col[0][u] * dp[1] + col[1][u] * dp[0]) * gw * voli;
-                dc[u][5] += (col[0][u] * dp[2] + col[2][u] * dp[0]) * gw * voli;
-                dc[u][6] += (col[1][u] * dp[2] + col[2][u] * dp[1]) * gw * voli;
-                dc[u][7] += (col[0][u] * (2. * dp[0])) * gw * voli;
-                dc[u][8] += (col[1][u] * (2. * dp[1])) * gw * voli;
-                dc[u][9] += (col[2][u] * (2. * dp[2])) * gw * voli;
+                dc[u][4] += (col[0][u] * dpT[1] + col[1][u] * dpT[0]) * gw * voli;
+                dc[u][5] += (col[0][u] * dpT[2] + col[2][u] * dpT[0]) * gw * voli;
+                dc[u][6] += (col[1][u] * dpT[2] + col[2][u] * dpT[1]) * gw * voli;
+                dc[u][7] += (col[0][u] * (T(2.) * dpT[0])) * gw * voli;
+                dc[u][8] += (col[1][u] * (T(2.) * dpT[1])) * gw * voli;
+                dc[u][9] += (col[2][u] * (T(2.) * dpT[2])) * gw * voli;
               }
             }
           }
@@ -686,12 +707,12 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
       FaceGeo<D> fg;
       face_geo<D>(g, i, j, k, xc, f, fg);
       const int ax = fg.axis;
-      const double sgn = fg.sgn;
-      double nrm[3] = {0, 0, 0};
+      const T sgn = fg.sgn;
+      T nrm[3] = {0, 0, 0};
 #pragma unroll
       for (int a = 0; a < 3; a++)
         if (a == ax) nrm[a] = sgn;
-      double DFn[M][U];
+      T DFn[M][U];
 #pragma unroll
       for (int m = 0; m < M; m++)
 #pragma unroll
@@ -699,7 +720,8 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
       const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
       for (int g1 = 0; g1 < NQ; g1++)
         for (int g2 = 0; g2 < (D == 3 ? NQ : 1); g2++) {
-          double dp[3], gw;
+          double dp[3];
+          T gw;
           if (D == 3) {
 #pragma unroll
             for (int a = 0; a < 3; a++) dp[a] = g.Gp[g1] * fg.tan0[a] + g.Gp[g2] * fg.tan1[a];
@@ -712,33 +734,36 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
           double dri[3];
 #pragma unroll
           for (int a = 0; a < 3; a++) dri[a] = dp[a] + fg.cen[a] - xc[a];
-          double Nb[M];
-          Nb[0] = 1.;
-          basis_tail<D, ORD>(dri, g.q12, Nb + 1);
-          double Ui[U];
+          double nb64[B];
+          basis_tail<D, ORD>(dri, g.q12, nb64);
+          T Nb[M];
+          Nb[0] = T(1.);
+#pragma unroll
+          for (int b = 0; b < B; b++) Nb[1 + b] = (T)nb64[b];
+          T Ui[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {
-            double acc = 0;
+            T acc = 0;
 #pragma unroll
             for (int b = 0; b < B; b++) acc += cf[u][1 + b] * Nb[1 + b];
             Ui[u] = cf[u][0] + lm[u] * acc;
           }
-          const double rho_i = Ui[0], E_i = Ui[U - 1];
-          double rvi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+          const T rho_i = Ui[0], E_i = Ui[U - 1];
+          T rvi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
 #pragma unroll
           for (int a = 0; a < D; a++) rvi[a] = Ui[1 + a];
 #pragma unroll
           for (int a = 0; a < D; a++) vi[a] = rvi[a] / rho_i;
-          double sq = 0;
+          T sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) sq += rvi[a] * rvi[a];
-          double e_i = E_i - 0.5 * sq / rho_i;
-          double P_i = eos_p(e_i, me);
-          double ci = sqrt(g.gamma[ss] * (g.pinf[ss] + P_i) / rho_i);
-          double vni = 0;
+          T e_i = E_i - T(0.5) * sq / rho_i;
+          T P_i = mt.p(e_i);
+          T ci = sqrt(mt.g * (mt.pinf + P_i) / rho_i);
+          T vni = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
-          double DFi[U];
+          T DFi[U];
           flux_col<D>(rho_i, rvi, E_i, vi, P_i, ax, DFi);
           if (fg.b == 0) {
             const int nb = fg.nbr;
@@ -749,32 +774,35 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
             double drj[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) drj[a] = dp[a] + fg.cen[a] - xn[a];
-            double phj[B];
-            basis_tail<D, ORD>(drj, g.q12, phj);
+            double pj64[B];
+            basis_tail<D, ORD>(drj, g.q12, pj64);
+            T phj[B];
+#pragma unroll
+            for (int b = 0; b < B; b++) phj[b] = (T)pj64[b];
             for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
               if (s1 >= g.F) break;
-              double alph_j = C.alph[s1 * nc + nb];
+              T alph_j = C.alph[s1 * nc + nb];
               if (alph_j == 0) continue;
-              double Uj[U];
+              T Uj[U];
 #pragma unroll
               for (int u = 0; u < U; u++) {
-                double acc = 0;
+                T acc = 0;
 #pragma unroll
-                for (int b = 0; b < B; b++) acc += C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * phj[b];
-                Uj[u] = C.coef[(((size_t)s1 * U + u) * M) * nc + nb] + C.lim[(s1 * U + u) * nc + nb] * acc;
+                for (int b = 0; b < B; b++) acc += (T)C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * phj[b];
+                Uj[u] = (T)C.coef[(((size_t)s1 * U + u) * M) * nc + nb] + (T)C.lim[(s1 * U + u) * nc + nb] * acc;
               }
-              double rho_j = Uj[0], E_j = Uj[U - 1];
-              double rvj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
+              T rho_j = Uj[0], E_j = Uj[U - 1];
+              T rvj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
 #pragma unroll
               for (int a = 0; a < D; a++) rvj[a] = Uj[1 + a];
 #pragma unroll
               for (int a = 0; a < D; a++) vj[a] = rvj[a] / rho_j;
-              double sqj = 0;
+              T sqj = 0;
 #pragma unroll
               for (int a = 0; a < D; a++) sqj += vj[a] * vj[a];
-              double rhoe_j = E_j - 0.5 * sqj * rho_j;
-              double P_j = eos_p(rhoe_j, Eos{g.gamma[s1], g.pinf[s1]});
-              double vnj = 0;
+              T rhoe_j = E_j - T(0.5) * sqj * rho_j;
+              T P_j = EosT<T>(Eos{g.gamma[s1], g.pinf[s1]}).p(rhoe_j);
+              T vnj = 0;
 #pragma unroll
               for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
               if (ss != s1) {
@@ -788,19 +816,19 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
                 for (int a = 0; a < D; a++) vj[a] = vi[a] - vni * nrm[a] + vnj * nrm[a];
 #pragma unroll
                 for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
-                rhoe_j = eos_rhoe(P_j, me);
-                double sq2 = 0;
+                rhoe_j = mt.rhoe(P_j);
+                T sq2 = 0;
 #pragma unroll
                 for (int a = 0; a < D; a++) sq2 += vj[a] * vj[a];
-                E_j = rhoe_j + 0.5 * rho_j * sq2;
+                E_j = rhoe_j + T(0.5) * rho_j * sq2;
               }
-              double cj = sqrt(g.gamma[ss] * (g.pinf[ss] + P_j) / rho_j);
-              double Ujs[U];
+              T cj = sqrt(mt.g * (mt.pinf + P_j) / rho_j);
+              T Ujs[U];
               Ujs[0] = rho_j;
 #pragma unroll
               for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
               Ujs[U - 1] = E_j;
-              double DFj[U];
+              T DFj[U];
               flux_col<D>(rho_j, rvj, E_j, vj, P_j, ax, DFj);
               vni = 0;
 #pragma unroll
@@ -808,17 +836,17 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
               vnj = 0;
 #pragma unroll
               for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
-              double SL = amin1(vni - ci, vnj - cj);
-              double SR = amax1(vni + ci, vnj + cj);
-              double S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
+              T SL = amin1(vni - ci, vnj - cj);
+              T SR = amax1(vni + ci, vnj + cj);
+              T S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
                            (rho_i * (SL - vni) - rho_j * (SR - vnj));
-              double Fh[U];
+              T Fh[U];
               if (SL > 0) {
 #pragma unroll
                 for (int u = 0; u < U; u++) Fh[u] = DFi[u];
               } else if (SL <= 0 && S_s >= 0) {
-                double rho_sL = rho_i * (SL - vni) / (SL - S_s);
-                double UsL[U];
+                T rho_sL = rho_i * (SL - vni) / (SL - S_s);
+                T UsL[U];
                 UsL[0] = rho_sL;
 #pragma unroll
                 for (int a = 0; a < D; a++) UsL[1 + a] = rho_sL * (vi[a] + (S_s - vni) * nrm[a]);
@@ -826,8 +854,8 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
                 for (int u = 0; u < U; u++) Fh[u] = DFi[u] + SL * (UsL[u] - Ui[u]) * sgn;
               } else if (S_s <= 0 && SR >= 0) {
-                double rho_sR = rho_j * (SR - vnj) / (SR - S_s);
-                double UsR[U];
+                T rho_sR = rho_j * (SR - vnj) / (SR - S_s);
+                T UsR[U];
                 UsR[0] = rho_sR;
 #pragma unroll
                 for (int a = 0; a < D; a++) UsR[1 + a] = rho_sR * (vj[a] + (S_s - vnj) * nrm[a]);
@@ -849,19 +877,19 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
           } else if (wall_like) {
 #pragma unroll
             for (int bi = 0; bi < M; bi++) {
-              const double t = P_i * Nb[bi] * gw;
+              const T t = P_i * Nb[bi] * gw;
 #pragma unroll
               for (int a = 0; a < D; a++)
                 if (a == ax) DFn[bi][1 + a] += t;
             }
           } else {  // S3 inlet / outlet / symmetry: HLL against a ghost state
-            double rho_j = rho_i;
-            double vj[3] = {0, 0, 0}, rvj[3] = {0, 0, 0};
+            T rho_j = rho_i;
+            T vj[3] = {0, 0, 0}, rvj[3] = {0, 0, 0};
 #pragma unroll
             for (int a = 0; a < D; a++) vj[a] = vi[a];
 #pragma unroll
             for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
-            double P_j = P_i, E_j = E_i;
+            T P_j = P_i, E_j = E_i;
             if (fg.b == -1) {
               rho_j = g.inlet_rho;
 #pragma unroll
@@ -869,10 +897,10 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
               for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
               P_j = g.inlet_p;
-              double sqv = 0;
+              T sqv = 0;
 #pragma unroll
               for (int a = 0; a < D; a++) sqv += vj[a] * vj[a];
-              E_j = eos_rhoe(P_j, me) + 0.5 * sqv * rho_j;
+              E_j = mt.rhoe(P_j) + T(0.5) * sqv * rho_j;
             }
             if (j == g.N[1] - 1 || j == 0) {
               vj[1] = -vi[1];
@@ -884,20 +912,20 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
               for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
             }
-            double Ujs[U];
+            T Ujs[U];
             Ujs[0] = rho_j;
 #pragma unroll
             for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
             Ujs[U - 1] = E_j;
-            double cj = sqrt(g.gamma[ss] * (g.pinf[ss] + P_j) / rho_j);
-            double vnj = 0;
+            T cj = sqrt(mt.g * (mt.pinf + P_j) / rho_j);
+            T vnj = 0;
 #pragma unroll
             for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
-            double DFj[U];
+            T DFj[U];
             flux_col<D>(rho_j, rvj, E_j, vj, P_j, ax, DFj);
-            double SL = amin1(vni - ci, vnj - cj);
-            double SR = amax1(vni + ci, vnj + cj);
-            double Fh[U];
+            T SL = amin1(vni - ci, vnj - cj);
+            T SR = amax1(vni + ci, vnj + cj);
+            T Fh[U];
             if (SL > 0) {
 #pragma unroll
               for (int u = 0; u < U; u++) Fh[u] = DFi[u];
@@ -941,7 +969,7 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
       }
     }
 #pragma unroll
-    for (int m = 0; m < M; m++) mij[m] *= voli;
+    for (int m = 0; m < M; m++) mij[m] *= vol64;
 #pragma unroll
     for (int m = 0; m < M; m++) {
       double inv = 1. / (mij[m] + 1e-20);
@@ -1085,11 +1113,20 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
       }
       break;
     case CELL_RHS:
-      switch (block) {
-        case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
-        case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
-        case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
-        default: return PGDG_ERR_ARG;
+      if (g.prec == PGDG_PREC_MIXED) {
+        switch (block) {
+          case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, float>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+          case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, float>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+          case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256, float>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+          default: return PGDG_ERR_ARG;
+        }
+      } else {
+        switch (block) {
+          case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, double>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+          case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, double>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+          case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256, double>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+          default: return PGDG_ERR_ARG;
+        }
       }
       break;
     case CELL_UPDATE:

@@ -609,7 +609,8 @@ __device__ inline void flux_col(T rho, const T* rv, T E, const T* v, T P, int b,
 // and sums with the MGFM Newton solve in FP64 and the mass division and
 // stored rates in FP64).
 template <int D, int ORD, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NF = 2 * D, NQ = Basis<D, ORD>::NQ;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
@@ -712,9 +713,13 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
       for (int a = 0; a < 3; a++)
         if (a == ax) nrm[a] = sgn;
-      T DFn[M][U];
+      // FP32 form: face terms go straight into the rates (scaled by -n S);
+      // the FP64 form sums each face first, in the oracle's order
+      constexpr bool kDirect = sizeof(T) == 4;
+      const T fs = (T)(-(double)fg.sgn * fg.S);
+      T DFn[kDirect ? 1 : M][U];
 #pragma unroll
-      for (int m = 0; m < M; m++)
+      for (int m = 0; m < (kDirect ? 1 : M); m++)
 #pragma unroll
         for (int u = 0; u < U; u++) DFn[m][u] = 0;
       const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
@@ -872,7 +877,10 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
               for (int bi = 0; bi < M; bi++)
 #pragma unroll
-                for (int u = 0; u < U; u++) DFn[bi][u] += Fh[u] * Nb[bi] * gw * alph_j;
+                for (int u = 0; u < U; u++) {
+                  if constexpr (kDirect) dc[u][bi] += Fh[u] * Nb[bi] * gw * alph_j * fs;
+                  else DFn[bi][u] += Fh[u] * Nb[bi] * gw * alph_j;
+                }
             }
           } else if (wall_like) {
 #pragma unroll
@@ -880,7 +888,10 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
               const T t = P_i * Nb[bi] * gw;
 #pragma unroll
               for (int a = 0; a < D; a++)
-                if (a == ax) DFn[bi][1 + a] += t;
+                if (a == ax) {
+                  if constexpr (kDirect) dc[1 + a][bi] += t * fs;
+                  else DFn[bi][1 + a] += t;
+                }
             }
           } else {  // S3 inlet / outlet / symmetry: HLL against a ghost state
             T rho_j = rho_i;
@@ -943,13 +954,17 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
             for (int bi = 0; bi < M; bi++)
 #pragma unroll
-              for (int u = 0; u < U; u++) DFn[bi][u] += Fh[u] * Nb[bi] * gw;
+              for (int u = 0; u < U; u++) {
+                if constexpr (kDirect) dc[u][bi] += Fh[u] * Nb[bi] * gw * fs;
+                else DFn[bi][u] += Fh[u] * Nb[bi] * gw;
+              }
           }
         }
 #pragma unroll
       for (int bi = 0; bi < M; bi++)
 #pragma unroll
-        for (int u = 0; u < U; u++) dc[u][bi] += (-(DFn[bi][u]) * sgn) * fg.S;
+        for (int u = 0; u < U; u++)
+          if constexpr (!kDirect) dc[u][bi] += (-(DFn[bi][u]) * sgn) * fg.S;
     }
     // diagonal mass matrix Vol (1, dx^2/12, ...)
     double mij[M];

@@ -999,14 +999,55 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
 // ---------------------------------------------------------------------------
 // md: 0 first stage (Euler), 1 Heun second stage (reference), 2 and 3 the
 // second and third SSP-RK3 stages.
+//
+// Two kernels: k_update_coef streams every coefficient (one lane per phase,
+// variable, mode and cell: slopes of coef and dcoef scaled by |Lim_u|, then
+// the stage combination for present phases), k_update_grid then derives the
+// cell state from the updated mode-0 coefficients.  Each value goes through
+// the same operations as in the reference's single loop.
+template <int D, int ORD>
+__global__ void __launch_bounds__(256) k_update_coef(Geo g, CellDev C, const int32_t* __restrict__ used,
+                                                     const DevScalars* __restrict__ sc, int md) {
+  constexpr int U = D + 2, M = Basis<D, ORD>::M;
+  const int nc = g.nc;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)g.F * U * M * nc) return;
+  const int c = (int)(t % nc);
+  const int r = (int)(t / nc);  // (s * U + u) * M + m
+  const int m = r % M, su = r / M, s = su / U;
+  if (!used[c]) return;
+  double cf = C.coef[t], dcf = C.dcoef[t];
+  if (m >= 1) {
+    const double l = fabs(C.lim[(size_t)su * nc + c]);
+    dcf *= l;
+    cf *= l;
+  }
+  if (C.ph_state[s * nc + c] == 1) {
+    const double dt = sc->dt;
+    if (md == 0) {
+      C.coef0[t] = cf;
+      cf += dcf * dt;
+    } else if (md == 1) {
+      cf = 0.5 * (dcf * dt + cf + C.coef0[t]);
+    } else if (md == 2) {
+      cf = 0.75 * C.coef0[t] + 0.25 * (cf + dcf * dt);
+    } else {
+      cf = (1. / 3.) * C.coef0[t] + (2. / 3.) * (cf + dcf * dt);
+    }
+  }
+  C.coef[t] = cf;
+  C.dcoef[t] = dcf;
+}
+
 template <int D, int ORD>
 __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int32_t* __restrict__ used,
                                                      const DevScalars* __restrict__ sc, int md) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M;
+  (void)sc;
+  (void)md;
   const int nc = g.nc;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
-  const double dt = sc->dt;
   const bool s3 = g.sem == PGDG_SEM_S3;
   double rva[3] = {0, 0, 0};
   double rho_a = 0, P_a = 0;
@@ -1017,54 +1058,14 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
     if (s >= g.F) break;
     C.c[s * nc + c] = 0;
     C.ac[s * nc + c] = 0;
-    double cf[U][M], dcf[U][M];
-#pragma unroll
-    for (int t = 0; t < U; t++) {
-      double l = fabs(C.lim[(s * U + t) * nc + c]);
-#pragma unroll
-      for (int m = 0; m < M; m++) {
-        cf[t][m] = C.coef[CIDX(s, t, m)];
-        dcf[t][m] = C.dcoef[CIDX(s, t, m)];
-      }
-#pragma unroll
-      for (int m = 1; m < M; m++) {
-        dcf[t][m] *= l;
-        cf[t][m] *= l;
-      }
-    }
     double alph = C.alph[s * nc + c];
     double rho, rv[3], E;
     if (C.ph_state[s * nc + c] == 1) {
-      if (md == 0) {
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-          for (int m = 0; m < M; m++) {
-            C.coef0[CIDX(s, u, m)] = cf[u][m];
-            cf[u][m] += dcf[u][m] * dt;
-          }
-      } else if (md == 1) {
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-          for (int m = 0; m < M; m++) cf[u][m] = 0.5 * (dcf[u][m] * dt + cf[u][m] + C.coef0[CIDX(s, u, m)]);
-      } else if (md == 2) {
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-          for (int m = 0; m < M; m++) cf[u][m] = 0.75 * C.coef0[CIDX(s, u, m)] + 0.25 * (cf[u][m] + dcf[u][m] * dt);
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-          for (int m = 0; m < M; m++)
-            cf[u][m] = (1. / 3.) * C.coef0[CIDX(s, u, m)] + (2. / 3.) * (cf[u][m] + dcf[u][m] * dt);
-      }
-      rho = cf[0][0];
+      rho = C.coef[CIDX(s, 0, 0)];
       rv[2] = C.rhov[(s * 3 + 2) * nc + c];
 #pragma unroll
-      for (int a = 0; a < D; a++) rv[a] = cf[1 + a][0];
-      E = cf[U - 1][0];
+      for (int a = 0; a < D; a++) rv[a] = C.coef[CIDX(s, 1 + a, 0)];
+      E = C.coef[CIDX(s, U - 1, 0)];
       C.rho[s * nc + c] = rho;
 #pragma unroll
       for (int a = 0; a < D; a++) C.rhov[(s * 3 + a) * nc + c] = rv[a];
@@ -1079,8 +1080,9 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
       cP = eos_p(e_j, Eos{g.gamma[s], g.pinf[s]});
       double cs = sqrt(g.gamma[s] * (g.pinf[s] + cP) / rho);
       C.c[s * nc + c] = cs;
+      const double d0 = C.dcoef[CIDX(s, 0, 0)];
 #pragma unroll
-      for (int a = 0; a < D; a++) ca[a] = (dcf[1 + a][0] - dcf[0][0] * cv[a]) / rho;
+      for (int a = 0; a < D; a++) ca[a] = (C.dcoef[CIDX(s, 1 + a, 0)] - d0 * cv[a]) / rho;
       double sqa = 0;
 #pragma unroll
       for (int a = 0; a < D; a++) sqa += ca[a] * ca[a];
@@ -1090,13 +1092,6 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
 #pragma unroll
       for (int a = 0; a < 3; a++) rv[a] = C.rhov[(s * 3 + a) * nc + c];
     }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-      for (int m = 0; m < M; m++) {
-        C.coef[CIDX(s, u, m)] = cf[u][m];
-        C.dcoef[CIDX(s, u, m)] = dcf[u][m];
-      }
 #pragma unroll
     for (int a = 0; a < 3; a++) rva[a] += rv[a] * alph;
     rho_a += rho * alph;
@@ -1145,6 +1140,9 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
       }
       break;
     case CELL_UPDATE:
+      hipLaunchKernelGGL((k_update_coef<D, ORD>),
+                         dim3((unsigned)(((int64_t)g.F * (D + 2) * Basis<D, ORD>::M * nc + 255) / 256)),
+                         dim3(256), 0, st, g, C, used, sc, arg);
       hipLaunchKernelGGL((k_update_grid<D, ORD>), dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
                          sc, arg);
       break;

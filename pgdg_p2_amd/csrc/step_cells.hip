@@ -609,9 +609,12 @@ __device__ inline void flux_col(T rho, const T* rv, T E, const T* v, T P, int b,
 // and sums with the MGFM Newton solve in FP64 and the mass division and
 // stored rates in FP64).
 template <int D, int ORD, int BS, typename T>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
-k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
+__global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NF = 2 * D, NQ = Basis<D, ORD>::NQ;
+  // per-lane LDS slot for the neighbour's coefficients and limiter of its
+  // first present phase, loaded once per face instead of once per face point
+  __shared__ T s_nb[U * M + U][BS];
+  const int ln = threadIdx.x;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
@@ -723,6 +726,17 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
 #pragma unroll
         for (int u = 0; u < U; u++) DFn[m][u] = 0;
       const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
+      int s_first = -1;
+      if (fg.b == 0) {
+        for (int s1 = 0; s1 < PGDG_MAX_PHASE && s_first < 0; s1++)
+          if (s1 < g.F && C.alph[s1 * nc + fg.nbr] != 0) s_first = s1;
+        if (s_first >= 0) {
+#pragma unroll
+          for (int v = 0; v < U * M; v++) s_nb[v][ln] = (T)C.coef[((size_t)s_first * U * M + v) * nc + fg.nbr];
+#pragma unroll
+          for (int u = 0; u < U; u++) s_nb[U * M + u][ln] = (T)C.lim[(s_first * U + u) * nc + fg.nbr];
+        }
+      }
       for (int g1 = 0; g1 < NQ; g1++)
         for (int g2 = 0; g2 < (D == 3 ? NQ : 1); g2++) {
           double dp[3];
@@ -789,12 +803,22 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
               T alph_j = C.alph[s1 * nc + nb];
               if (alph_j == 0) continue;
               T Uj[U];
+              if (s1 == s_first) {
 #pragma unroll
-              for (int u = 0; u < U; u++) {
-                T acc = 0;
+                for (int u = 0; u < U; u++) {
+                  T acc = 0;
 #pragma unroll
-                for (int b = 0; b < B; b++) acc += (T)C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * phj[b];
-                Uj[u] = (T)C.coef[(((size_t)s1 * U + u) * M) * nc + nb] + (T)C.lim[(s1 * U + u) * nc + nb] * acc;
+                  for (int b = 0; b < B; b++) acc += s_nb[u * M + 1 + b][ln] * phj[b];
+                  Uj[u] = s_nb[u * M][ln] + s_nb[U * M + u][ln] * acc;
+                }
+              } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                  T acc = 0;
+#pragma unroll
+                  for (int b = 0; b < B; b++) acc += (T)C.coef[(((size_t)s1 * U + u) * M + 1 + b) * nc + nb] * phj[b];
+                  Uj[u] = (T)C.coef[(((size_t)s1 * U + u) * M) * nc + nb] + (T)C.lim[(s1 * U + u) * nc + nb] * acc;
+                }
               }
               T rho_j = Uj[0], E_j = Uj[U - 1];
               T rvj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};

@@ -67,16 +67,19 @@ def stage_bytes(stage: str, case, n: int, nwall: int) -> float:
     return 0.0
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "step_shock_bubble_256x128x128_kernels_and_traffic.txt"
+TRAFFIC_PROFILES = {
+    "p2rk3": ROOT / "profiles" / "step_shock_bubble_256x128x128_p2_rk3_kernels_and_traffic.txt",
+}
 
 
-def profiled_traffic(kernel: str, size: str):
+def profiled_traffic(kernel: str, size: str, scheme: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
     calibration), for the full-size workload only; None if absent."""
-    if size != "full" or not TRAFFIC_PROFILE.exists():
+    prof = TRAFFIC_PROFILES.get(scheme)
+    if size != "full" or prof is None or not prof.exists():
         return None
-    for line in TRAFFIC_PROFILE.read_text().splitlines():
+    for line in prof.read_text().splitlines():
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cols) == 4 and cols[0] == f"`{kernel}`":
             try:
@@ -317,14 +320,18 @@ def main():
     per_launch_ms = stages[dom]["ms_total"] / stages[dom]["launches"]
     alg = stage_bytes(dom, case, R["n_mid"], walls)
     achieved = alg / (per_launch_ms * 1e-3) / 1e9 if alg > 0 else None
-    kname = {"pst": "pgdg::k_pst_mixed<3>", "rhs": "pgdg::k_rhs<3>", "p2g": "pgdg::k_p2g<3>",
-             "g2p": "pgdg::k_g2p<3>"}.get(dom, "")
-    traffic = profiled_traffic(kname, args.size)
+    order = case.order
+    tb = R["tuning"] or {}
+    kname = {"pst": f"pgdg::k_pst_mixed<3, {tb.get('pst', 256)}>",
+             "rhs": f"pgdg::k_rhs<3, {order}, {tb.get('rhs', 64)}, float>",
+             "p2g": f"pgdg::k_p2g<3, {order}>", "g2p": f"pgdg::k_g2p<3, {order}>"}.get(dom, "")
+    traffic = profiled_traffic(kname, args.size, args.scheme)
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic, "alg_bytes_per_launch": alg, "avg_launch_ms": per_launch_ms}
     if traffic is not None:
-        roof["traffic_source"] = str(TRAFFIC_PROFILE.relative_to(ROOT))
+        roof["traffic_source"] = str(TRAFFIC_PROFILES[args.scheme].relative_to(ROOT))
+        roof["traffic_kernel"] = kname
 
     line = {
         "metric": "P2 DG cell-steps/s + MGFM Newton its/face, mixed vs FP64 plain at equal L1 err",

@@ -34,8 +34,10 @@ def mgfm_faces(n: int, seed: int = MGFM_SEED) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # Isolated MLS P2G + per-cell least-squares solve: n^3 cells on [0,1]^3,
 # Poisson(8) particles per cell clipped to [1, 27], uniform in the cell; six
-# fields, each a sum of 8 seeded random Fourier modes plus a unit jump across
-# the sphere |x - (1/2,1/2,1/2)| = 0.3.
+# fields (rho, rho u, rho v, rho w, E, level set): the first five each a sum
+# of 8 seeded random Fourier modes plus a unit jump across the sphere
+# |x - (1/2,1/2,1/2)| = 0.3, the sixth the level set of that sphere (signed
+# distance 0.3 - |x - (1/2,1/2,1/2)|, positive inside).
 # ---------------------------------------------------------------------------
 MLS_SEED = 0x5EED0001
 MLS_FIELDS = 6
@@ -43,12 +45,16 @@ MLS_FIELDS = 6
 
 def mls_fields(x, nfields: int = MLS_FIELDS, seed: int = MLS_SEED):
     """The seeded fields at points x[3][m]: 8 Fourier modes each plus the unit
-    jump inside |x - (1/2,1/2,1/2)| < 0.3.  Returns [nfields][m]."""
+    jump inside |x - (1/2,1/2,1/2)| < 0.3; with six fields the last is the
+    sphere's level set 0.3 - |x - (1/2,1/2,1/2)|.  Returns [nfields][m]."""
     x = np.asarray(x, dtype=np.float64)
     r = np.sqrt((x[0] - 0.5) ** 2 + (x[1] - 0.5) ** 2 + (x[2] - 0.5) ** 2)
     jump = (r < 0.3).astype(np.float64)
     val = np.empty((nfields, x.shape[1]))
     for f in range(nfields):
+        if nfields == MLS_FIELDS and f == MLS_FIELDS - 1:
+            val[f] = 0.3 - r  # level set of the interface
+            continue
         fr = np.random.Generator(np.random.PCG64(seed + f))
         a = fr.uniform(-0.5, 0.5, 8)
         kv = fr.integers(-3, 4, (8, 3))

@@ -107,6 +107,37 @@ def cpu_baseline(scheme, sample=(48, 24, 24), steps=2):
                       f"steps 2..{steps + 1}, same initial condition, oracle FP64 single thread"}
 
 
+def three_phase_bench(steps=20, warmup=2, nx=1792, ny=768):
+    """The 2-D three-phase triple point at 1792x768 P2 cells (gamma 1.5 / 1.4 /
+    1.67), the reference's one-particle-per-cell lattice, SSP-RK3, FP64 form:
+    cell-steps/s and split + merge events per second over `steps` steps."""
+    from pgdg_p2_amd import cases
+    from pgdg_p2_amd.step import GpuSim
+    t0 = time.perf_counter()
+    case, st = cases.triple_point_2d(nx=nx, ny=ny, ppc_axis=1, jitter=0.0, nphase=3,
+                                     scheme=dict(order=2, rk_stages=3))
+    gpu = GpuSim(case, st)
+    print(f"[3-phase] setup {time.perf_counter() - t0:.1f}s: {nx}x{ny} cells, {st.n} particles",
+          file=sys.stderr, flush=True)
+    gpu.step(warmup)
+    gpu.sync()
+    n0 = gpu.count()
+    ev = 0
+    t = time.perf_counter()
+    for _ in range(steps):
+        gpu.step(1)
+        sc = gpu.scalars()
+        ev += sc.n_split + sc.n_merge
+    gpu.sync()
+    el = time.perf_counter() - t
+    n1 = gpu.count()
+    gpu.close()
+    return {"cells": nx * ny, "particles_start": n0, "particles_end": n1, "steps": steps,
+            "ms_per_step": el / steps * 1e3, "cell_steps_per_s": nx * ny * steps / el,
+            "split_merge_events": ev, "split_merge_events_per_s": ev / el,
+            "scheme": "P2, SSP-RK3, FP64 form, 1 particle per cell lattice, 3 phases"}
+
+
 def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
     """Batched MGFM on n stiffened-gas interface face pairs resident on the
     device: faces/s and Newton iterations per face for the reference
@@ -229,6 +260,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mgfm", action="store_true")
     ap.add_argument("--no-mls", action="store_true")
+    ap.add_argument("--no-3phase", action="store_true")
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--only-mgfm", action="store_true")
@@ -372,6 +404,8 @@ def main():
     }
     if not args.no_mgfm:
         line["mgfm"] = mgfm_bench()
+    if not args.no_3phase:
+        line["three_phase_2d"] = three_phase_bench()
     if not args.no_mls:
         line["mls"] = mls_bench(args.mls_n)
     if not args.no_cpu:

@@ -4,7 +4,7 @@ from __future__ import annotations
 import ctypes as C
 
 from . import check, lib
-from .state import Case, HostState
+from .state import Case, HostState, Scalars
 
 FN = {"set_timestep": 0, "used_state_update": 1, "bin": 2, "split": 3, "merge": 4,
       "membership": 5, "neighbors": 6, "p2g": 7, "limiter": 8, "rhs": 9, "update_grid": 10,
@@ -44,6 +44,13 @@ class GpuSim:
         check(self.L.pgdg_download_state(self.h, None, C.byref(Cc), C.byref(st.s)),
               "pgdg_download_state")
         return st
+
+    def scalars(self) -> Scalars:
+        """The time-loop scalars and the last step's split/inlet/outlet/merge
+        counts, without downloading particle or cell arrays."""
+        sc = Scalars()
+        check(self.L.pgdg_download_state(self.h, None, None, C.byref(sc)), "pgdg_download_state")
+        return sc
 
     def graphs(self, on: bool):
         """Run the RK stages as captured HIP graphs (default) or launch by launch."""

@@ -376,7 +376,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 state and fluxes, f32 particle-shifting pair terms (f64 sums)",
+        "dtype": ("f64 state, MGFM and mass division; f32 DG quadrature and fluxes, "
+                  "f32 particle-shifting pair terms (f64 sums)"),
         "data": "synthetic",
         "config": {"workload": f"3-D shock-bubble step, {nx}x{ny}x{nz} cells, 8 particles/cell, " + (
                                "P2 DG (10 modes, 3-point Gauss), SSP-RK3, split/merge with per-cell "

@@ -388,6 +388,10 @@ int pgdg_graphs_enable(pgdg_ctx* ctx, int on);
  * the tuning step has run.                                                */
 int pgdg_autotune_enable(pgdg_ctx* ctx, int on);
 int pgdg_tuning_read(pgdg_ctx* ctx, int* blocks);
+/* Mixed form, 3-D: stage the particle-shifting candidates in LDS (default
+ * on) or let every lane read its candidates from global memory.  Both give
+ * the same sums bit for bit; the switch exists to compare them.           */
+int pgdg_pst_staging_enable(pgdg_ctx* ctx, int on);
 int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 
 #ifdef __cplusplus

@@ -25,6 +25,7 @@ def declare(L: C.CDLL) -> None:
     L.pgdg_profile_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_graphs_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_autotune_enable.argtypes = [_CTX, C.c_int]
+    L.pgdg_pst_staging_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_tuning_read.argtypes = [_CTX, C.POINTER(C.c_int)]
     L.pgdg_profile_read.argtypes = [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
     for name in _STAGES:

@@ -143,6 +143,9 @@ struct Ctx {
   // kernels, chosen by timing the candidates in place during the first step
   Tuning tune;
   bool autotune = true, tuned = false, tuning_now = false;
+  // mixed form: candidate entries of the particle-shifting sum staged in LDS
+  // (3-D); off = every lane walks its candidates in global memory
+  int pst_staging = 1;
   template <typename F>
   int pick_block(F launch, int* best) {
     static const int cand[3] = {64, 128, 256};
@@ -317,12 +320,9 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   B.bmax = B.bmin + 3 * (size_t)nc;
   B.ph = bi_slab;
   B.ref = bi_slab + ne;
-  const int p2g_rows_n =
-      kP2gLight + (D == 3 ? (cs.order == 2 ? p2g_heavy<3, 2>() : p2g_heavy<3, 1>())
-                          : (cs.order == 2 ? p2g_heavy<2, 2>() : p2g_heavy<2, 1>()));
+  const int p2g_rows_n = kP2gLight;
   PB.ne = ne;
   PGDG_HIP_TRY(hipMalloc((void**)&PB.light, sizeof(double) * p2g_rows_n * (size_t)ne));
-  PB.heavy = PB.light + kP2gLight * (size_t)ne;
   size_t mb_bytes = 0;
   if (cs.precision == PGDG_PREC_MIXED) {
     // boxes 32 B per cell; r1, r2 (16 B), r3 (8 B), ecell (4 B) per entry
@@ -520,6 +520,13 @@ static_assert(FN_SET_TIMESTEP == PGDG_FN_SET_TIMESTEP && FN_USED == PGDG_FN_USED
                   FN_UPDATE_PARTICLE == PGDG_FN_UPDATE_PARTICLE && FN_RUN == PGDG_FN_RUN,
               "stage numbers must match include/pgdg.h");
 
+// Particle-update argument inside a time step: the stage code, plus
+// kCopyLater for the stages before the last (0 of either scheme, 2 of
+// SSP-RK3), whose particle coefficient copies the last stage overwrites
+// before anything reads them.  Stage-level calls always write the copy.
+constexpr int kCopyLater = 0x100;
+static int g2p_code(int md) { return (md == 0 || md == 2) ? (md | kCopyLater) : md; }
+
 static int prof_slot(int fn) {
   switch (fn) {
     case 0: return PS_TIMESTEP;
@@ -602,18 +609,20 @@ int Ctx::stage_impl(int fn, int arg) {
       const int64_t n_p = capturing ? PA.cap : np, n_e = capturing ? B.ne_cap : T.n_ent;
       Profiler::Rec r;
       if (prof.on) PGDG_TRY(prof.begin(PS_G2P, st, r));
-      PGDG_TRY(launch_g2p<D_>(g, P, C, S, n_p, sc, arg, st));
+      PGDG_TRY(launch_g2p<D_>(g, P, C, S, n_p, sc, arg & 3, !(arg & kCopyLater), st));
       if (prof.on) {
         PGDG_TRY(prof.end(r, st));
         PGDG_TRY(prof.begin(PS_PST, st, r));
       }
       if (cs.precision == PGDG_PREC_MIXED) {
-        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst));
+        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst,
+                                      pst_staging));
         if (tuning_now) {  // the gathered records are in place: time the shifting kernel alone
           PGDG_TRY(pick_block([&](int b) {
-            return launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, b);
+            return launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, b, pst_staging);
           }, &tune.pst));
-          PGDG_TRY(launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, tune.pst));
+          PGDG_TRY(launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, tune.pst,
+                                               pst_staging));
         }
       } else {
         PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst));
@@ -657,7 +666,7 @@ int Ctx::run_simulation(int64_t n) {
       if (rc == PGDG_OK) rc = stage_impl<D_>(FN_LIMITER, 0);
       if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, 0);
       if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_GRID, md);
-      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, md);
+      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, g2p_code(md));
       tuning_now = false;
       PGDG_TRY(rc);
     }
@@ -673,7 +682,7 @@ int Ctx::run_simulation(int64_t n) {
       PGDG_TRY(stage<D_>(FN_LIMITER, 0));
       PGDG_TRY(stage<D_>(FN_RHS, 0));
       PGDG_TRY(stage<D_>(FN_UPDATE_GRID, md));
-      PGDG_TRY(stage<D_>(FN_UPDATE_PARTICLE, md));
+      PGDG_TRY(stage<D_>(FN_UPDATE_PARTICLE, g2p_code(md)));
     }
   }
   // lists the next step's split/merge passes read (built from the last
@@ -705,7 +714,7 @@ int Ctx::rk_stage(int md) {
   if (rc == PGDG_OK) rc = stage_impl<D_>(FN_LIMITER, 0);
   if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, 0);
   if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_GRID, md);
-  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, md);
+  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, g2p_code(md));
   capturing = false;
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
@@ -800,6 +809,8 @@ int pgdg_step(pgdg_ctx* x, int nsteps) {
 int pgdg_stage(pgdg_ctx* x, int fn, int arg) {
   if (!x) return PGDG_ERR_ARG;
   if (!x->c.loaded) return PGDG_ERR_STATE;
+  if ((fn == pgdg::FN_UPDATE_GRID || fn == pgdg::FN_UPDATE_PARTICLE) && (arg < 0 || arg > 3))
+    return PGDG_ERR_ARG;  // stage codes 0-3
   return x->c.D == 2 ? x->c.stage<2>(fn, arg) : x->c.stage<3>(fn, arg);
 }
 
@@ -833,6 +844,13 @@ int pgdg_autotune_enable(pgdg_ctx* x, int on) {
   if (!x) return PGDG_ERR_ARG;
   x->c.autotune = on != 0;
   x->c.tuned = false;
+  x->c.drop_graphs();
+  return PGDG_OK;
+}
+
+int pgdg_pst_staging_enable(pgdg_ctx* x, int on) {
+  if (!x) return PGDG_ERR_ARG;
+  x->c.pst_staging = on != 0;
   x->c.drop_graphs();
   return PGDG_OK;
 }

@@ -30,10 +30,9 @@ __device__ inline void cell_center(const Geo& g, int i, int j, int k, double* x)
 // ---------------------------------------------------------------------------
 // MLS P2G (S3:mpm.cpp:274-382)
 // ---------------------------------------------------------------------------
-template <int D, int ORD>
-__global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent,
-                             const int* __restrict__ bin_total, P2gBuf pb) {
-  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, H = p2g_heavy<D, ORD>();
+template <int D>
+__global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent, const int* __restrict__ bin_total,
+                             P2gBuf pb) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= pb.ne || e >= *bin_total) return;
   const int64_t cap = P.cap;
@@ -50,30 +49,19 @@ __global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent,
   r[4] = (double)(P.phase[p] - 1);
 #pragma unroll
   for (int q = 0; q < kP2gLight; q += 2) *reinterpret_cast<double2*>(l + q) = make_double2(r[q], r[q + 1]);
-  double h[H];
-  h[0] = P.rho[p];
-#pragma unroll
-  for (int a = 0; a < D; a++) h[1 + a] = P.rhov[a * cap + p];
-  h[U - 1] = P.E[p];
-#pragma unroll
-  for (int u = 0; u < U; u++)
-#pragma unroll
-    for (int b = 0; b < B; b++) h[U + u * B + b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
-  if (H > U * M) h[H - 1] = 0.0;
-  double* hp = pb.heavy + e * H;
-#pragma unroll
-  for (int q = 0; q < H; q += 2) *reinterpret_cast<double2*>(hp + q) = make_double2(h[q], h[q + 1]);
 }
 
 // One lane per cell, cells taken in XCD-local runs (xcd_block).  The first
-// walk over the 3^D neighbour bins reads only the light records and forms
-// the phase masses; a second walk, in the same order and only for cells
-// where a phase appears (ph_state 0 -> 1), forms the weighted state and
-// slope averages from the heavy records.
+// walk over the 3^D neighbour bins reads only the gathered records and
+// forms the phase masses; a second walk, in the same order and only for
+// cells where a phase appears (ph_state 0 -> 1, a thin layer at moving
+// interfaces), forms the weighted state and slope averages from the
+// particle arrays themselves.
 template <int D, int ORD>
-__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const int* __restrict__ bin_off,
+__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, P2gBuf pb,
+                                             const int* __restrict__ bin_off, const int* __restrict__ bin_ent,
                                              const int32_t* __restrict__ used) {
-  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, H = p2g_heavy<D, ORD>();
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
   if (c >= nc || !used[c]) return;
@@ -156,17 +144,12 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const 
             double dist = sqrt(sq);
             if (dist < g.p2g_cut) {
               double ww = kern_W<D>(dist, hi) * l23.y;
-              const double* __restrict__ hp = pb.heavy + (int64_t)e * H;
-              double h[H];
-#pragma unroll
-              for (int q2 = 0; q2 < H; q2 += 2) {
-                double2 v = *reinterpret_cast<const double2*>(hp + q2);
-                h[q2] = v.x;
-                h[q2 + 1] = v.y;
-              }
+              const int64_t p = bin_ent[e], cap = P.cap;
               double pj[U];
+              pj[0] = P.rho[p];
 #pragma unroll
-              for (int u = 0; u < U; u++) pj[u] = h[u];
+              for (int a = 0; a < D; a++) pj[1 + a] = P.rhov[a * cap + p];
+              pj[U - 1] = P.E[p];
               // the particle's polynomial at the cell centre
               const double nx[3] = {-xij[0], -xij[1], D == 3 ? -xij[2] : 0.0};
               double psi[B];
@@ -177,7 +160,7 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const 
                 double acc = 0;
 #pragma unroll
                 for (int b = 0; b < B; b++) {
-                  sl[u][b] = h[U + u * B + b];
+                  sl[u][b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
                   acc += sl[u][b] * psi[b];
                 }
                 pj[u] += acc;
@@ -1194,15 +1177,14 @@ int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_o
   // every bin entry (the device-side total bin_off[nc]) is gathered; the
   // grid covers the buffer's capacity
   (void)n_ent;
-  if (g.ord == 2) {
-    hipLaunchKernelGGL((k_p2g_gather<D, 2>), dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P,
-                       bin_ent, bin_off + g.nc, pb);
-    hipLaunchKernelGGL((k_p2g<D, 2>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, pb, bin_off, used);
-  } else {
-    hipLaunchKernelGGL((k_p2g_gather<D, 1>), dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P,
-                       bin_ent, bin_off + g.nc, pb);
-    hipLaunchKernelGGL((k_p2g<D, 1>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, pb, bin_off, used);
-  }
+  hipLaunchKernelGGL((k_p2g_gather<D>), dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P, bin_ent,
+                     bin_off + g.nc, pb);
+  if (g.ord == 2)
+    hipLaunchKernelGGL((k_p2g<D, 2>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, P, pb, bin_off,
+                       bin_ent, used);
+  else
+    hipLaunchKernelGGL((k_p2g<D, 1>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, P, pb, bin_off,
+                       bin_ent, used);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

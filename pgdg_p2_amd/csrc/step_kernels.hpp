@@ -19,18 +19,15 @@ int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, con
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
                  hipStream_t st, int block = 256);
 
-// Particle data MLS_P2G reads, gathered into bin-entry order.  Every cell
-// needs the kernel weights (position, volume, phase: the 8-double `light`
-// record, phase -1 for walls); only cells where a phase appears also need
-// the state and slopes (the `heavy` record of p2g_heavy<D>() doubles).
+// Particle data MLS_P2G's kernel weights need (position, volume, phase;
+// phase -1 for walls), gathered into bin-entry order.  The state and slopes
+// (needed only in cells where a phase appears) are read from the particle
+// arrays.
 struct P2gBuf {
   double* light;   // [ne][kP2gLight]
-  double* heavy;   // [ne][p2g_heavy<D>()]
   int64_t ne;
 };
 constexpr int kP2gLight = 6;  // x[3] (or x[2], 0), vol, phase, pad
-template <int D, int ORD = 1>
-constexpr int p2g_heavy() { return ((D + 2) * Basis<D, ORD>::M + 1) / 2 * 2; }
 
 template <int D>
 int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off, const int* bin_ent,
@@ -51,7 +48,7 @@ template <int D>
 int launch_in_ele(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, hipStream_t st);
 template <int D>
 int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
-               const DevScalars* sc, int md, hipStream_t st);
+               const DevScalars* sc, int md, bool copy, hipStream_t st);
 // Neighbour data in cell-bin order (entry e of the bins): positions after
 // and before the stage update, volume, density, sound speed, phase (10 for
 // walls) and the particle reference; plus the bounding box of the stage-start
@@ -90,11 +87,11 @@ struct MixedBins {
 template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block);
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging);
 template <int D>
 int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                             const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                            const DevScalars* sc, hipStream_t st, int block);
+                            const DevScalars* sc, hipStream_t st, int block, int staging);
 
 template <int D>
 int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32_t* used,

@@ -9,6 +9,8 @@
 // derivative uses dW/(r) = -5 C q^3 / h^2 (the reference's
 // -5 C q^3 s / h / r with s = r / h), and the per-target reciprocals are
 // formed once.  The boundary penalty and the clamps stay FP64.
+#include <climits>
+
 #include "dev_common.hpp"
 #include "hip_util.hpp"
 #include "step_kernels.hpp"
@@ -89,132 +91,292 @@ __global__ void k_cell_bbox_mixed(Geo g, MixedBins MB, const int* __restrict__ b
   MB.box[2 * c + 1] = hi;
 }
 
+// Per-target constants of the FP32 pair terms.
+struct PstConst {
+  float r2, r2_box, Cf, inv_h, dWc, inv_Wdp, tmp1_base, rhoi_f, cri;
+  int phi;
+};
+
+// Does the candidate cell's box (its entries' stage-start positions, in its
+// own frame) come within the search radius of vb (the target's stage-start
+// position in that frame)?
+template <int D>
+__device__ __forceinline__ bool box_reach(const float4 blo, const float4 bhi, const float* vb, float r2_box) {
+  const float lo[3] = {blo.x, blo.y, blo.z}, hb_[3] = {bhi.x, bhi.y, bhi.z};
+  float bd = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const float d = fmaxf(fmaxf(lo[a] - vb[a], vb[a] - hb_[a]), 0.f);
+    bd = fmaf(d, d, bd);
+  }
+  return bd <= r2_box;
+}
+
+// Pair terms of one target with the candidate entries x in [x0, x1) of one
+// cell: rec1(x) / rec2(x) return the entry's records, glob(x) its bin-entry
+// index (for the density and sound speed of interface pairs and for wall
+// normals).  vb, vu: the target's stage-start and current positions in the
+// candidate cell's frame.
+template <int D, typename R1, typename R2, typename G>
+__device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const float* vu, const PstConst& k,
+                                          R1 rec1, R2 rec2, G glob, const MixedBins& MB, const BinData& B,
+                                          const WallDev& W, double* pst, double* sharp) {
+  for (int x = x0; x < x1; x++) {
+    const float4 c1 = rec1(x);
+    const float cb[3] = {c1.x, c1.y, c1.z};
+    float sqb = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      const float d = vb[a] - cb[a];
+      sqb = fmaf(d, d, sqb);
+    }
+    if (!(sqb <= k.r2)) continue;
+    const float4 c2 = rec2(x);
+    const float cu[3] = {c2.x, c2.y, c2.z};
+    float dr[3] = {0, 0, 0};
+    float sq = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      dr[a] = vu[a] - cu[a];
+      sq = fmaf(dr[a], dr[a], sq);
+    }
+    if (sq == 0.f) continue;
+    // Wendland C2: W = C q^4 (1 + 2 s), dW/r = -5 C q^3 / h^2 with
+    // q = max(1 - s/2, 0), s = r / h
+    const float dist = __builtin_amdgcn_sqrtf(sq);
+    const float sk = dist * k.inv_h;
+    const float q = fmaxf(fmaf(-0.5f, sk, 1.f), 0.f);
+    const float q3 = (q * q) * q;
+    const float W_ = k.Cf * (q3 * q) * fmaf(2.f, sk, 1.f);
+    const float aw = k.dWc * q3;
+    const float rr = W_ * k.inv_Wdp;
+    const float q2 = rr * rr;
+    const float tmp1 = k.tmp1_base * fmaf(0.2f, q2 * q2, 1.f);
+    const int phj = __float_as_int(c1.w);
+    const float vol_j = c2.w;
+    if (phj != 10) {
+      const float fac = aw * tmp1 * vol_j;
+#pragma unroll
+      for (int a = 0; a < D; a++) pst[a] += (double)(dr[a] * fac);
+      if (phj != k.phi) {
+        const float2 c3 = MB.r3[glob(x)];
+        const float kk = -0.1f * fmaf(c3.y * c3.y, c3.x, k.cri) * __builtin_amdgcn_rcpf(k.rhoi_f + c3.x);
+        const float fs = kk * aw * vol_j;
+#pragma unroll
+        for (int a = 0; a < D; a++) sharp[a] += (double)(dr[a] * fs);
+      }
+    } else {
+      const int w = ref_wall(B.ref[glob(x)]);
+      const float fw = W_ * tmp1 * vol_j;
+#pragma unroll
+      for (int a = 0; a < D; a++) pst[a] += (double)((float)W.norm[a * W.n + w] * fw);
+    }
+  }
+}
+
+// One lane per bin entry (target); workgroups take runs of consecutive
+// entries, i.e. of consecutive cells.  3-D: the workgroup's targets lie on
+// one or two (i, j) cell lines; for each candidate line offset the
+// candidate entries of the cells within reach of those lines (a contiguous
+// entry range per line) are staged in LDS, so that every lane reads its
+// candidates from LDS rather than through the vector memory path.  Each
+// lane still visits its own candidate cells (i, j, k ascending) and entries
+// in bin order, so the staged sum equals the per-lane walk (the path taken
+// in 2-D, for workgroups spanning more lines, candidate ranges wider than
+// the LDS tables, or with staging switched off) bit for bit.
 template <int D, int BS>
 __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                    MixedBins MB, const int* __restrict__ bin_off,
-                                                   const DevScalars* __restrict__ sc) {
-  const int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= sc->nent_dev) return;
-  const int i = B.ref[e];
-  if (i < 0) return;  // wall or removed entry
+                                                   const DevScalars* __restrict__ sc, int staging) {
+  const int64_t E0 = (int64_t)xcd_block() * BS;
+  const int64_t nent = sc->nent_dev;
+  if (E0 >= nent) return;  // the whole workgroup is past the entries
+  const int64_t e = E0 + threadIdx.x;
+  const int i = e < nent ? B.ref[e] : -1;
+  const bool active = i >= 0;  // a particle (not a wall, removed or past-the-end entry)
   const int64_t cap = P.cap;
   const double dt = sc->dt;
   const double max_v = sc->max_v;
-  const int ct = MB.ecell[e];
-  int cti, ctj, ctk;
-  cell_ijk(g, ct, cti, ctj, ctk);
-  const float4 t1 = MB.r1[e], t2 = MB.r2[e];
-  const float2 t3 = MB.r3[e];
-  const float ubt[3] = {t1.x, t1.y, t1.z}, ut[3] = {t2.x, t2.y, t2.z};
-  const int phi = __float_as_int(t1.w);
-  double xbi[3], xi[3];
+  int ct = 0, cti = 0, ctj = 0, ctk = 0, pi = 0, pj = 0, pk = 0;
+  float ubt[3] = {0, 0, 0}, ut[3] = {0, 0, 0};
+  PstConst k{};
+  if (active) {
+    ct = MB.ecell[e];
+    cell_ijk(g, ct, cti, ctj, ctk);
+    const float4 t1 = MB.r1[e], t2 = MB.r2[e];
+    const float2 t3 = MB.r3[e];
+    ubt[0] = t1.x, ubt[1] = t1.y, ubt[2] = t1.z;
+    ut[0] = t2.x, ut[1] = t2.y, ut[2] = t2.z;
+    k.phi = __float_as_int(t1.w);
+    double xbi[3];
 #pragma unroll
-  for (int a = 0; a < D; a++) {
-    xbi[a] = S.xb[a * cap + i];
-    xi[a] = P.x[a * cap + i];
+    for (int a = 0; a < D; a++) xbi[a] = S.xb[a * cap + i];
+    const double hi = P.h[i], hbi = S.hb[i];
+    const double rhoi = P.rho[i];
+    const double ca = max_v * 5.;
+    const double P_b = rhoi * ca * ca;
+    const double dxk = hi / kCellRatio * kCutOff;
+    const double Cw = kern_C<D>(hi);
+    k.Cf = (float)Cw;
+    k.inv_h = (float)(1.0 / hi);
+    k.dWc = (float)(-5.0 * Cw / (hi * hi));
+    k.inv_Wdp = (float)(1.0 / kern_W<D>(dxk, hi));
+    const float rlim = (float)(hbi * kCutOff);
+    k.r2 = rlim * rlim;
+    k.r2_box = k.r2 * 1.0001f;
+    k.tmp1_base = (float)(-1. * P_b * (1. / rhoi));
+    k.rhoi_f = t3.x;
+    k.cri = t3.y * t3.y * t3.x;
+    const int pos = cell_of<D>(g, xbi);
+    cell_ijk(g, pos, pi, pj, pk);
   }
-  const double hi = P.h[i], hbi = S.hb[i];
-  const double rhoi = P.rho[i], ci = P.c[i];
-  const double ca = max_v * 5.;
-  const double P_b = rhoi * ca * ca;
-  const double dxk = hi / kCellRatio * kCutOff;
-  const double Cw = kern_C<D>(hi);
-  const float Cf = (float)Cw;
-  const float inv_h = (float)(1.0 / hi);
-  const float dWc = (float)(-5.0 * Cw / (hi * hi));
-  const float inv_Wdp = (float)(1.0 / kern_W<D>(dxk, hi));
-  const float rlim = (float)(hbi * kCutOff);
-  const float r2 = rlim * rlim;
-  const float r2_box = r2 * 1.0001f;
-  const float tmp1_base = (float)(-1. * P_b * (1. / rhoi));
-  const float rhoi_f = t3.x, ci_f = t3.y;
-  const float cri = ci_f * ci_f * rhoi_f;
   double pst[3] = {0, 0, 0}, sharp[3] = {0, 0, 0};
-  const int pos = cell_of<D>(g, xbi);
-  int pi, pj, pk;
-  cell_ijk(g, pos, pi, pj, pk);
   const int r = g.radius;
-  const int rk = D == 3 ? r : 0;
   const float psf[3] = {(float)g.ps[0], (float)g.ps[1], (float)g.ps[2]};
-  for (int t = max(pi - r, 0); t <= min(pi + r, g.N[0] - 1); t++)
-    for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
-      for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
-        const int cc = (t * g.N[1] + s) * g.N[2] + m;
-        // the target's positions in the candidate cell's frame:
-        // (target offset) - (cell offset) * cell size
-        const float off[3] = {(float)(t - cti) * psf[0], (float)(s - ctj) * psf[1],
-                              D == 3 ? (float)(m - ctk) * psf[2] : 0.f};
-        float vb[3], vu[3];
+  // the target's positions in the frame of a candidate cell (t, s, m):
+  // (target offset) - (cell offset) * cell size
+  auto frame = [&](int t, int s, int m, float* vb, float* vu) {
+    const float off[3] = {(float)(t - cti) * psf[0], (float)(s - ctj) * psf[1],
+                          D == 3 ? (float)(m - ctk) * psf[2] : 0.f};
 #pragma unroll
-        for (int a = 0; a < D; a++) {
-          vb[a] = ubt[a] - off[a];
-          vu[a] = ut[a] - off[a];
-        }
-        const float4 blo = MB.box[2 * cc], bhi = MB.box[2 * cc + 1];
-        const float lo[3] = {blo.x, blo.y, blo.z}, hb_[3] = {bhi.x, bhi.y, bhi.z};
-        float bd = 0;
+    for (int a = 0; a < D; a++) {
+      vb[a] = ubt[a] - off[a];
+      vu[a] = ut[a] - off[a];
+    }
+  };
+  auto grec1 = [&](int x) { return MB.r1[x]; };
+  auto grec2 = [&](int x) { return MB.r2[x]; };
+  auto gidx = [](int x) { return x; };
+  bool staged = false;
+  if constexpr (D == 3) {
+    constexpr int CAP = BS == 256 ? 640 : BS == 128 ? 384 : 192;  // staged entries
+    constexpr int KMAX = BS == 256 ? 96 : BS == 128 ? 64 : 40;    // staged cells per line
+    __shared__ float4 s_r1[CAP], s_r2[CAP];
+    __shared__ float4 s_box[2][KMAX][2];
+    __shared__ int s_off[2][KMAX + 1];
+    __shared__ int s_agg[2][6];  // per line: min, max of (pi - i), (pj - j), pk
+    const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+    const int64_t e_last = (E0 + BS < nent ? E0 + BS : nent) - 1;
+    const int line_lo = MB.ecell[E0] / N2, line_hi = MB.ecell[e_last] / N2;
+    staged = staging && line_hi - line_lo <= 1;
+    if (staged) {
+      if (threadIdx.x < 12) (&s_agg[0][0])[threadIdx.x] = (threadIdx.x & 1) ? INT_MIN : INT_MAX;
+      __syncthreads();
+      const int sg = active ? ct / N2 - line_lo : 0;
+      const int li = (line_lo + sg) / N1, lj = (line_lo + sg) % N1;
+      if (active) {
+        atomicMin(&s_agg[sg][0], pi - li);
+        atomicMax(&s_agg[sg][1], pi - li);
+        atomicMin(&s_agg[sg][2], pj - lj);
+        atomicMax(&s_agg[sg][3], pj - lj);
+        atomicMin(&s_agg[sg][4], pk);
+        atomicMax(&s_agg[sg][5], pk);
+      }
+      __syncthreads();
+      // workgroup-uniform plan (scalar registers)
+      int di0 = INT_MAX, di1 = INT_MIN, dj0 = INT_MAX, dj1 = INT_MIN;
+      int kA[2] = {0, 0}, kB[2] = {-1, -1};
+      bool has[2];
 #pragma unroll
-        for (int a = 0; a < D; a++) {
-          const float d = fmaxf(fmaxf(lo[a] - vb[a], vb[a] - hb_[a]), 0.f);
-          bd = fmaf(d, d, bd);
-        }
-        if (!(bd <= r2_box)) continue;  // empty cell or wholly outside
-        const int e1 = bin_off[cc + 1];
-        for (int f = bin_off[cc]; f < e1; f++) {
-          const float4 c1 = MB.r1[f];
-          const float cb[3] = {c1.x, c1.y, c1.z};
-          float sqb = 0;
+      for (int q = 0; q < 2; q++) {
+        int ag[6];
 #pragma unroll
-          for (int a = 0; a < D; a++) {
-            const float d = vb[a] - cb[a];
-            sqb = fmaf(d, d, sqb);
-          }
-          if (!(sqb <= r2)) continue;
-          const float4 c2 = MB.r2[f];
-          const float cu[3] = {c2.x, c2.y, c2.z};
-          float dr[3] = {0, 0, 0};
-          float sq = 0;
-#pragma unroll
-          for (int a = 0; a < D; a++) {
-            dr[a] = vu[a] - cu[a];
-            sq = fmaf(dr[a], dr[a], sq);
-          }
-          if (sq == 0.f) continue;
-          // Wendland C2: W = C q^4 (1 + 2 s), dW/r = -5 C q^3 / h^2 with
-          // q = max(1 - s/2, 0), s = r / h
-          const float dist = __builtin_amdgcn_sqrtf(sq);
-          const float sk = dist * inv_h;
-          const float q = fmaxf(fmaf(-0.5f, sk, 1.f), 0.f);
-          const float q3 = (q * q) * q;
-          const float W_ = Cf * (q3 * q) * fmaf(2.f, sk, 1.f);
-          const float aw = dWc * q3;
-          const float rr = W_ * inv_Wdp;
-          const float q2 = rr * rr;
-          const float tmp1 = tmp1_base * fmaf(0.2f, q2 * q2, 1.f);
-          const int phj = __float_as_int(c1.w);
-          const float vol_j = c2.w;
-          if (phj != 10) {
-            const float fac = aw * tmp1 * vol_j;
-#pragma unroll
-            for (int a = 0; a < D; a++) pst[a] += (double)(dr[a] * fac);
-            if (phj != phi) {
-              const float2 c3 = MB.r3[f];
-              const float kk = -0.1f * fmaf(c3.y * c3.y, c3.x, cri) *
-                               __builtin_amdgcn_rcpf(rhoi_f + c3.x);
-              const float fs = kk * aw * vol_j;
-#pragma unroll
-              for (int a = 0; a < D; a++) sharp[a] += (double)(dr[a] * fs);
-            }
-          } else {
-            const int w = ref_wall(B.ref[f]);
-            const float fw = W_ * tmp1 * vol_j;
-#pragma unroll
-            for (int a = 0; a < D; a++) pst[a] += (double)((float)W.norm[a * W.n + w] * fw);
-          }
+        for (int u = 0; u < 6; u++) ag[u] = __builtin_amdgcn_readfirstlane(s_agg[q][u]);
+        has[q] = ag[4] <= ag[5];
+        if (has[q]) {
+          di0 = min(di0, ag[0]);
+          di1 = max(di1, ag[1]);
+          dj0 = min(dj0, ag[2]);
+          dj1 = max(dj1, ag[3]);
+          kA[q] = max(ag[4] - r, 0);
+          kB[q] = min(ag[5] + r, N2 - 1);
+          if (kB[q] - kA[q] + 1 > KMAX) staged = false;
         }
       }
+      const int kA_l = sg ? kA[1] : kA[0];
+      if (staged && (has[0] || has[1])) {
+        for (int dI = di0 - r; dI <= di1 + r; dI++)
+          for (int dJ = dj0 - r; dJ <= dj1 + r; dJ++) {
+            int a0[2] = {0, 0}, n0[2] = {0, 0}, cb[2] = {0, 0};
+            bool v[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+              const int I = (line_lo + q) / N1 + dI, J = (line_lo + q) % N1 + dJ;
+              v[q] = has[q] && I >= 0 && I < N0 && J >= 0 && J < N1;
+              if (v[q]) {
+                cb[q] = (I * N1 + J) * N2;
+                a0[q] = bin_off[cb[q] + kA[q]];
+                n0[q] = bin_off[cb[q] + kB[q] + 1] - a0[q];
+              }
+            }
+            const int ntot = n0[0] + n0[1];
+            if (ntot == 0) continue;  // workgroup-uniform
+            // the candidate cells' entry offsets and boxes
+            for (int q = threadIdx.x; q < 2 * (KMAX + 1); q += BS) {
+              const int s_ = q / (KMAX + 1), lc = q % (KMAX + 1);
+              if (v[s_] && lc <= kB[s_] - kA[s_] + 1) {
+                const int c = cb[s_] + kA[s_] + lc;
+                s_off[s_][lc] = bin_off[c];
+                if (lc <= kB[s_] - kA[s_]) {
+                  s_box[s_][lc][0] = MB.box[2 * c];
+                  s_box[s_][lc][1] = MB.box[2 * c + 1];
+                }
+              }
+            }
+            const int I = li + dI, J = lj + dJ;
+            const bool mine = active && (sg ? v[1] : v[0]) && abs(I - pi) <= r && abs(J - pj) <= r;
+            const int base = sg == 0 ? 0 : n0[0];
+            const int sh = (sg ? a0[1] : a0[0]) - base;  // entry index - staged index
+            for (int q0 = 0; q0 < ntot; q0 += CAP) {
+              const int qn = min(ntot - q0, CAP);
+              for (int q = threadIdx.x; q < qn; q += BS) {
+                const int x = q0 + q;
+                const int f = x < n0[0] ? a0[0] + x : a0[1] + (x - n0[0]);
+                s_r1[q] = MB.r1[f];
+                s_r2[q] = MB.r2[f];
+              }
+              __syncthreads();
+              if (mine) {
+                const int m1 = min(pk + r, N2 - 1);
+                for (int m = max(pk - r, 0); m <= m1; m++) {
+                  const int lc = m - kA_l;
+                  const int x0 = max(s_off[sg][lc] - sh, q0);
+                  const int x1 = min(s_off[sg][lc + 1] - sh, q0 + qn);
+                  if (x0 >= x1) continue;
+                  float vb[3], vu[3];
+                  frame(I, J, m, vb, vu);
+                  if (!box_reach<D>(s_box[sg][lc][0], s_box[sg][lc][1], vb, k.r2_box)) continue;
+                  pst_pairs<D>(
+                      x0, x1, vb, vu, k, [&](int x) { return s_r1[x - q0]; },
+                      [&](int x) { return s_r2[x - q0]; }, [&](int x) { return x + sh; }, MB, B, W, pst,
+                      sharp);
+                }
+              }
+              __syncthreads();
+            }
+          }
+      }
+    }
+  }
+  if (!active) return;
+  if (!staged) {
+    const int rk = D == 3 ? r : 0;
+    for (int t = max(pi - r, 0); t <= min(pi + r, g.N[0] - 1); t++)
+      for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
+        for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
+          const int cc = (t * g.N[1] + s) * g.N[2] + m;
+          float vb[3], vu[3];
+          frame(t, s, m, vb, vu);
+          if (!box_reach<D>(MB.box[2 * cc], MB.box[2 * cc + 1], vb, k.r2_box)) continue;  // empty or out of reach
+          pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, gidx, MB, B, W, pst, sharp);
+        }
+  }
   // boundary penalty, clamps and the transport velocity in FP64, as in the
   // FP64 kernel
+  double xi[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) xi[a] = P.x[a * cap + i];
+  const double ci = P.c[i];
   double pen[3] = {0, 0, 0};
   {
     const int c = S.in_ele[i];
@@ -282,19 +444,19 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
 template <int D>
 int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                             const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                            const DevScalars* sc, hipStream_t st, int block) {
+                            const DevScalars* sc, hipStream_t st, int block, int staging) {
   switch (block) {
     case 64:
       hipLaunchKernelGGL((k_pst_mixed<D, 64>), dim3(grid_xcd(n_ent, 64)), dim3(64), 0, st, g, P, W, S, B,
-                         MB, bin_off, sc);
+                         MB, bin_off, sc, staging);
       break;
     case 128:
       hipLaunchKernelGGL((k_pst_mixed<D, 128>), dim3(grid_xcd(n_ent, 128)), dim3(128), 0, st, g, P, W, S,
-                         B, MB, bin_off, sc);
+                         B, MB, bin_off, sc, staging);
       break;
     case 256:
       hipLaunchKernelGGL((k_pst_mixed<D, 256>), dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S,
-                         B, MB, bin_off, sc);
+                         B, MB, bin_off, sc, staging);
       break;
     default:
       return PGDG_ERR_ARG;
@@ -306,23 +468,24 @@ int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, con
 template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block) {
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging) {
   if (n_ent <= 0) return PGDG_OK;
   if (n_ent > MB.ne_cap) return PGDG_ERR_CAPACITY;
   hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
                      P, W, S, B, MB, bin_off, bin_ent, sc);
   hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
-  PGDG_TRY(launch_pst_mixed_kernel<D>(g, P, W, S, B, MB, n_ent, bin_off, sc, st, block));
+  PGDG_TRY(launch_pst_mixed_kernel<D>(g, P, W, S, B, MB, n_ent, bin_off, sc, st, block, staging));
   return PGDG_OK;
 }
 
 #define PGDG_INST_MIXED(D)                                                                         \
   template int launch_pst_mixed<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,     \
                                    const BinData&, const MixedBins&, int64_t, const int*,          \
-                                   const int*, const DevScalars*, hipStream_t, int);               \
+                                   const int*, const DevScalars*, hipStream_t, int, int);          \
   template int launch_pst_mixed_kernel<D>(const Geo&, const PtlDev&, const WallDev&,                \
                                           const StageBufs&, const BinData&, const MixedBins&,       \
-                                          int64_t, const int*, const DevScalars*, hipStream_t, int);
+                                          int64_t, const int*, const DevScalars*, hipStream_t, int, \
+                                          int);
 PGDG_INST_MIXED(2)
 PGDG_INST_MIXED(3)
 #undef PGDG_INST_MIXED

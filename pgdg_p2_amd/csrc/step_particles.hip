@@ -50,10 +50,12 @@ __global__ void k_in_ele(Geo g, PtlDev P, StageBufs S, int64_t n) {
 // md: 0 first stage, 1 Heun second stage, 2 and 3 the SSP-RK3 stages (the
 // cell update's stage codes).  Order 2: the particle's copy carries the
 // gradient of the cell polynomial at the particle and its second-order
-// coefficients.
+// coefficients.  copy == false skips the coefficient copy: within a time
+// step only P2G (first stage) and split/merge (between steps) read it, so
+// the copies written by stages before the last are never read.
 template <int D, int ORD>
 __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S,
-                                             const DevScalars* __restrict__ sc, int md) {
+                                             const DevScalars* __restrict__ sc, int md, bool copy) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= sc->np_dev) return;
@@ -118,7 +120,8 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
       for (int a = 0; a < D; a++) cw[1 + a] = gr[a];
     }
 #pragma unroll
-    for (int m = 0; m < M; m++) P.coef[(size_t)(t * M + m) * cap + i] = cw[m];
+    for (int m = 0; m < M; m++)
+      if (copy) P.coef[(size_t)(t * M + m) * cap + i] = cw[m];
   }
   double rho = Ui[0], E = Ui[U - 1];
   double rv[3] = {0, 0, 0};
@@ -134,10 +137,11 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
   double Pp = eos_p(ei, Eos{g.gamma[ss], g.pinf[ss]});
   double cs = sqrt(g.gamma[ss] * (g.pinf[ss] + Pp) / rho);
   if (Pp < -0.05 * g.pinf[ss] || rho < 0) {
+    if (copy)
 #pragma unroll
-    for (int t = 0; t < U; t++)
+      for (int t = 0; t < U; t++)
 #pragma unroll
-      for (int b = 1; b < M; b++) P.coef[(size_t)(t * M + b) * cap + i] = 0;
+        for (int b = 1; b < M; b++) P.coef[(size_t)(t * M + b) * cap + i] = 0;
     rho = C.rho[ss * nc + c];
 #pragma unroll
     for (int a = 0; a < 3; a++) rv[a] = C.rhov[(ss * 3 + a) * nc + c];
@@ -543,14 +547,14 @@ int launch_in_ele(const Geo& g, const PtlDev& P, const StageBufs& S, int64_t n, 
 
 template <int D>
 int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
-               const DevScalars* sc, int md, hipStream_t st) {
+               const DevScalars* sc, int md, bool copy, hipStream_t st) {
   if (n <= 0) return PGDG_OK;
   if (g.ord == 2)
     hipLaunchKernelGGL((k_g2p<D, 2>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
-                       md);
+                       md, copy);
   else
-  hipLaunchKernelGGL((k_g2p<D, 1>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
-                     md);
+    hipLaunchKernelGGL((k_g2p<D, 1>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
+                       md, copy);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
@@ -577,7 +581,7 @@ int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32
                                     const DevScalars*, hipStream_t);                              \
   template int launch_in_ele<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, hipStream_t); \
   template int launch_g2p<D>(const Geo&, const PtlDev&, const CellDev&, const StageBufs&, int64_t, \
-                             const DevScalars*, int, hipStream_t);                                \
+                             const DevScalars*, int, bool, hipStream_t);                          \
   template int launch_pst_binned<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \
                                     const BinData&, int64_t, const int*, const int*,              \
                                     const DevScalars*, hipStream_t, int);                         \

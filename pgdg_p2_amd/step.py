@@ -56,6 +56,11 @@ class GpuSim:
         """Run the RK stages as captured HIP graphs (default) or launch by launch."""
         check(self.L.pgdg_graphs_enable(self.h, 1 if on else 0), "pgdg_graphs_enable")
 
+    def pst_staging(self, on: bool):
+        """Mixed form, 3-D: particle-shifting candidates staged in LDS (default)
+        or read by every lane from global memory (same sums bit for bit)."""
+        check(self.L.pgdg_pst_staging_enable(self.h, 1 if on else 0), "pgdg_pst_staging_enable")
+
     def autotune(self, on: bool):
         check(self.L.pgdg_autotune_enable(self.h, 1 if on else 0), "pgdg_autotune_enable")
 

@@ -69,3 +69,31 @@ def test_mixed_step_triple_point_2d(cuda):
 def test_mixed_step_p2_rk3_shock_bubble_3d(cuda):
     sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
     _compare(lambda: cases.shock_bubble_3d(nx=32, ny=16, nz=16, scheme=sch), 12, "sb3d-p2-mixed")
+
+
+@pytest.mark.gpu
+def test_mixed_pst_staged_equals_per_lane_walk(cuda):
+    # 64x32x32 P2 cells, 8 particles per cell: a workgroup's targets lie on
+    # one or two cell lines (candidates staged in LDS) or, where cells hold
+    # fewer particles, on more lines (every lane walks its candidates in
+    # global memory); both must give the per-lane sums bit for bit
+    from pgdg_p2_amd.compare import compare_states
+    from pgdg_p2_amd.step import GpuSim
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    case, st = cases.shock_bubble_3d(nx=64, ny=32, nz=32, scheme=sch)
+    case.precision = 1
+    ga, gb = GpuSim(case, st), GpuSim(case, st)
+    gb.pst_staging(False)
+    a, b = st.copy(), st.copy()
+    try:
+        for k in range(6):
+            print(f"staged-vs-walk: step {k}", flush=True)
+            ga.step(1)
+            gb.step(1)
+            ga.save(a)
+            gb.save(b)
+            bad = compare_states(a, b)
+            assert not bad, f"step {k}: {bad[:6]}"
+    finally:
+        ga.close()
+        gb.close()

@@ -1,0 +1,190 @@
+// cell_flux.hpp - point-wise pieces of the DG face and volume integrals
+// (Particle_base::Grid_acceleration, S3:particle_flow.cpp:16-405): the
+// stiffened-gas EOS in arithmetic T, the axis flux column, the HLLC flux of
+// an interior face point with the MGFM ghost state at material interfaces,
+// and the face quadrature points in a fixed (canonical) order.
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace pgdg {
+
+// Stiffened-gas EOS in arithmetic T (the same operations as eos_p and
+// eos_rhoe for T = double).
+template <typename T>
+struct EosT {
+  T g, pinf, gm1, gpinf;
+  __device__ explicit EosT(const Eos& m)
+      : g((T)m.gamma), pinf((T)m.pinf), gm1((T)(m.gamma - 1)), gpinf((T)(m.gamma * m.pinf)) {}
+  __device__ T p(T rhoe) const { return rhoe * gm1 - gpinf; }
+  __device__ T rhoe(T p_) const { return (p_ + gpinf) / gm1; }
+};
+
+// Flux column b of DF = U v^T + P I (+ v_b P in the energy row).  b is a
+// runtime axis: it selects through comparisons rather than array indexing,
+// so the operands stay in registers (a runtime index into a local array
+// would place it in scratch memory).
+template <int D, typename T>
+__device__ inline T pick_axis(const T* v, int b) {
+  return b == 0 ? v[0] : (b == 1 || D == 2 ? v[1] : v[2]);
+}
+
+template <int D, typename T>
+__device__ inline void flux_col(T rho, const T* rv, T E, const T* v, T P, int b, T* out) {
+  constexpr int U = D + 2;
+  const T vb = pick_axis<D>(v, b);
+  out[0] = rho * vb;
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    const T f = rv[a] * vb;
+    out[1 + a] = a == b ? f + P : f;
+  }
+  out[U - 1] = E * vb + vb * P;
+}
+
+// Face quadrature point q of a face normal to axis `ax`, in the canonical
+// order shared by the face and cell kernels: the tangential axes t0 < t1,
+// q = g1 * NQ + g2 (2-D: q = g1).  off: offset from the face centre; gw: the
+// product of the 1-D weights.
+template <int D, int ORD>
+__device__ inline void face_point(const Geo& g, int ax, int q, const double* hp, double* off, double& gw) {
+  constexpr int NQ = Basis<D, ORD>::NQ;
+  const int t0 = ax == 0 ? 1 : 0;
+  const int t1 = ax == 2 ? 1 : 2;
+  off[0] = off[1] = off[2] = 0;
+  if (D == 3) {
+    const int g1 = q / NQ, g2 = q % NQ;
+    off[t0] = g.Gp[g1] * hp[t0];
+    off[t1] = g.Gp[g2] * hp[t1];
+    gw = g.Gw[g1] * g.Gw[g2];
+  } else {
+    off[t0] = g.Gp[q] * hp[t0];
+    gw = g.Gw[q];
+  }
+}
+
+// Slot of an interior-face point flux (mixed form).  Face (ax, c) lies
+// between cell c and its +ax neighbour; side 0 holds the flux seen from c
+// (its +ax face), side 1 the flux seen from the neighbour (its -ax face);
+// ss is the phase of that side's cell, q the canonical point, u the
+// variable.  Consecutive c are contiguous.
+template <int D, int ORD>
+__device__ inline size_t face_slot(const Geo& g, int side, int ss, int q, int u, int ax, int c) {
+  constexpr int U = D + 2, NQ = Basis<D, ORD>::NQ, NQF = D == 3 ? NQ * NQ : NQ;
+  return ((((size_t)side * g.F + ss) * NQF + q) * U + u) * ((size_t)D * g.nc) + (size_t)ax * g.nc + c;
+}
+
+// HLLC flux at one interior face point (the per-point body of the
+// reference's face integral): cell i's phase ss (state Ui) against the
+// neighbour's phase s1 (state Uj), normal sgn * e_ax.  For ss != s1 the
+// neighbour state is replaced by the MGFM ghost state in phase ss's EOS
+// (Newton solve in FP64); kSamePhase: the caller guarantees ss == s1 (no
+// ghost state code).  Fh: the flux the reference multiplies by the basis,
+// the weights and alpha_j.
+template <int D, typename T, bool kSamePhase = false>
+__device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss, int s1, int ax, T sgn,
+                                  T* Fh) {
+  constexpr int U = D + 2;
+  const bool s3 = g.sem == PGDG_SEM_S3;
+  const Eos me{g.gamma[ss], g.pinf[ss]};
+  const EosT<T> mt(me);
+  T nrm[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+    if (a == ax) nrm[a] = sgn;
+  const T rho_i = Ui[0], E_i = Ui[U - 1];
+  T rvi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) rvi[a] = Ui[1 + a];
+#pragma unroll
+  for (int a = 0; a < D; a++) vi[a] = rvi[a] / rho_i;
+  T sq = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) sq += rvi[a] * rvi[a];
+  const T e_i = E_i - T(0.5) * sq / rho_i;
+  const T P_i = mt.p(e_i);
+  const T ci = sqrt(mt.g * (mt.pinf + P_i) / rho_i);
+  T vni = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
+  T DFi[U];
+  flux_col<D>(rho_i, rvi, E_i, vi, P_i, ax, DFi);
+  T rho_j = Uj[0], E_j = Uj[U - 1];
+  T rvj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) rvj[a] = Uj[1 + a];
+#pragma unroll
+  for (int a = 0; a < D; a++) vj[a] = rvj[a] / rho_j;
+  T sqj = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) sqj += vj[a] * vj[a];
+  T rhoe_j = E_j - T(0.5) * sqj * rho_j;
+  T P_j = EosT<T>(Eos{g.gamma[s1], g.pinf[s1]}).p(rhoe_j);
+  T vnj = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
+  if (!kSamePhase && ss != s1) {
+    MgfmOut o = mgfm_solve(vni, vnj, P_i, P_j, rho_i, rho_j, me, Eos{g.gamma[s1], g.pinf[s1]},
+                           s3 ? MGFM_S3 : MGFM_T2, TOL_REF, 0.0, 50, -1.0);
+    rho_j = o.rho_l;
+    P_j = o.pstar;
+    vnj = o.u_l;
+#pragma unroll
+    for (int a = 0; a < D; a++) vj[a] = vi[a] - vni * nrm[a] + vnj * nrm[a];
+#pragma unroll
+    for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+    rhoe_j = mt.rhoe(P_j);
+    T sq2 = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) sq2 += vj[a] * vj[a];
+    E_j = rhoe_j + T(0.5) * rho_j * sq2;
+  }
+  const T cj = sqrt(mt.g * (mt.pinf + P_j) / rho_j);
+  T Ujs[U];
+  Ujs[0] = rho_j;
+#pragma unroll
+  for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
+  Ujs[U - 1] = E_j;
+  T DFj[U];
+  flux_col<D>(rho_j, rvj, E_j, vj, P_j, ax, DFj);
+  vni = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
+  vnj = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
+  const T SL = amin1(vni - ci, vnj - cj);
+  const T SR = amax1(vni + ci, vnj + cj);
+  const T S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
+                (rho_i * (SL - vni) - rho_j * (SR - vnj));
+  if (SL > 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = DFi[u];
+  } else if (SL <= 0 && S_s >= 0) {
+    const T rho_sL = rho_i * (SL - vni) / (SL - S_s);
+    T UsL[U];
+    UsL[0] = rho_sL;
+#pragma unroll
+    for (int a = 0; a < D; a++) UsL[1 + a] = rho_sL * (vi[a] + (S_s - vni) * nrm[a]);
+    UsL[U - 1] = rho_sL * (E_i / rho_i + (S_s - vni) * (S_s + (P_i) / (rho_i * (SL - vni))));
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = DFi[u] + SL * (UsL[u] - Ui[u]) * sgn;
+  } else if (S_s <= 0 && SR >= 0) {
+    const T rho_sR = rho_j * (SR - vnj) / (SR - S_s);
+    T UsR[U];
+    UsR[0] = rho_sR;
+#pragma unroll
+    for (int a = 0; a < D; a++) UsR[1 + a] = rho_sR * (vj[a] + (S_s - vnj) * nrm[a]);
+    UsR[U - 1] = rho_sR * (E_j / rho_j + (S_s - vnj) * (S_s + (P_j) / (rho_j * (SR - vnj))));
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = DFj[u] + SR * (UsR[u] - Ujs[u]) * sgn;
+  } else if (SR <= 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = DFj[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = 0;
+  }
+}
+
+}  // namespace pgdg

@@ -120,6 +120,7 @@ struct Ctx {
   P2gBuf PB{};
   MixedBins MB{};
   void* mb_slab = nullptr;
+  void* ff_slab = nullptr;
   int32_t* bi_slab = nullptr;
   Topo T;
   DevScalars* sc = nullptr;
@@ -298,6 +299,18 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   C.ph_state = ci_slab;
   C.ph_state_b = ci_slab + fn;
   PGDG_HIP_TRY(hipMalloc((void**)&used, sizeof(int32_t) * nc));
+  size_t ff_bytes = 0;
+  if (cs.precision == PGDG_PREC_MIXED) {
+    // interior-face point fluxes [2][F][NQF][U][D][nc] (FP32), the general
+    // face list [D][nc] and its count
+    const int nq = cs.order + 1, nqf = D == 3 ? nq * nq : nq;
+    const size_t nflux = (size_t)2 * F * nqf * U * D * nc;
+    ff_bytes = sizeof(float) * nflux + sizeof(int32_t) * ((size_t)D * nc + 64);
+    PGDG_HIP_TRY(hipMalloc(&ff_slab, ff_bytes));
+    C.fflux = (float*)ff_slab;
+    C.flist = (int32_t*)(C.fflux + nflux);
+    C.fcount = C.flist + (size_t)D * nc;
+  }
   // stage buffers: xb[3], hb, x0[3], vol0
   PGDG_HIP_TRY(hipMalloc((void**)&s_slab, sizeof(double) * 8 * cap));
   PGDG_HIP_TRY(hipMemset(s_slab, 0, sizeof(double) * 8 * cap));
@@ -342,7 +355,7 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall +
                             9 * (size_t)ne + 6 * (size_t)nc + p2g_rows_n * (size_t)ne) +
-          T.bytes_total + mb_bytes;
+          T.bytes_total + mb_bytes + ff_bytes;
   return PGDG_OK;
 }
 
@@ -351,7 +364,8 @@ void Ctx::release() {
   drop_graphs();
   PA.release();
   T.release();
-  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light, mb_slab};
+  void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light, mb_slab,
+                  ff_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (own_stream && st) (void)hipStreamDestroy(st);
@@ -767,6 +781,12 @@ int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_c
   if (c->ppc_max > 0 && c->ppc_min > c->ppc_max) return PGDG_ERR_ARG;
   if (c->semantics != PGDG_SEM_T2 && c->semantics != PGDG_SEM_S3) return PGDG_ERR_ARG;
   if (c->precision != PGDG_PREC_FP64 && c->precision != PGDG_PREC_MIXED) return PGDG_ERR_ARG;
+  // every domain side needs a boundary treatment (interior is no tag for a
+  // side of the domain); sides 4 and 5 (z) only in 3-D
+  for (int f = 0; f < 2 * c->dim; f++)
+    if (c->bc[f] != PGDG_BC_WALL && c->bc[f] != PGDG_BC_INLET && c->bc[f] != PGDG_BC_OUTLET &&
+        c->bc[f] != PGDG_BC_SYMMETRY)
+      return PGDG_ERR_ARG;
   if (particle_cap > (int64_t)0x7ffffff0) return PGDG_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PGDG_ERR_NODEVICE;

@@ -62,22 +62,22 @@ struct Basis {
 // Order 2 follows the zero-mean P2 form of UX:glbfunc.h:210-222 (2D: x, y,
 // xy, x^2 - dx^2/12, y^2 - dy^2/12); 3D: x, y, z, xy, xz, yz, x^2 - dx^2/12,
 // y^2 - dy^2/12, z^2 - dz^2/12.
-template <int D, int ORD>
-__device__ inline void basis_tail(const double* r, const double* q12, double* phi) {
+template <int D, int ORD, typename T>
+__device__ inline void basis_tail(const T* r, const double* q12, T* phi) {
 #pragma unroll
   for (int a = 0; a < D; a++) phi[a] = r[a];
   if constexpr (ORD == 2) {
     if constexpr (D == 2) {
       phi[2] = r[0] * r[1];
-      phi[3] = r[0] * r[0] - q12[0];
-      phi[4] = r[1] * r[1] - q12[1];
+      phi[3] = r[0] * r[0] - (T)q12[0];
+      phi[4] = r[1] * r[1] - (T)q12[1];
     } else {
       phi[3] = r[0] * r[1];
       phi[4] = r[0] * r[2];
       phi[5] = r[1] * r[2];
-      phi[6] = r[0] * r[0] - q12[0];
-      phi[7] = r[1] * r[1] - q12[1];
-      phi[8] = r[2] * r[2] - q12[2];
+      phi[6] = r[0] * r[0] - (T)q12[0];
+      phi[7] = r[1] * r[1] - (T)q12[1];
+      phi[8] = r[2] * r[2] - (T)q12[2];
     }
   }
 }
@@ -117,6 +117,10 @@ struct PtlDev {
 struct CellDev {
   double *coef, *coef0, *dcoef, *lim, *alph, *rho, *E, *c, *ac, *rhov, *v, *a, *P, *G;
   int32_t *ph_state, *ph_state_b;
+  // mixed form only: interior-face point fluxes, the faces left to the
+  // general face kernel and their count
+  float* fflux;
+  int32_t *flist, *fcount;
 };
 
 struct WallDev {

@@ -12,6 +12,7 @@
 // Face fluxes are accumulated only along the face normal: the reference
 // forms the full U x D flux and contracts it with an axis-aligned normal, so
 // the tangential columns contribute exact zeros.
+#include "cell_flux.hpp"
 #include "dev_common.hpp"
 #include "hip_util.hpp"
 #include "step_kernels.hpp"
@@ -554,49 +555,23 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
 // ---------------------------------------------------------------------------
 // DG right-hand side (S3:particle_flow.cpp:16-405)
 // ---------------------------------------------------------------------------
-// Flux column b of DF = U v^T + P I (+ v_b P in the energy row).  b is a
-// runtime axis: it selects through comparisons rather than array indexing,
-// so the operands stay in registers (a runtime index into a local array
-// would place it in scratch memory).
-// Stiffened-gas EOS in arithmetic T (the same operations as eos_p and
-// eos_rhoe for T = double).
-template <typename T>
-struct EosT {
-  T g, pinf, gm1, gpinf;
-  __device__ explicit EosT(const Eos& m)
-      : g((T)m.gamma), pinf((T)m.pinf), gm1((T)(m.gamma - 1)), gpinf((T)(m.gamma * m.pinf)) {}
-  __device__ T p(T rhoe) const { return rhoe * gm1 - gpinf; }
-  __device__ T rhoe(T p_) const { return (p_ + gpinf) / gm1; }
-};
-
-template <int D, typename T>
-__device__ inline T pick_axis(const T* v, int b) {
-  return b == 0 ? v[0] : (b == 1 || D == 2 ? v[1] : v[2]);
-}
-
-template <int D, typename T>
-__device__ inline void flux_col(T rho, const T* rv, T E, const T* v, T P, int b, T* out) {
-  constexpr int U = D + 2;
-  const T vb = pick_axis<D>(v, b);
-  out[0] = rho * vb;
-#pragma unroll
-  for (int a = 0; a < D; a++) {
-    const T f = rv[a] * vb;
-    out[1 + a] = a == b ? f + P : f;
-  }
-  out[U - 1] = E * vb + vb * P;
-}
-
 // T: the arithmetic of the quadrature (double: the reference's FP64 order,
 // bit-identical to the oracle; float: the mixed form, FP32 evaluation, fluxes
 // and sums with the MGFM Newton solve in FP64 and the mass division and
-// stored rates in FP64).
+// stored rates in FP64).  The float form takes the interior-face point
+// fluxes from the face kernels (each face's HLLC evaluated once for both of
+// its cells) and evaluates only the volume integral and the domain-boundary
+// faces itself.
 template <int D, int ORD, int BS, typename T>
-__global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NF = 2 * D, NQ = Basis<D, ORD>::NQ;
-  // per-lane LDS slot for the neighbour's coefficients and limiter of its
-  // first present phase, loaded once per face instead of once per face point
-  __shared__ T s_nb[U * M + U][BS];
+  constexpr bool kFaceBuf = sizeof(T) == 4;
+  constexpr int NQF = D == 3 ? NQ * NQ : NQ;
+  // FP64 form: per-lane LDS slot for the neighbour's coefficients and
+  // limiter of its first present phase, loaded once per face instead of once
+  // per face point
+  __shared__ T s_nb[kFaceBuf ? 1 : U * M + U][BS];
   const int ln = threadIdx.x;
   const int nc = g.nc;
   int c = xcd_block() * blockDim.x + threadIdx.x;
@@ -628,9 +603,7 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
 #pragma unroll
       for (int m = 0; m < M; m++) dc[u][m] = 0;
     // volume integral, Gauss points in g1, g2[, g3] order
-    for (int g1 = 0; g1 < NQ; g1++)
-      for (int g2 = 0; g2 < NQ; g2++)
-        for (int g3 = 0; g3 < (D == 3 ? NQ : 1); g3++) {
+    auto vol_point = [&](int g1, int g2, int g3) {
           double dp[3] = {0, 0, 0};
           dp[0] = g.Gp[g1] * hp[0];
           dp[1] = g.Gp[g2] * hp[1];
@@ -640,11 +613,8 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
             gw = g.Gw[g1] * g.Gw[g2] * g.Gw[g3];
           }
           const T dpT[3] = {(T)dp[0], (T)dp[1], (T)dp[2]};
-          double ph64[B];
-          basis_tail<D, ORD>(dp, g.q12, ph64);
           T phv[B];
-#pragma unroll
-          for (int b = 0; b < B; b++) phv[b] = (T)ph64[b];
+          basis_tail<D, ORD>(dpT, g.q12, phv);
           T Up[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {
@@ -688,7 +658,18 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
               }
             }
           }
-        }
+    };
+    if constexpr (kFaceBuf) {
+      // float form: one rolled loop over the points (unrolled, the point
+      // bodies hold more values live than the register file has)
+      constexpr int NZ = D == 3 ? NQ : 1;
+#pragma unroll 1
+      for (int p = 0; p < NQ * NQ * NZ; p++) vol_point(p / (NQ * NZ), (p / NZ) % NQ, p % NZ);
+    } else {
+      for (int g1 = 0; g1 < NQ; g1++)
+        for (int g2 = 0; g2 < NQ; g2++)
+          for (int g3 = 0; g3 < (D == 3 ? NQ : 1); g3++) vol_point(g1, g2, g3);
+    }
     // face integral
     for (int f = 0; f < NF; f++) {
       FaceGeo<D> fg;
@@ -708,9 +689,34 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
       for (int m = 0; m < (kDirect ? 1 : M); m++)
 #pragma unroll
         for (int u = 0; u < U; u++) DFn[m][u] = 0;
+      if constexpr (kFaceBuf) {
+        if (fg.b == 0) {  // interior face: the face kernels' point fluxes
+          const int side = fg.sgn > 0 ? 0 : 1;
+          const int fc = fg.sgn > 0 ? c : fg.nbr;
+#pragma unroll 1
+          for (int q = 0; q < NQF; q++) {
+            double off[3], gwd;
+            face_point<D, ORD>(g, ax, q, hp, off, gwd);
+            T dri[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) dri[a] = (T)(off[a] + fg.cen[a] - xc[a]);
+            T Nb[M];
+            Nb[0] = T(1.);
+            basis_tail<D, ORD>(dri, g.q12, Nb + 1);
+            const T wq = (T)gwd * fs;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+              const T Fh = C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, fc)];
+#pragma unroll
+              for (int bi = 0; bi < M; bi++) dc[u][bi] += Fh * Nb[bi] * wq;
+            }
+          }
+          continue;
+        }
+      }
       const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
       int s_first = -1;
-      if (fg.b == 0) {
+      if (!kFaceBuf && fg.b == 0) {
         for (int s1 = 0; s1 < PGDG_MAX_PHASE && s_first < 0; s1++)
           if (s1 < g.F && C.alph[s1 * nc + fg.nbr] != 0) s_first = s1;
         if (s_first >= 0) {
@@ -767,7 +773,7 @@ __global__ void __launch_bounds__(BS) k_rhs(Geo g, CellDev C, const int32_t* __r
           for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
           T DFi[U];
           flux_col<D>(rho_i, rvi, E_i, vi, P_i, ax, DFi);
-          if (fg.b == 0) {
+          if (!kFaceBuf && fg.b == 0) {
             const int nb = fg.nbr;
             int ni, nj, nk;
             cell_ijk(g, nb, ni, nj, nk);
@@ -1131,6 +1137,7 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
       break;
     case CELL_RHS:
       if (g.prec == PGDG_PREC_MIXED) {
+        PGDG_TRY(launch_faces<D>(g, C, used, st));
         switch (block) {
           case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, float>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
           case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, float>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;

@@ -1,0 +1,210 @@
+// step_faces.hip - interior-face fluxes of the mixed-precision DG
+// right-hand side (the face integral of Particle_base::Grid_acceleration,
+// S3:particle_flow.cpp:16-405, evaluated once per face).
+//
+// The reference (and the FP64 form) evaluates every interior face twice,
+// once from each of its cells.  Where both cells hold the same single phase
+// the two HLLC evaluations are the same flux (HLLC is conservative: with
+// the roles of the two states and the normal exchanged, SL, SR and S* swap
+// and change sign and the same branch formula results), so the mixed form
+// evaluates it once per face point and stores it for both cells, each
+// weighted by the other cell's volume fraction.  Faces with several phases
+// or a material interface (MGFM ghost states, one-sided) are listed and
+// evaluated per side by a second kernel.  The cell kernel (k_rhs, float)
+// projects the stored point fluxes onto its modes.
+#include "cell_flux.hpp"
+#include "dev_common.hpp"
+#include "hip_util.hpp"
+#include "step_kernels.hpp"
+
+namespace pgdg {
+
+__global__ void k_face_clear(int32_t* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0;
+}
+
+// Phase masks of a cell: present (ph_state 1) and nonzero volume fraction.
+__device__ inline void phase_masks(const Geo& g, const CellDev& C, int c, unsigned& pm, unsigned& am) {
+  pm = am = 0;
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    if (s >= g.F) break;
+    pm |= (C.ph_state[s * g.nc + c] == 1 ? 1u : 0u) << s;
+    am |= (C.alph[s * g.nc + c] != 0 ? 1u : 0u) << s;
+  }
+}
+
+// Cell c's phase s at a face point (offset dr from its centre), FP32.
+template <int D, int ORD>
+__device__ inline void point_state(const Geo& g, const CellDev& C, int c, int s, const double* dr, float* Uo) {
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1;
+  const int nc = g.nc;
+  double ph64[B];
+  basis_tail<D, ORD>(dr, g.q12, ph64);
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    float acc = 0;
+#pragma unroll
+    for (int b = 0; b < B; b++)
+      acc += (float)C.coef[(((size_t)s * U + u) * M + 1 + b) * nc + c] * (float)ph64[b];
+    Uo[u] = (float)C.coef[(((size_t)s * U + u) * M) * nc + c] + (float)C.lim[(s * U + u) * nc + c] * acc;
+  }
+}
+
+// One lane per cell c, its faces towards +x, +y(, +z).  Same-single-phase
+// faces: one HLLC per point, stored for both sides; other faces: appended
+// to the general list.  Only sides whose cell is in use are stored.
+template <int D, int ORD>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NQ = Basis<D, ORD>::NQ;
+  constexpr int NQF = D == 3 ? NQ * NQ : NQ;
+  const int nc = g.nc;
+  const int c = xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int ijk[3];
+  cell_ijk(g, c, ijk[0], ijk[1], ijk[2]);
+  const int stride[3] = {g.N[1] * g.N[2], g.N[2], 1};
+  const bool uc = used[c] != 0;
+  const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
+  unsigned pc, ac;
+  phase_masks(g, C, c, pc, ac);
+  // c's coefficients of its single phase, loaded once for its three faces
+  // (the phase's rows start at base pointers, so the row offsets are
+  // uniform multiples of nc)
+  const bool one = pc == ac && __popc(pc) == 1;
+  const int s = one ? __ffs(pc) - 1 : 0;
+  const size_t rows = (size_t)s * U * M;
+  const size_t ncz = (size_t)nc;
+  float cl[U][M], ll[U];
+  if (one) {
+    const double* pl = C.coef + rows * ncz + c;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      ll[u] = (float)C.lim[((size_t)s * U + u) * ncz + c];
+#pragma unroll
+      for (int m = 0; m < M; m++) cl[u][m] = (float)pl[(size_t)(u * M + m) * ncz];
+    }
+  }
+#pragma unroll 1
+  for (int ax = 0; ax < D; ax++) {
+    if (ijk[ax] == g.N[ax] - 1) continue;  // domain boundary: the cell kernel's boundary flux
+    const int nb = c + stride[ax];
+    const bool un = used[nb] != 0;
+    if (!uc && !un) continue;
+    unsigned pn, an;
+    phase_masks(g, C, nb, pn, an);
+    if (!(one && pn == an && pn == pc)) {
+      C.flist[atomicAdd(C.fcount, 1)] = ax * nc + c;
+      continue;
+    }
+    float cr[U][M], lr[U];
+    const double* pr = C.coef + rows * ncz + nb;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      lr[u] = (float)C.lim[((size_t)s * U + u) * ncz + nb];
+#pragma unroll
+      for (int m = 0; m < M; m++) cr[u][m] = (float)pr[(size_t)(u * M + m) * ncz];
+    }
+    const float wl = (float)C.alph[s * nc + c], wr = (float)C.alph[s * nc + nb];
+#pragma unroll 1
+    for (int q = 0; q < NQF; q++) {
+      double off[3], gwd;
+      face_point<D, ORD>(g, ax, q, hp, off, gwd);
+      float dl[3] = {(float)off[0], (float)off[1], (float)off[2]}, dr[3] = {dl[0], dl[1], dl[2]};
+      dl[ax] += (float)hp[ax];
+      dr[ax] -= (float)hp[ax];
+      float pl64[B], pr64[B];
+      basis_tail<D, ORD>(dl, g.q12, pl64);
+      basis_tail<D, ORD>(dr, g.q12, pr64);
+      float UL[U], UR[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        float al = 0, ar = 0;
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+          al += cl[u][1 + b] * (float)pl64[b];
+          ar += cr[u][1 + b] * (float)pr64[b];
+        }
+        UL[u] = cl[u][0] + ll[u] * al;
+        UR[u] = cr[u][0] + lr[u] * ar;
+      }
+      float Fh[U];
+      iface_flux<D, float, true>(g, UL, UR, s, s, ax, 1.f, Fh);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (uc) C.fflux[face_slot<D, ORD>(g, 0, s, q, u, ax, c)] = wr * Fh[u];
+        if (un) C.fflux[face_slot<D, ORD>(g, 1, s, q, u, ax, c)] = wl * Fh[u];
+      }
+    }
+  }
+}
+
+// The listed faces, each side as its cell sees it: for every present phase
+// ss of the side's cell, the sum over the other cell's phases s1 (alpha != 0)
+// of alpha_s1 * flux(ss, s1), with the MGFM ghost state where s1 != ss.
+template <int D, int ORD>
+__global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const int32_t* __restrict__ used) {
+  constexpr int U = D + 2, NQ = Basis<D, ORD>::NQ;
+  constexpr int NQF = D == 3 ? NQ * NQ : NQ;
+  const int nc = g.nc;
+  const int n = *C.fcount;
+  const int stride[3] = {g.N[1] * g.N[2], g.N[2], 1};
+  const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int id = C.flist[t];
+    const int ax = id / nc, c = id % nc;
+    const int nb = c + stride[ax];
+    for (int side = 0; side < 2; side++) {
+      const int ci = side == 0 ? c : nb, cj = side == 0 ? nb : c;
+      if (!used[ci]) continue;
+      const float sgn = side == 0 ? 1.f : -1.f;
+      for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
+        if (ss >= g.F) break;
+        if (C.ph_state[ss * nc + ci] != 1) continue;
+#pragma unroll 1
+        for (int q = 0; q < NQF; q++) {
+          double off[3], gwd;
+          face_point<D, ORD>(g, ax, q, hp, off, gwd);
+          double di[3] = {off[0], off[1], off[2]}, dj[3] = {off[0], off[1], off[2]};
+          di[ax] += side == 0 ? hp[ax] : -hp[ax];
+          dj[ax] -= side == 0 ? hp[ax] : -hp[ax];
+          float Ui[U], acc[U];
+          point_state<D, ORD>(g, C, ci, ss, di, Ui);
+#pragma unroll
+          for (int u = 0; u < U; u++) acc[u] = 0;
+          for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
+            if (s1 >= g.F) break;
+            const float alph_j = (float)C.alph[s1 * nc + cj];
+            if (alph_j == 0) continue;
+            float Uj[U], Fh[U];
+            point_state<D, ORD>(g, C, cj, s1, dj, Uj);
+            iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh);
+#pragma unroll
+            for (int u = 0; u < U; u++) acc[u] += Fh[u] * alph_j;
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, c)] = acc[u];
+        }
+      }
+    }
+  }
+}
+
+template <int D>
+int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st) {
+  if (!C.fflux || !C.flist || !C.fcount) return PGDG_ERR_STATE;
+  hipLaunchKernelGGL(k_face_clear, dim3(1), dim3(64), 0, st, C.fcount);
+  if (g.ord == 2) {
+    hipLaunchKernelGGL((k_face_simple<D, 2>), dim3(grid_xcd(g.nc, 256)), dim3(256), 0, st, g, C, used);
+    hipLaunchKernelGGL((k_face_general<D, 2>), dim3(1024), dim3(128), 0, st, g, C, used);
+  } else {
+    hipLaunchKernelGGL((k_face_simple<D, 1>), dim3(grid_xcd(g.nc, 256)), dim3(256), 0, st, g, C, used);
+    hipLaunchKernelGGL((k_face_general<D, 1>), dim3(1024), dim3(128), 0, st, g, C, used);
+  }
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+template int launch_faces<2>(const Geo&, const CellDev&, const int32_t*, hipStream_t);
+template int launch_faces<3>(const Geo&, const CellDev&, const int32_t*, hipStream_t);
+
+}  // namespace pgdg

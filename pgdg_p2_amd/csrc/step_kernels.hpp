@@ -14,6 +14,10 @@ struct Tuning {
   int limiter = 128;
 };
 
+// Mixed form: interior-face point fluxes into C.fflux (before k_rhs<float>).
+template <int D>
+int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st);
+
 template <int D>
 int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,

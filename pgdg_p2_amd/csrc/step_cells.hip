@@ -256,10 +256,11 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, P2gBuf 
 // ---------------------------------------------------------------------------
 // Calculate_limiter (S3:mpm.cpp:392-738; T2:mpm.cpp:887-1225)
 // ---------------------------------------------------------------------------
-__device__ inline double limiter_fn(double ds, int bd_fg) {  // S3:glbfunc.h:283-299
-  double val = 1;
-  if (ds > 0 && ds < 0.5) val = 2 * ds;
-  else if (ds >= 0.5) val = 1;
+template <typename T>
+__device__ inline T limiter_fn(T ds, int bd_fg) {  // S3:glbfunc.h:283-299
+  T val = 1;
+  if (ds > 0 && ds < T(0.5)) val = 2 * ds;
+  else if (ds >= T(0.5)) val = 1;
   if (bd_fg == 1) val = (fabs(ds) + ds) / (fabs(ds) + 1);
   return val;
 }
@@ -302,7 +303,10 @@ __device__ inline void face_geo(const Geo& g, int i, int j, int k, const double*
   fg.b = in ? 0 : g.bc[f];
 }
 
-template <int D, int ORD, int BS>
+// T: the arithmetic of the slope ratios and the positivity probe (double:
+// the reference's FP64 order; float: the mixed form).  The neighbour
+// extrema and the velocity gradient stay FP64.
+template <int D, int ORD, int BS, typename T>
 __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, NF = 2 * D;
   const int nc = g.nc;
@@ -475,12 +479,12 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
   // ratio of the allowed range to the slope increment at neighbour centres
   for (int s = 0; s < PGDG_MAX_PHASE; s++) {
     if (s >= F) break;
-    double sl[U][M - 1];
+    T sl[U][M - 1];
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-      for (int b = 0; b < M - 1; b++) sl[u][b] = C.coef[CIDX(s, u, 1 + b)];
-    double lim[U];
+      for (int b = 0; b < M - 1; b++) sl[u][b] = (T)C.coef[CIDX(s, u, 1 + b)];
+    T lim[U];
 #pragma unroll
     for (int u = 0; u < U; u++) lim[u] = 1.;
     int lim_fg = bd_fg;
@@ -500,19 +504,20 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
 #pragma unroll
         for (int a = 0; a < 3; a++) dr[a] = fg.cen[a] - xc[a];
       }
-      double phi[M - 1];
-      basis_tail<D, ORD>(dr, g.q12, phi);
+      const T drT[3] = {(T)dr[0], (T)dr[1], (T)dr[2]};
+      T phi[M - 1];
+      basis_tail<D, ORD>(drT, g.q12, phi);
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        double dU = 0;
+        T dU = 0;
 #pragma unroll
         for (int b = 0; b < M - 1; b++) dU += sl[u][b] * phi[b];
-        double sd = 1;
+        T sd = 1;
         if (dU >= 0) {
-          sd = umax[s][u] / (dU + 1e-20);
+          sd = (T)umax[s][u] / (dU + T(1e-20));
           sd = limiter_fn(fabs(sd), lim_fg);
         } else if (dU < 0) {
-          sd = umin[s][u] / (dU + 1e-20);
+          sd = (T)umin[s][u] / (dU + T(1e-20));
           sd = limiter_fn(fabs(sd), lim_fg);
         }
         lim[u] = amin1(lim[u], sd);
@@ -520,31 +525,33 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
     }
     // positivity probe
     int fgp = 0;
-    double c0[U];
+    T c0[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) c0[u] = C.coef[CIDX(s, u, 0)];
+    for (int u = 0; u < U; u++) c0[u] = (T)C.coef[CIDX(s, u, 0)];
+    const EosT<T> mt(Eos{g.gamma[s], g.pinf[s]});
     for (int jj = 0; jj < 4 && !fgp; jj++) {
       for (int k1 = 0; k1 < (D == 3 ? 2 : 1) && !fgp; k1++) {
         double dr[3] = {0, 0, 0};
         dr[0] = g.sqrt2 * g.p_g * g.probe_c[jj];
         dr[1] = g.sqrt2 * g.p_g * g.probe_s[jj];
         if (D == 3) dr[2] = (-0.5 + k1) * g.p_g;
-        double phi[M - 1];
-        basis_tail<D, ORD>(dr, g.q12, phi);
-        double Ui[U];
+        const T drT[3] = {(T)dr[0], (T)dr[1], (T)dr[2]};
+        T phi[M - 1];
+        basis_tail<D, ORD>(drT, g.q12, phi);
+        T Ui[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          double acc = 0;
+          T acc = 0;
 #pragma unroll
           for (int b = 0; b < M - 1; b++) acc += (lim[u] * sl[u][b]) * phi[b];
           Ui[u] = c0[u] + acc;
         }
-        double sq = 0;
+        T sq = 0;
 #pragma unroll
         for (int a = 0; a < D; a++) sq += Ui[1 + a] * Ui[1 + a];
-        double e_p = Ui[U - 1] - 0.5 * sq / Ui[0];
-        double P_p = eos_p(e_p, Eos{g.gamma[s], g.pinf[s]});
-        if (P_p < -0.05 * g.pinf[s] || Ui[0] < 0) fgp = 1;
+        T e_p = Ui[U - 1] - T(0.5) * sq / Ui[0];
+        T P_p = mt.p(e_p);
+        if (P_p < T(-0.05) * mt.pinf || Ui[0] < 0) fgp = 1;
       }
     }
 #pragma unroll
@@ -1029,6 +1036,11 @@ __global__ void __launch_bounds__(256) k_update_coef(Geo g, CellDev C, const int
   const int r = (int)(t / nc);  // (s * U + u) * M + m
   const int m = r % M, su = r / M, s = su / U;
   if (!used[c]) return;
+  // mixed form: an absent phase's coefficients are zero (P2G clears them)
+  // and its rates are read by nothing before the RHS rewrites them, so the
+  // scaling by |Lim_u| is skipped there (the FP64 form keeps the
+  // reference's full sweep)
+  if (g.prec == PGDG_PREC_MIXED && C.ph_state[s * nc + c] != 1) return;
   double cf = C.coef[t], dcf = C.dcoef[t];
   if (m >= 1) {
     const double l = fabs(C.lim[(size_t)su * nc + c]);
@@ -1128,11 +1140,20 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
   const int nc = g.nc;
   switch (which) {
     case CELL_LIMITER:
-      switch (block) {
-        case 64: hipLaunchKernelGGL((k_limiter<D, ORD, 64>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
-        case 128: hipLaunchKernelGGL((k_limiter<D, ORD, 128>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
-        case 256: hipLaunchKernelGGL((k_limiter<D, ORD, 256>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
-        default: return PGDG_ERR_ARG;
+      if (g.prec == PGDG_PREC_MIXED) {
+        switch (block) {
+          case 64: hipLaunchKernelGGL((k_limiter<D, ORD, 64, float>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+          case 128: hipLaunchKernelGGL((k_limiter<D, ORD, 128, float>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+          case 256: hipLaunchKernelGGL((k_limiter<D, ORD, 256, float>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+          default: return PGDG_ERR_ARG;
+        }
+      } else {
+        switch (block) {
+          case 64: hipLaunchKernelGGL((k_limiter<D, ORD, 64, double>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
+          case 128: hipLaunchKernelGGL((k_limiter<D, ORD, 128, double>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
+          case 256: hipLaunchKernelGGL((k_limiter<D, ORD, 256, double>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+          default: return PGDG_ERR_ARG;
+        }
       }
       break;
     case CELL_RHS:

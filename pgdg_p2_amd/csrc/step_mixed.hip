@@ -116,11 +116,13 @@ __device__ __forceinline__ bool box_reach(const float4 blo, const float4 bhi, co
 // cell: rec1(x) / rec2(x) return the entry's records, glob(x) its bin-entry
 // index (for the density and sound speed of interface pairs and for wall
 // normals).  vb, vu: the target's stage-start and current positions in the
-// candidate cell's frame.
+// candidate cell's frame.  The terms are summed in FP32 within the cell
+// (pf, sf: at most the cell's entries) and the caller adds each cell's sums
+// to the FP64 totals.
 template <int D, typename R1, typename R2, typename G>
 __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const float* vu, const PstConst& k,
                                           R1 rec1, R2 rec2, G glob, const MixedBins& MB, const BinData& B,
-                                          const WallDev& W, double* pst, double* sharp) {
+                                          const WallDev& W, float* pst, float* sharp) {
   for (int x = x0; x < x1; x++) {
     const float4 c1 = rec1(x);
     const float cb[3] = {c1.x, c1.y, c1.z};
@@ -157,19 +159,19 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
     if (phj != 10) {
       const float fac = aw * tmp1 * vol_j;
 #pragma unroll
-      for (int a = 0; a < D; a++) pst[a] += (double)(dr[a] * fac);
+      for (int a = 0; a < D; a++) pst[a] = fmaf(dr[a], fac, pst[a]);
       if (phj != k.phi) {
         const float2 c3 = MB.r3[glob(x)];
         const float kk = -0.1f * fmaf(c3.y * c3.y, c3.x, k.cri) * __builtin_amdgcn_rcpf(k.rhoi_f + c3.x);
         const float fs = kk * aw * vol_j;
 #pragma unroll
-        for (int a = 0; a < D; a++) sharp[a] += (double)(dr[a] * fs);
+        for (int a = 0; a < D; a++) sharp[a] = fmaf(dr[a], fs, sharp[a]);
       }
     } else {
       const int w = ref_wall(B.ref[glob(x)]);
       const float fw = W_ * tmp1 * vol_j;
 #pragma unroll
-      for (int a = 0; a < D; a++) pst[a] += (double)((float)W.norm[a * W.n + w] * fw);
+      for (int a = 0; a < D; a++) pst[a] = fmaf((float)W.norm[a * W.n + w], fw, pst[a]);
     }
   }
 }
@@ -231,6 +233,15 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
     cell_ijk(g, pos, pi, pj, pk);
   }
   double pst[3] = {0, 0, 0}, sharp[3] = {0, 0, 0};
+  float pf[3] = {0, 0, 0}, sf[3] = {0, 0, 0};  // the current cell's FP32 sums
+  auto flush = [&]() {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+      pst[a] += (double)pf[a];
+      sharp[a] += (double)sf[a];
+      pf[a] = sf[a] = 0.f;
+    }
+  };
   const int r = g.radius;
   const float psf[3] = {(float)g.ps[0], (float)g.ps[1], (float)g.ps[2]};
   // the target's positions in the frame of a candidate cell (t, s, m):
@@ -340,16 +351,19 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
                 const int m1 = min(pk + r, N2 - 1);
                 for (int m = max(pk - r, 0); m <= m1; m++) {
                   const int lc = m - kA_l;
+                  const int xe = s_off[sg][lc + 1] - sh;  // the cell's end
                   const int x0 = max(s_off[sg][lc] - sh, q0);
-                  const int x1 = min(s_off[sg][lc + 1] - sh, q0 + qn);
+                  const int x1 = min(xe, q0 + qn);
                   if (x0 >= x1) continue;
                   float vb[3], vu[3];
                   frame(I, J, m, vb, vu);
                   if (!box_reach<D>(s_box[sg][lc][0], s_box[sg][lc][1], vb, k.r2_box)) continue;
                   pst_pairs<D>(
                       x0, x1, vb, vu, k, [&](int x) { return s_r1[x - q0]; },
-                      [&](int x) { return s_r2[x - q0]; }, [&](int x) { return x + sh; }, MB, B, W, pst,
-                      sharp);
+                      [&](int x) { return s_r2[x - q0]; }, [&](int x) { return x + sh; }, MB, B, W, pf,
+                      sf);
+                  // a cell cut by the chunk end keeps its sums for the next chunk
+                  if (x1 == xe) flush();
                 }
               }
               __syncthreads();
@@ -368,7 +382,8 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
           float vb[3], vu[3];
           frame(t, s, m, vb, vu);
           if (!box_reach<D>(MB.box[2 * cc], MB.box[2 * cc + 1], vb, k.r2_box)) continue;  // empty or out of reach
-          pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, gidx, MB, B, W, pst, sharp);
+          pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, gidx, MB, B, W, pf, sf);
+          flush();
         }
   }
   // boundary penalty, clamps and the transport velocity in FP64, as in the

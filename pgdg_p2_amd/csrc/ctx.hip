@@ -335,7 +335,12 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   B.ref = bi_slab + ne;
   const int p2g_rows_n = kP2gLight;
   PB.ne = ne;
-  PGDG_HIP_TRY(hipMalloc((void**)&PB.light, sizeof(double) * p2g_rows_n * (size_t)ne));
+  // light records, then the appearing cells' masses and flags
+  const size_t p2g_bytes = sizeof(double) * (p2g_rows_n * (size_t)ne + PGDG_MAX_PHASE * (size_t)nc) +
+                           sizeof(int32_t) * ((size_t)nc + 64);
+  PGDG_HIP_TRY(hipMalloc((void**)&PB.light, p2g_bytes));
+  PB.mass = PB.light + p2g_rows_n * (size_t)ne;
+  PB.flag = (int32_t*)(PB.mass + PGDG_MAX_PHASE * (size_t)nc);
   size_t mb_bytes = 0;
   if (cs.precision == PGDG_PREC_MIXED) {
     // boxes 32 B per cell; r1, r2 (16 B), r3 (8 B), ecell (4 B) per entry
@@ -354,8 +359,8 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PGDG_TRY(T.init(cap, nwall, nc, st));
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall +
-                            9 * (size_t)ne + 6 * (size_t)nc + p2g_rows_n * (size_t)ne) +
-          T.bytes_total + mb_bytes + ff_bytes;
+                            9 * (size_t)ne + 6 * (size_t)nc) +
+          p2g_bytes + T.bytes_total + mb_bytes + ff_bytes;
   return PGDG_OK;
 }
 

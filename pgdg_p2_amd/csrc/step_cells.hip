@@ -52,135 +52,34 @@ __global__ void k_p2g_gather(PtlDev P, const int* __restrict__ bin_ent, const in
   for (int q = 0; q < kP2gLight; q += 2) *reinterpret_cast<double2*>(l + q) = make_double2(r[q], r[q + 1]);
 }
 
-// One lane per cell, cells taken in XCD-local runs (xcd_block).  The first
-// walk over the 3^D neighbour bins reads only the gathered records and
-// forms the phase masses; a second walk, in the same order and only for
-// cells where a phase appears (ph_state 0 -> 1, a thin layer at moving
-// interfaces), forms the weighted state and slope averages from the
-// particle arrays themselves.
-template <int D, int ORD>
-__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, P2gBuf pb,
-                                             const int* __restrict__ bin_off, const int* __restrict__ bin_ent,
-                                             const int32_t* __restrict__ used) {
+// Kernel weight of a particle at squared distance sq from the cell centre
+// (0 beyond the cut-off).  MIX (mixed form): the cut-off test compares
+// squared distances in FP64 and the weight is evaluated in FP32; the FP64
+// form takes the square root and the FP64 weight, as the reference.
+template <int D, bool MIX>
+__device__ inline double p2g_weight(const Geo& g, double sq, double vol) {
+  const double hi = g.p_g * 1 * 3.2 / kCutOff;
+  if constexpr (MIX) {
+    if (!(sq < g.p2g_cut * g.p2g_cut)) return 0.0;
+    const float sk = __builtin_amdgcn_sqrtf((float)sq) * (float)(1.0 / hi);
+    const float q = 1.f - 0.5f * sk;
+    return (double)((float)kern_C<D>(hi) * (((q * q) * q) * q) * (1.f + 2.f * sk) * (float)vol);
+  } else {
+    double dist = sqrt(sq);
+    if (!(dist < g.p2g_cut)) return 0.0;
+    return kern_W<D>(dist, hi) * vol;
+  }
+}
+
+// The end of MLS_P2G for cell c: phase states and fractions, the appearing
+// phases' coefficients from the weighted averages (pav [phase][U], cf
+// [phase][U][B]; read only for appearing phases), absent phases cleared,
+// velocity and pressure.  APPEAR false: the caller knows no phase appears.
+template <int D, int ORD, bool APPEAR>
+__device__ inline void p2g_finish(const Geo& g, const CellDev& C, int c, const double* mass_t,
+                                  const double* pav, const double* cf) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1;
   const int nc = g.nc;
-  int c = xcd_block() * blockDim.x + threadIdx.x;
-  if (c >= nc || !used[c]) return;
-  int i, j, k;
-  cell_ijk(g, c, i, j, k);
-  double xc[3];
-  cell_center<D>(g, i, j, k, xc);
-  const double hi = g.p_g * 1 * 3.2 / kCutOff;
-  double mass_t[PGDG_MAX_PHASE] = {0, 0, 0};
-  double pav[PGDG_MAX_PHASE][U];
-  double cf[PGDG_MAX_PHASE][U][B];
-#pragma unroll
-  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      pav[s][u] = 0;
-#pragma unroll
-      for (int b = 0; b < B; b++) cf[s][u][b] = 0;
-    }
-  }
-  const int kc = D == 3 ? 1 : 0;
-  const int t0 = max(i - 1, 0), t1 = min(i + 1, g.N[0] - 1);
-  const int s0 = max(j - 1, 0), s1 = min(j + 1, g.N[1] - 1);
-  const int q0 = max(k - kc, 0), q1 = min(k + kc, g.N[2] - 1);
-  // pass 1: phase masses
-  for (int t = t0; t <= t1; t++)
-    for (int s_ = s0; s_ <= s1; s_++)
-      for (int q = q0; q <= q1; q++) {
-        int cc = (t * g.N[1] + s_) * g.N[2] + q;
-        int e0 = bin_off[cc], e1 = bin_off[cc + 1];
-        for (int e = e0; e < e1; e++) {
-          const double* __restrict__ lp = pb.light + (int64_t)e * kP2gLight;
-          const double2 l01 = *reinterpret_cast<const double2*>(lp);
-          const double2 l23 = *reinterpret_cast<const double2*>(lp + 2);
-          const double2 l45 = *reinterpret_cast<const double2*>(lp + 4);
-          const int ph = (int)l45.x;
-          if (ph < 0) continue;  // wall particles are not cell members
-          const double xr[3] = {l01.x, l01.y, l23.x};
-          double sq = 0;
-#pragma unroll
-          for (int a = 0; a < D; a++) {
-            const double xa = xr[a] - xc[a];
-            sq += xa * xa;
-          }
-          double dist = sqrt(sq);
-          if (dist < g.p2g_cut) {
-            double ww = kern_W<D>(dist, hi) * l23.y;
-#pragma unroll
-            for (int s = 0; s < PGDG_MAX_PHASE; s++)
-              if (s < g.F && ph == s) mass_t[s] += ww;
-          }
-        }
-      }
-  // which phases appear in this cell
-  unsigned appear = 0;
-#pragma unroll
-  for (int s = 0; s < PGDG_MAX_PHASE; s++)
-    if (s < g.F && mass_t[s] > 0 && C.ph_state_b[s * nc + c] == 0) appear |= 1u << s;
-  if (appear) {
-    // pass 2: state and slope averages of the appearing phases
-    for (int t = t0; t <= t1; t++)
-      for (int s_ = s0; s_ <= s1; s_++)
-        for (int q = q0; q <= q1; q++) {
-          int cc = (t * g.N[1] + s_) * g.N[2] + q;
-          int e0 = bin_off[cc], e1 = bin_off[cc + 1];
-          for (int e = e0; e < e1; e++) {
-            const double* __restrict__ lp = pb.light + (int64_t)e * kP2gLight;
-            const double2 l01 = *reinterpret_cast<const double2*>(lp);
-            const double2 l23 = *reinterpret_cast<const double2*>(lp + 2);
-            const double2 l45 = *reinterpret_cast<const double2*>(lp + 4);
-            const int ph = (int)l45.x;
-            if (ph < 0 || !((appear >> ph) & 1u)) continue;
-            const double xr[3] = {l01.x, l01.y, l23.x};
-            double xij[3];
-#pragma unroll
-            for (int a = 0; a < D; a++) xij[a] = xr[a] - xc[a];
-            double sq = 0;
-#pragma unroll
-            for (int a = 0; a < D; a++) sq += xij[a] * xij[a];
-            double dist = sqrt(sq);
-            if (dist < g.p2g_cut) {
-              double ww = kern_W<D>(dist, hi) * l23.y;
-              const int64_t p = bin_ent[e], cap = P.cap;
-              double pj[U];
-              pj[0] = P.rho[p];
-#pragma unroll
-              for (int a = 0; a < D; a++) pj[1 + a] = P.rhov[a * cap + p];
-              pj[U - 1] = P.E[p];
-              // the particle's polynomial at the cell centre
-              const double nx[3] = {-xij[0], -xij[1], D == 3 ? -xij[2] : 0.0};
-              double psi[B];
-              incr_tail<D, ORD>(nx, psi);
-              double sl[U][B];
-#pragma unroll
-              for (int u = 0; u < U; u++) {
-                double acc = 0;
-#pragma unroll
-                for (int b = 0; b < B; b++) {
-                  sl[u][b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
-                  acc += sl[u][b] * psi[b];
-                }
-                pj[u] += acc;
-              }
-#pragma unroll
-              for (int s = 0; s < PGDG_MAX_PHASE; s++) {
-                if (s < g.F && ph == s) {
-#pragma unroll
-                  for (int u = 0; u < U; u++) pav[s][u] += pj[u] * ww;
-#pragma unroll
-                  for (int u = 0; u < U; u++)
-#pragma unroll
-                    for (int b = 0; b < B; b++) cf[s][u][b] += sl[u][b] * ww;
-                }
-              }
-            }
-          }
-        }
-  }
   double mass_total = 0;
 #pragma unroll
   for (int s = 0; s < PGDG_MAX_PHASE; s++)
@@ -194,16 +93,17 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, P2gBuf 
     C.ph_state[ss * nc + c] = st;
     C.alph[ss * nc + c] = mass_t[ss] / (mass_total + 1e-20);
     int stb = C.ph_state_b[ss * nc + c];
-    if (st == 1 && stb == 0) {
+    if (APPEAR && st == 1 && stb == 0) {
+      const double* pv = pav + ss * U;
+      const double* cs = cf + ss * U * B;
       double pa[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) pa[u] = pav[ss][u] / (mass_t[ss] + 1e-20);
+      for (int u = 0; u < U; u++) pa[u] = pv[u] / (mass_t[ss] + 1e-20);
       if constexpr (ORD == 2)  // centre value -> mean of the zero-mean basis
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-          for (int a = 0; a < D; a++)
-            pa[u] += cf[ss][u][(D == 2 ? 3 : 6) + a] / (mass_t[ss] + 1e-20) * g.q12[a];
+          for (int a = 0; a < D; a++) pa[u] += cs[u * B + (D == 2 ? 3 : 6) + a] / (mass_t[ss] + 1e-20) * g.q12[a];
       double rho = pa[0];
       C.coef[CIDX(ss, 0, 0)] = rho;
       C.rho[ss * nc + c] = rho;
@@ -221,7 +121,7 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, P2gBuf 
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int b = 0; b < B; b++) C.coef[CIDX(ss, u, 1 + b)] = cf[ss][u][b] / (mass_t[ss] + 1e-20);
+        for (int b = 0; b < B; b++) C.coef[CIDX(ss, u, 1 + b)] = cs[u * B + b] / (mass_t[ss] + 1e-20);
 #pragma unroll
       for (int a = 0; a < 3; a++) cv[a] = rv[a] / rho;
       double sq = 0;
@@ -251,6 +151,176 @@ __global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, PtlDev P, P2gBuf 
 #pragma unroll
   for (int a = 0; a < 3; a++) C.v[a * nc + c] = cv[a];
   C.P[c] = cP;
+}
+
+// The 3^D neighbour bins of cell (i, j, k) (clipped to the domain).
+struct P2gBox {
+  int t0, t1, s0, s1, q0, q1;
+};
+template <int D>
+__device__ inline P2gBox p2g_box(const Geo& g, int i, int j, int k) {
+  const int kc = D == 3 ? 1 : 0;
+  return {max(i - 1, 0), min(i + 1, g.N[0] - 1), max(j - 1, 0), min(j + 1, g.N[1] - 1), max(k - kc, 0),
+          min(k + kc, g.N[2] - 1)};
+}
+
+// MLS_P2G, phase masses: one lane per cell, cells taken in XCD-local runs
+// (xcd_block); the walk over the neighbour bins reads only the gathered
+// records.  A cell where no phase appears (ph_state 0 -> 1) is finished
+// here; a cell where one does (a thin layer at moving interfaces) keeps its
+// masses and is flagged for k_p2g_appear.  The pass is bound by L2
+// capacity: each cell reads its 3^D neighbour bins' records, and with more
+// than one wavefront per SIMD the cells in flight on an XCD need more
+// neighbour records than its L2 holds (measured 2x slower).  The workgroup
+// therefore reserves LDS so that one wavefront per SIMD runs.
+constexpr int kP2gMassLds = 80 * 1024;
+template <int D, int ORD, bool MIX>
+__global__ void __launch_bounds__(128) k_p2g(Geo g, CellDev C, P2gBuf pb, const int* __restrict__ bin_off,
+                                             const int32_t* __restrict__ used) {
+  __shared__ int s_occupancy[kP2gMassLds / 4];  // occupancy control only
+  if (threadIdx.x == 0) s_occupancy[0] = 0;
+  const int nc = g.nc;
+  const int c = xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= nc || !used[c]) return;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  double xc[3];
+  cell_center<D>(g, i, j, k, xc);
+  const P2gBox bx = p2g_box<D>(g, i, j, k);
+  double mass_t[PGDG_MAX_PHASE] = {0, 0, 0};
+  for (int t = bx.t0; t <= bx.t1; t++)
+    for (int s_ = bx.s0; s_ <= bx.s1; s_++)
+      for (int q = bx.q0; q <= bx.q1; q++) {
+        int cc = (t * g.N[1] + s_) * g.N[2] + q;
+        int e0 = bin_off[cc], e1 = bin_off[cc + 1];
+        for (int e = e0; e < e1; e++) {
+          const double* __restrict__ lp = pb.light + (int64_t)e * kP2gLight;
+          const double2 l01 = *reinterpret_cast<const double2*>(lp);
+          const double2 l23 = *reinterpret_cast<const double2*>(lp + 2);
+          const double2 l45 = *reinterpret_cast<const double2*>(lp + 4);
+          const int ph = (int)l45.x;
+          if (ph < 0) continue;  // wall particles are not cell members
+          const double xr[3] = {l01.x, l01.y, l23.x};
+          double sq = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            const double xa = xr[a] - xc[a];
+            sq += xa * xa;
+          }
+          const double ww = p2g_weight<D, MIX>(g, sq, l23.y);  // > 0 exactly within the cut-off
+          if (ww != 0.0) {
+#pragma unroll
+            for (int s = 0; s < PGDG_MAX_PHASE; s++)
+              if (s < g.F && ph == s) mass_t[s] += ww;
+          }
+        }
+      }
+  bool appear = false;
+#pragma unroll
+  for (int s = 0; s < PGDG_MAX_PHASE; s++)
+    if (s < g.F && mass_t[s] > 0 && C.ph_state_b[s * nc + c] == 0) appear = true;
+  pb.flag[c] = appear ? 1 : 0;
+  if (appear) {
+#pragma unroll
+    for (int s = 0; s < PGDG_MAX_PHASE; s++) pb.mass[(size_t)s * nc + c] = mass_t[s];
+    return;
+  }
+  p2g_finish<D, ORD, false>(g, C, c, mass_t, nullptr, nullptr);
+}
+
+// MLS_P2G, flagged cells: one lane per cell (the others exit at once), so
+// that the appearing cells spread over the whole grid; the neighbour bins
+// are walked again in the same order, forming the weighted state and slope
+// averages of the appearing phases from the particle arrays.
+template <int D, int ORD, bool MIX>
+__global__ void __launch_bounds__(64) k_p2g_appear(Geo g, CellDev C, PtlDev P, P2gBuf pb,
+                                                   const int* __restrict__ bin_off,
+                                                   const int* __restrict__ bin_ent,
+                                                   const int32_t* __restrict__ used) {
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1;
+  const int nc = g.nc;
+  const int c = xcd_block() * blockDim.x + threadIdx.x;
+  if (c >= nc || !used[c] || !pb.flag[c]) return;
+  const int64_t cap = P.cap;
+  int i, j, k;
+  cell_ijk(g, c, i, j, k);
+  double xc[3];
+  cell_center<D>(g, i, j, k, xc);
+  double mass_t[PGDG_MAX_PHASE];
+  unsigned appear = 0;
+#pragma unroll
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+    mass_t[s] = pb.mass[(size_t)s * nc + c];
+    if (s < g.F && mass_t[s] > 0 && C.ph_state_b[s * nc + c] == 0) appear |= 1u << s;
+  }
+  double pav[PGDG_MAX_PHASE][U];
+  double cf[PGDG_MAX_PHASE][U][B];
+#pragma unroll
+  for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      pav[s][u] = 0;
+#pragma unroll
+      for (int b = 0; b < B; b++) cf[s][u][b] = 0;
+    }
+  }
+  const P2gBox bx = p2g_box<D>(g, i, j, k);
+  for (int t = bx.t0; t <= bx.t1; t++)
+    for (int s_ = bx.s0; s_ <= bx.s1; s_++)
+      for (int q = bx.q0; q <= bx.q1; q++) {
+        int cc = (t * g.N[1] + s_) * g.N[2] + q;
+        int e0 = bin_off[cc], e1 = bin_off[cc + 1];
+        for (int e = e0; e < e1; e++) {
+          const double* __restrict__ lp = pb.light + (int64_t)e * kP2gLight;
+          const double2 l01 = *reinterpret_cast<const double2*>(lp);
+          const double2 l23 = *reinterpret_cast<const double2*>(lp + 2);
+          const double2 l45 = *reinterpret_cast<const double2*>(lp + 4);
+          const int ph = (int)l45.x;
+          if (ph < 0 || !((appear >> ph) & 1u)) continue;
+          const double xr[3] = {l01.x, l01.y, l23.x};
+          double xij[3];
+#pragma unroll
+          for (int a = 0; a < D; a++) xij[a] = xr[a] - xc[a];
+          double sq = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) sq += xij[a] * xij[a];
+          const double ww = p2g_weight<D, MIX>(g, sq, l23.y);
+          if (ww == 0.0) continue;
+          const int64_t p = bin_ent[e];
+          double pj[U];
+          pj[0] = P.rho[p];
+#pragma unroll
+          for (int a = 0; a < D; a++) pj[1 + a] = P.rhov[a * cap + p];
+          pj[U - 1] = P.E[p];
+          // the particle's polynomial at the cell centre
+          const double nx[3] = {-xij[0], -xij[1], D == 3 ? -xij[2] : 0.0};
+          double psi[B];
+          incr_tail<D, ORD>(nx, psi);
+          double sl[U][B];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            double acc = 0;
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+              sl[u][b] = P.coef[(size_t)(u * M + 1 + b) * cap + p];
+              acc += sl[u][b] * psi[b];
+            }
+            pj[u] += acc;
+          }
+#pragma unroll
+          for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+            if (s < g.F && ph == s) {
+#pragma unroll
+              for (int u = 0; u < U; u++) pav[s][u] += pj[u] * ww;
+#pragma unroll
+              for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int b = 0; b < B; b++) cf[s][u][b] += sl[u][b] * ww;
+            }
+          }
+        }
+      }
+  p2g_finish<D, ORD, true>(g, C, c, mass_t, &pav[0][0], &cf[0][0][0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1207,12 +1277,21 @@ int launch_p2g(const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_o
   (void)n_ent;
   hipLaunchKernelGGL((k_p2g_gather<D>), dim3((unsigned)((pb.ne + 255) / 256)), dim3(256), 0, st, P, bin_ent,
                      bin_off + g.nc, pb);
-  if (g.ord == 2)
-    hipLaunchKernelGGL((k_p2g<D, 2>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, P, pb, bin_off,
-                       bin_ent, used);
-  else
-    hipLaunchKernelGGL((k_p2g<D, 1>), dim3(grid_xcd(g.nc, 128)), dim3(128), 0, st, g, C, P, pb, bin_off,
-                       bin_ent, used);
+  const dim3 gr(grid_xcd(g.nc, 128)), bl(128), ga(grid_xcd(g.nc, 64)), ba(64);
+  const bool mix = g.prec == PGDG_PREC_MIXED;
+#define PGDG_P2G(O, X)                                                                                 \
+  do {                                                                                                \
+    hipLaunchKernelGGL((k_p2g<D, O, X>), gr, bl, 0, st, g, C, pb, bin_off, used);                     \
+    hipLaunchKernelGGL((k_p2g_appear<D, O, X>), ga, ba, 0, st, g, C, P, pb, bin_off, bin_ent, used);  \
+  } while (0)
+  if (g.ord == 2) {
+    if (mix) PGDG_P2G(2, true);
+    else PGDG_P2G(2, false);
+  } else {
+    if (mix) PGDG_P2G(1, true);
+    else PGDG_P2G(1, false);
+  }
+#undef PGDG_P2G
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

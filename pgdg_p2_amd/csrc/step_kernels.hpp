@@ -30,6 +30,8 @@ int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, con
 struct P2gBuf {
   double* light;   // [ne][kP2gLight]
   int64_t ne;
+  double* mass;    // [PGDG_MAX_PHASE][nc]: phase masses of the flagged cells
+  int32_t* flag;   // [nc]: 1 where a phase appears
 };
 constexpr int kP2gLight = 6;  // x[3] (or x[2], 0), vol, phase, pad
 

@@ -144,6 +144,7 @@ struct Ctx {
   // kernels, chosen by timing the candidates in place during the first step
   Tuning tune;
   bool autotune = true, tuned = false, tuning_now = false;
+  bool rhs_fused = false;  // the last RHS launch applied the coefficient update
   // mixed form: candidate entries of the particle-shifting sum staged in LDS
   // (3-D); off = every lane walks its candidates in global memory
   int pst_staging = 1;
@@ -343,14 +344,18 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PB.flag = (int32_t*)(PB.mass + PGDG_MAX_PHASE * (size_t)nc);
   size_t mb_bytes = 0;
   if (cs.precision == PGDG_PREC_MIXED) {
-    // boxes 32 B per cell; r1, r2 (16 B), r3 (8 B), ecell (4 B) per entry
-    mb_bytes = 32 * (size_t)nc + (size_t)ne * (16 + 16 + 8 + 4) + 64;
+    // boxes 32 B per cell; r1 (16 B, plus one spare record: the shifting
+    // loop reads one record ahead), r2 (16 B), r3 (8 B), ph, ecell (4 B)
+    // per entry
+    mb_bytes = 32 * (size_t)nc + (size_t)ne * (16 + 16 + 8 + 4 + 4) + 16 + 64;
     PGDG_HIP_TRY(hipMalloc(&mb_slab, mb_bytes));
+    PGDG_HIP_TRY(hipMemset(mb_slab, 0, mb_bytes));
     char* q = (char*)mb_slab;
     MB.box = (float4*)q; q += 32 * (size_t)nc;
-    MB.r1 = (float4*)q; q += 16 * (size_t)ne;
+    MB.r1 = (float4*)q; q += 16 * ((size_t)ne + 1);
     MB.r2 = (float4*)q; q += 16 * (size_t)ne;
     MB.r3 = (float2*)q; q += 8 * (size_t)ne;
+    MB.ph = (int32_t*)q; q += 4 * (size_t)ne;
     MB.ecell = (int32_t*)q;
     MB.ne_cap = ne;
   }
@@ -544,6 +549,8 @@ static_assert(FN_SET_TIMESTEP == PGDG_FN_SET_TIMESTEP && FN_USED == PGDG_FN_USED
 // SSP-RK3), whose particle coefficient copies the last stage overwrites
 // before anything reads them.  Stage-level calls always write the copy.
 constexpr int kCopyLater = 0x100;
+// RHS argument inside a time step: kFuseRk | stage code (see FN_RHS).
+constexpr int kFuseRk = 0x200;
 static int g2p_code(int md) { return (md == 0 || md == 2) ? (md | kCopyLater) : md; }
 
 static int prof_slot(int fn) {
@@ -615,14 +622,24 @@ int Ctx::stage_impl(int fn, int arg) {
         }, &tune.limiter));
       return launch_cells<D_>(CELL_LIMITER, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st,
                               tune.limiter);
-    case FN_RHS:
+    case FN_RHS: {
+      // mixed form inside a time step (kFuseRk | stage code, and not while
+      // the candidate sizes are timed: the fused update is not idempotent):
+      // the RHS kernel applies the stage's coefficient update itself
+      const bool fuse = cs.precision == PGDG_PREC_MIXED && (arg & kFuseRk) && !tuning_now;
+      rhs_fused = fuse;
+      const int ra = fuse ? (arg & 3) : -1;
       if (tuning_now)
         PGDG_TRY(pick_block([&](int b) {
-          return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st, b);
+          return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, -1, st, b);
         }, &tune.rhs));
-      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, 0, st, tune.rhs);
-    case FN_UPDATE_GRID:
-      return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, arg, st);
+      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, ra, st, tune.rhs);
+    }
+    case FN_UPDATE_GRID: {
+      const int ua = (arg & 3) | (rhs_fused ? kCellSkipCoef : 0);
+      rhs_fused = false;
+      return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, ua, st);
+    }
     case FN_UPDATE_PARTICLE: {
       if (!capturing) PGDG_TRY(sync_counts());
       const int64_t n_p = capturing ? PA.cap : np, n_e = capturing ? B.ne_cap : T.n_ent;
@@ -683,7 +700,7 @@ int Ctx::run_simulation(int64_t n) {
       int rc = stage_impl<D_>(FN_NEIGHBORS, md);
       if (rc == PGDG_OK && md == 0) rc = stage_impl<D_>(FN_P2G, 0);
       if (rc == PGDG_OK) rc = stage_impl<D_>(FN_LIMITER, 0);
-      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, 0);
+      if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, kFuseRk | md);
       if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_GRID, md);
       if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, g2p_code(md));
       tuning_now = false;
@@ -699,7 +716,7 @@ int Ctx::run_simulation(int64_t n) {
       PGDG_TRY(stage<D_>(FN_NEIGHBORS, md));
       if (md == 0) PGDG_TRY(stage<D_>(FN_P2G, 0));
       PGDG_TRY(stage<D_>(FN_LIMITER, 0));
-      PGDG_TRY(stage<D_>(FN_RHS, 0));
+      PGDG_TRY(stage<D_>(FN_RHS, kFuseRk | md));
       PGDG_TRY(stage<D_>(FN_UPDATE_GRID, md));
       PGDG_TRY(stage<D_>(FN_UPDATE_PARTICLE, g2p_code(md)));
     }
@@ -731,7 +748,7 @@ int Ctx::rk_stage(int md) {
   int rc = stage_impl<D_>(FN_NEIGHBORS, md);
   if (rc == PGDG_OK && md == 0) rc = stage_impl<D_>(FN_P2G, 0);
   if (rc == PGDG_OK) rc = stage_impl<D_>(FN_LIMITER, 0);
-  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, 0);
+  if (rc == PGDG_OK) rc = stage_impl<D_>(FN_RHS, kFuseRk | md);
   if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_GRID, md);
   if (rc == PGDG_OK) rc = stage_impl<D_>(FN_UPDATE_PARTICLE, g2p_code(md));
   capturing = false;
@@ -836,6 +853,7 @@ int pgdg_stage(pgdg_ctx* x, int fn, int arg) {
   if (!x->c.loaded) return PGDG_ERR_STATE;
   if ((fn == pgdg::FN_UPDATE_GRID || fn == pgdg::FN_UPDATE_PARTICLE) && (arg < 0 || arg > 3))
     return PGDG_ERR_ARG;  // stage codes 0-3
+  if (fn == pgdg::FN_RHS && arg != 0) return PGDG_ERR_ARG;
   return x->c.D == 2 ? x->c.stage<2>(fn, arg) : x->c.stage<3>(fn, arg);
 }
 

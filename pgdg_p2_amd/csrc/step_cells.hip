@@ -639,9 +639,10 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
 // fluxes from the face kernels (each face's HLLC evaluated once for both of
 // its cells) and evaluates only the volume integral and the domain-boundary
 // faces itself.
+// md >= 0 (float form only): also apply stage md's coefficient update.
 template <int D, int ORD, int BS, typename T>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
-k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
+k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __restrict__ sc, int md) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NF = 2 * D, NQ = Basis<D, ORD>::NQ;
   constexpr bool kFaceBuf = sizeof(T) == 4;
   constexpr int NQF = D == 3 ? NQ * NQ : NQ;
@@ -1081,6 +1082,36 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used) {
 #pragma unroll
       for (int u = 0; u < U; u++) C.dcoef[CIDX(ss, u, m)] = dc[u][m] * inv;
     }
+    if (kFaceBuf && md >= 0) {
+      // float form inside a time step: the RK coefficient update of this
+      // present phase (slopes of the coefficients and rates scaled by
+      // |Lim_u|, then the stage combination a0 U0 + a1 (U + dt L)) in place
+      // of k_update_coef's sweep.  The rates just stored are read back from
+      // the cache one variable at a time (holding them in registers
+      // through the update spills).  The stored rates stay unscaled:
+      // Update_grid reads only the means' rates, which the scaling leaves
+      // unchanged.
+      const double dt = sc->dt;
+      const double a0 = md == 0 ? 0. : md == 1 ? 0.5 : md == 2 ? 0.75 : 1. / 3.;
+      const double a1 = md == 0 ? 1. : md == 1 ? 0.5 : md == 2 ? 0.25 : 2. / 3.;
+#pragma unroll 1
+      for (int u = 0; u < U; u++) {
+        const double l = fabs(C.lim[(ss * U + u) * nc + c]);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+          const size_t o = (((size_t)ss * U + u) * M + m) * nc + c;
+          double dcf = C.dcoef[o], cf = C.coef[o];
+          if (m >= 1) {
+            dcf *= l;
+            cf *= l;
+          }
+          double nw = a1 * (cf + dcf * dt);
+          if (md == 0) C.coef0[o] = cf;
+          else nw += a0 * C.coef0[o];
+          C.coef[o] = nw;
+        }
+      }
+    }
   }
 }
 
@@ -1230,26 +1261,29 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
       if (g.prec == PGDG_PREC_MIXED) {
         PGDG_TRY(launch_faces<D>(g, C, used, st));
         switch (block) {
-          case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, float>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
-          case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, float>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
-          case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256, float>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+          case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, float>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used, sc, arg); break;
+          case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, float>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used, sc, arg); break;
+          case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256, float>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used, sc, arg); break;
           default: return PGDG_ERR_ARG;
         }
       } else {
         switch (block) {
-          case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, double>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used); break;
-          case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, double>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used); break;
-          case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256, double>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used); break;
+          case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, double>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used, sc, -1); break;
+          case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, double>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used, sc, -1); break;
+          case 256: hipLaunchKernelGGL((k_rhs<D, ORD, 256, double>), dim3(grid_xcd(nc, 256)), dim3(256), 0, st, g, C, used, sc, -1); break;
           default: return PGDG_ERR_ARG;
         }
       }
       break;
     case CELL_UPDATE:
-      hipLaunchKernelGGL((k_update_coef<D, ORD>),
-                         dim3((unsigned)(((int64_t)g.F * (D + 2) * Basis<D, ORD>::M * nc + 255) / 256)),
-                         dim3(256), 0, st, g, C, used, sc, arg);
+      // arg: the stage code, plus kCellSkipCoef when the RHS kernel has
+      // already applied the coefficient update
+      if (!(arg & kCellSkipCoef))
+        hipLaunchKernelGGL((k_update_coef<D, ORD>),
+                           dim3((unsigned)(((int64_t)g.F * (D + 2) * Basis<D, ORD>::M * nc + 255) / 256)),
+                           dim3(256), 0, st, g, C, used, sc, arg & 3);
       hipLaunchKernelGGL((k_update_grid<D, ORD>), dim3((nc + 255) / 256), dim3(256), 0, st, g, C, used,
-                         sc, arg);
+                         sc, arg & 3);
       break;
     default:
       return PGDG_ERR_ARG;  // P2G: launch_p2g

@@ -6,6 +6,10 @@
 namespace pgdg {
 
 enum { CELL_P2G = 0, CELL_LIMITER = 1, CELL_RHS = 2, CELL_UPDATE = 3 };
+// CELL_RHS arg: -1, or (mixed form) a stage code whose coefficient update
+// the RHS kernel applies itself; CELL_UPDATE arg: the stage code, plus
+// kCellSkipCoef after such an RHS launch.
+constexpr int kCellSkipCoef = 0x100;
 
 // Workgroup sizes of the tuned kernels (64, 128 or 256).
 struct Tuning {
@@ -83,8 +87,9 @@ int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
 //   and upper corner of the box of cell c's stage-start positions in the
 //   cell's own frame (x, y, z; w unused).
 struct MixedBins {
-  float4 *r1, *r2;
-  float2* r3;
+  float4 *r1, *r2;  // r1: stage-start position (cell frame) and its squared norm; r2: position, volume
+  float2* r3;       // density, sound speed
+  int32_t* ph;      // phase (10 for walls, -1 removed)
   int32_t* ecell;
   float4* box;  // [nc][2]
   int64_t ne_cap;

@@ -65,9 +65,10 @@ __global__ void k_bin_gather_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinD
     vol = rho = cs = 0.f;
     ph = -1;
   }
-  MB.r1[e] = make_float4(ub[0], ub[1], ub[2], __int_as_float(ph));
+  MB.r1[e] = make_float4(ub[0], ub[1], ub[2], fmaf(ub[0], ub[0], fmaf(ub[1], ub[1], ub[2] * ub[2])));
   MB.r2[e] = make_float4(u[0], u[1], u[2], vol);
   MB.r3[e] = make_float2(rho, cs);
+  MB.ph[e] = ph;
 }
 
 template <int D>
@@ -94,6 +95,7 @@ __global__ void k_cell_bbox_mixed(Geo g, MixedBins MB, const int* __restrict__ b
 // Per-target constants of the FP32 pair terms.
 struct PstConst {
   float r2, r2_box, Cf, inv_h, dWc, inv_Wdp, tmp1_base, rhoi_f, cri;
+  float cw, ca;  // Cf * inv_Wdp and dWc * tmp1_base
   int phi;
 };
 
@@ -119,20 +121,30 @@ __device__ __forceinline__ bool box_reach(const float4 blo, const float4 bhi, co
 // candidate cell's frame.  The terms are summed in FP32 within the cell
 // (pf, sf: at most the cell's entries) and the caller adds each cell's sums
 // to the FP64 totals.
-template <int D, typename R1, typename R2, typename G>
+template <int D, typename R1, typename R2, typename RP, typename G>
 __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const float* vu, const PstConst& k,
-                                          R1 rec1, R2 rec2, G glob, const MixedBins& MB, const BinData& B,
-                                          const WallDev& W, float* pst, float* sharp) {
-  for (int x = x0; x < x1; x++) {
-    const float4 c1 = rec1(x);
-    const float cb[3] = {c1.x, c1.y, c1.z};
-    float sqb = 0;
+                                          R1 rec1, R2 rec2, RP recp, G glob, const MixedBins& MB,
+                                          const BinData& B, const WallDev& W, float* pst, float* sharp) {
+  // list test |vb - cb|^2 <= r2 as |cb|^2 - 2 vb.cb <= r2 - |vb|^2 (the
+  // record carries |cb|^2)
+  float m2vb[3], thr = k.r2;
 #pragma unroll
-    for (int a = 0; a < D; a++) {
-      const float d = vb[a] - cb[a];
-      sqb = fmaf(d, d, sqb);
-    }
-    if (!(sqb <= k.r2)) continue;
+  for (int a = 0; a < D; a++) {
+    m2vb[a] = -2.f * vb[a];
+    thr = fmaf(-vb[a], vb[a], thr);
+  }
+  if (x0 >= x1) return;
+  // the next record is loaded one iteration ahead (the last iteration reads
+  // one record past x1: both record tables carry a spare record at the end)
+  float4 c1n = rec1(x0);
+  for (int x = x0; x < x1; x++) {
+    const float4 c1 = c1n;
+    c1n = rec1(x + 1);
+    const float cb[3] = {c1.x, c1.y, c1.z};
+    float t = c1.w;
+#pragma unroll
+    for (int a = 0; a < D; a++) t = fmaf(m2vb[a], cb[a], t);
+    if (!(t <= thr)) continue;
     const float4 c2 = rec2(x);
     const float cu[3] = {c2.x, c2.y, c2.z};
     float dr[3] = {0, 0, 0};
@@ -142,34 +154,35 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
       dr[a] = vu[a] - cu[a];
       sq = fmaf(dr[a], dr[a], sq);
     }
-    if (sq == 0.f) continue;
+    // (a pair at zero distance, the target itself included, adds dr = 0)
     // Wendland C2: W = C q^4 (1 + 2 s), dW/r = -5 C q^3 / h^2 with
-    // q = max(1 - s/2, 0), s = r / h
+    // q = max(1 - s/2, 0), s = r / h; the particle-shifting factor
+    // (dW/r) * tmp1 with tmp1 = base (1 + 0.2 (W / W(dx))^4) is formed as
+    // (dWc base) q^3 (1 + 0.2 rr^4), rr = (C / W(dx)) q^4 (1 + 2 s)
     const float dist = __builtin_amdgcn_sqrtf(sq);
     const float sk = dist * k.inv_h;
     const float q = fmaxf(fmaf(-0.5f, sk, 1.f), 0.f);
     const float q3 = (q * q) * q;
-    const float W_ = k.Cf * (q3 * q) * fmaf(2.f, sk, 1.f);
-    const float aw = k.dWc * q3;
-    const float rr = W_ * k.inv_Wdp;
+    const float t2 = fmaf(2.f, sk, 1.f);
+    const float rr = k.cw * (q3 * q) * t2;
     const float q2 = rr * rr;
-    const float tmp1 = k.tmp1_base * fmaf(0.2f, q2 * q2, 1.f);
-    const int phj = __float_as_int(c1.w);
+    const float poly = fmaf(0.2f, q2 * q2, 1.f);
+    const int phj = recp(x);
     const float vol_j = c2.w;
     if (phj != 10) {
-      const float fac = aw * tmp1 * vol_j;
+      const float fac = (k.ca * q3) * poly * vol_j;
 #pragma unroll
       for (int a = 0; a < D; a++) pst[a] = fmaf(dr[a], fac, pst[a]);
       if (phj != k.phi) {
         const float2 c3 = MB.r3[glob(x)];
         const float kk = -0.1f * fmaf(c3.y * c3.y, c3.x, k.cri) * __builtin_amdgcn_rcpf(k.rhoi_f + c3.x);
-        const float fs = kk * aw * vol_j;
+        const float fs = kk * (k.dWc * q3) * vol_j;
 #pragma unroll
         for (int a = 0; a < D; a++) sharp[a] = fmaf(dr[a], fs, sharp[a]);
       }
     } else {
       const int w = ref_wall(B.ref[glob(x)]);
-      const float fw = W_ * tmp1 * vol_j;
+      const float fw = (k.Cf * (q3 * q) * t2) * (k.tmp1_base * poly) * vol_j;
 #pragma unroll
       for (int a = 0; a < D; a++) pst[a] = fmaf((float)W.norm[a * W.n + w], fw, pst[a]);
     }
@@ -209,7 +222,7 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
     const float2 t3 = MB.r3[e];
     ubt[0] = t1.x, ubt[1] = t1.y, ubt[2] = t1.z;
     ut[0] = t2.x, ut[1] = t2.y, ut[2] = t2.z;
-    k.phi = __float_as_int(t1.w);
+    k.phi = MB.ph[e];
     double xbi[3];
 #pragma unroll
     for (int a = 0; a < D; a++) xbi[a] = S.xb[a * cap + i];
@@ -227,6 +240,8 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
     k.r2 = rlim * rlim;
     k.r2_box = k.r2 * 1.0001f;
     k.tmp1_base = (float)(-1. * P_b * (1. / rhoi));
+    k.cw = (float)(Cw / kern_W<D>(dxk, hi));
+    k.ca = (float)(-5.0 * Cw / (hi * hi) * (-1. * P_b * (1. / rhoi)));
     k.rhoi_f = t3.x;
     k.cri = t3.y * t3.y * t3.x;
     const int pos = cell_of<D>(g, xbi);
@@ -257,12 +272,14 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
   };
   auto grec1 = [&](int x) { return MB.r1[x]; };
   auto grec2 = [&](int x) { return MB.r2[x]; };
+  auto grecp = [&](int x) { return MB.ph[x]; };
   auto gidx = [](int x) { return x; };
   bool staged = false;
   if constexpr (D == 3) {
     constexpr int CAP = BS == 256 ? 640 : BS == 128 ? 384 : 192;  // staged entries
     constexpr int KMAX = BS == 256 ? 96 : BS == 128 ? 64 : 40;    // staged cells per line
-    __shared__ float4 s_r1[CAP], s_r2[CAP];
+    __shared__ float4 s_r1[CAP + 1], s_r2[CAP];  // s_r1[CAP]: the spare record
+    __shared__ int s_ph[CAP];
     __shared__ float4 s_box[2][KMAX][2];
     __shared__ int s_off[2][KMAX + 1];
     __shared__ int s_agg[2][6];  // per line: min, max of (pi - i), (pj - j), pk
@@ -345,29 +362,28 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
                 const int f = x < n0[0] ? a0[0] + x : a0[1] + (x - n0[0]);
                 s_r1[q] = MB.r1[f];
                 s_r2[q] = MB.r2[f];
+                s_ph[q] = MB.ph[f];
               }
               __syncthreads();
               if (mine) {
                 const int m1 = min(pk + r, N2 - 1);
                 for (int m = max(pk - r, 0); m <= m1; m++) {
                   const int lc = m - kA_l;
-                  const int xe = s_off[sg][lc + 1] - sh;  // the cell's end
                   const int x0 = max(s_off[sg][lc] - sh, q0);
-                  const int x1 = min(xe, q0 + qn);
+                  const int x1 = min(s_off[sg][lc + 1] - sh, q0 + qn);
                   if (x0 >= x1) continue;
                   float vb[3], vu[3];
                   frame(I, J, m, vb, vu);
                   if (!box_reach<D>(s_box[sg][lc][0], s_box[sg][lc][1], vb, k.r2_box)) continue;
                   pst_pairs<D>(
                       x0, x1, vb, vu, k, [&](int x) { return s_r1[x - q0]; },
-                      [&](int x) { return s_r2[x - q0]; }, [&](int x) { return x + sh; }, MB, B, W, pf,
-                      sf);
-                  // a cell cut by the chunk end keeps its sums for the next chunk
-                  if (x1 == xe) flush();
+                      [&](int x) { return s_r2[x - q0]; }, [&](int x) { return s_ph[x - q0]; },
+                      [&](int x) { return x + sh; }, MB, B, W, pf, sf);
                 }
               }
               __syncthreads();
             }
+            flush();  // the line's FP32 sums (a line may span chunks)
           }
       }
     }
@@ -376,15 +392,16 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
   if (!staged) {
     const int rk = D == 3 ? r : 0;
     for (int t = max(pi - r, 0); t <= min(pi + r, g.N[0] - 1); t++)
-      for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
+      for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++) {
         for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
           const int cc = (t * g.N[1] + s) * g.N[2] + m;
           float vb[3], vu[3];
           frame(t, s, m, vb, vu);
           if (!box_reach<D>(MB.box[2 * cc], MB.box[2 * cc + 1], vb, k.r2_box)) continue;  // empty or out of reach
-          pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, gidx, MB, B, W, pf, sf);
-          flush();
+          pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, grecp, gidx, MB, B, W, pf, sf);
         }
+        flush();  // the line's FP32 sums
+      }
   }
   // boundary penalty, clamps and the transport velocity in FP64, as in the
   // FP64 kernel

@@ -546,8 +546,9 @@ static_assert(FN_SET_TIMESTEP == PGDG_FN_SET_TIMESTEP && FN_USED == PGDG_FN_USED
 
 // Particle-update argument inside a time step: the stage code, plus
 // kCopyLater for the stages before the last (0 of either scheme, 2 of
-// SSP-RK3), whose particle coefficient copies the last stage overwrites
-// before anything reads them.  Stage-level calls always write the copy.
+// SSP-RK3), whose particle coefficient copies and conserved state the last
+// stage overwrites before anything reads them.  Stage-level calls always
+// write them.
 constexpr int kCopyLater = 0x100;
 // RHS argument inside a time step: kFuseRk | stage code (see FN_RHS).
 constexpr int kFuseRk = 0x200;

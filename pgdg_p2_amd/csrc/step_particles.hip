@@ -50,9 +50,10 @@ __global__ void k_in_ele(Geo g, PtlDev P, StageBufs S, int64_t n) {
 // md: 0 first stage, 1 Heun second stage, 2 and 3 the SSP-RK3 stages (the
 // cell update's stage codes).  Order 2: the particle's copy carries the
 // gradient of the cell polynomial at the particle and its second-order
-// coefficients.  copy == false skips the coefficient copy: within a time
-// step only P2G (first stage) and split/merge (between steps) read it, so
-// the copies written by stages before the last are never read.
+// coefficients.  copy == false skips the coefficient copy and the
+// conserved state (momentum, energy, pressure, mass): within a time step
+// only P2G (first stage) and split/merge (between steps) read them, so the
+// values written by stages before the last are never read.
 template <int D, int ORD>
 __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S,
                                              const DevScalars* __restrict__ sc, int md, bool copy) {
@@ -153,15 +154,16 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
   P.vol[i] = vol;
   P.rho[i] = rho;
 #pragma unroll
-  for (int a = 0; a < 3; a++) {
-    P.rhov[a * cap + i] = rv[a];
-    P.v[a * cap + i] = v[a];
-  }
-  P.E[i] = E;
-  P.P[i] = Pp;
+  for (int a = 0; a < 3; a++) P.v[a * cap + i] = v[a];
   P.c[i] = cs;
-  P.mass[i] = vol * rho;
   P.h[i] = root_d<D>(vol) * kCellRatio / kCutOff;
+  if (copy) {  // read only by P2G and split/merge, as the copy
+#pragma unroll
+    for (int a = 0; a < 3; a++) P.rhov[a * cap + i] = rv[a];
+    P.E[i] = E;
+    P.P[i] = Pp;
+    P.mass[i] = vol * rho;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -55,7 +55,7 @@ __global__ void k_used_mark(Geo g, const int32_t* ph_state, int32_t* used, int D
 // the context stream; pgdg_profile_read sums the elapsed times per slot.
 enum {
   PS_TIMESTEP = 0, PS_USED, PS_BIN, PS_SPLIT, PS_MERGE, PS_MEMBERSHIP, PS_STAGE_SAVE, PS_P2G,
-  PS_LIMITER, PS_RHS, PS_UPDATE_GRID, PS_G2P, PS_PST, PS_LISTS, PS_NSLOT
+  PS_LIMITER, PS_RHS, PS_UPDATE_GRID, PS_G2P, PS_PST, PS_LISTS, PS_PST_PREP, PS_NSLOT
 };
 
 struct Profiler {
@@ -647,13 +647,25 @@ int Ctx::stage_impl(int fn, int arg) {
       Profiler::Rec r;
       if (prof.on) PGDG_TRY(prof.begin(PS_G2P, st, r));
       PGDG_TRY(launch_g2p<D_>(g, P, C, S, n_p, sc, arg & 3, !(arg & kCopyLater), st));
+      // parts: 1 the neighbour records and cell boxes, 2 the shifting
+      // kernel, 3 both (timed separately when profiling)
+      auto pst = [&](int parts) -> int {
+        if (cs.precision == PGDG_PREC_MIXED)
+          return launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst,
+                                      pst_staging, parts);
+        return launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst, parts);
+      };
       if (prof.on) {
         PGDG_TRY(prof.end(r, st));
+        PGDG_TRY(prof.begin(PS_PST_PREP, st, r));
+        PGDG_TRY(pst(1));
+        PGDG_TRY(prof.end(r, st));
         PGDG_TRY(prof.begin(PS_PST, st, r));
+        PGDG_TRY(pst(2));
+      } else {
+        PGDG_TRY(pst(3));
       }
       if (cs.precision == PGDG_PREC_MIXED) {
-        PGDG_TRY(launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst,
-                                      pst_staging));
         if (tuning_now) {  // the gathered records are in place: time the shifting kernel alone
           PGDG_TRY(pick_block([&](int b) {
             return launch_pst_mixed_kernel<D_>(g, P, W, S, B, MB, n_e, T.bin_off, sc, st, b, pst_staging);
@@ -662,7 +674,6 @@ int Ctx::stage_impl(int fn, int arg) {
                                                pst_staging));
         }
       } else {
-        PGDG_TRY(launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst));
         if (tuning_now) {
           PGDG_TRY(pick_block([&](int b) {
             return launch_pst_kernel<D_>(g, P, W, S, B, n_e, T.bin_off, sc, st, b);

@@ -74,7 +74,7 @@ struct BinData {
 template <int D>
 int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                       const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
-                      const DevScalars* sc, hipStream_t st, int block);
+                      const DevScalars* sc, hipStream_t st, int block, int parts = 3);
 template <int D>
 int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                       const BinData& B, int64_t n_ent, const int* bin_off, const DevScalars* sc,
@@ -98,7 +98,8 @@ struct MixedBins {
 template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging);
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging,
+                     int parts = 3);
 template <int D>
 int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                             const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,

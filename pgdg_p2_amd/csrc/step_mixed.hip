@@ -500,20 +500,24 @@ int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, con
 template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
-                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging) {
+                     const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging,
+                     int parts) {
   if (n_ent <= 0) return PGDG_OK;
   if (n_ent > MB.ne_cap) return PGDG_ERR_CAPACITY;
-  hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
-                     P, W, S, B, MB, bin_off, bin_ent, sc);
-  hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
-  PGDG_TRY(launch_pst_mixed_kernel<D>(g, P, W, S, B, MB, n_ent, bin_off, sc, st, block, staging));
+  if (parts & 1) {
+    hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
+                       P, W, S, B, MB, bin_off, bin_ent, sc);
+    hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
+    PGDG_HIP_TRY(hipGetLastError());
+  }
+  if (parts & 2) PGDG_TRY(launch_pst_mixed_kernel<D>(g, P, W, S, B, MB, n_ent, bin_off, sc, st, block, staging));
   return PGDG_OK;
 }
 
 #define PGDG_INST_MIXED(D)                                                                         \
   template int launch_pst_mixed<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,     \
                                    const BinData&, const MixedBins&, int64_t, const int*,          \
-                                   const int*, const DevScalars*, hipStream_t, int, int);          \
+                                   const int*, const DevScalars*, hipStream_t, int, int, int);     \
   template int launch_pst_mixed_kernel<D>(const Geo&, const PtlDev&, const WallDev&,                \
                                           const StageBufs&, const BinData&, const MixedBins&,       \
                                           int64_t, const int*, const DevScalars*, hipStream_t, int, \

@@ -430,12 +430,14 @@ int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
 template <int D>
 int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                       const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
-                      const DevScalars* sc, hipStream_t st, int block) {
+                      const DevScalars* sc, hipStream_t st, int block, int parts) {
   if (n_ent <= 0) return PGDG_OK;
-  hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S,
-                     B, bin_ent, sc);
-  hipLaunchKernelGGL(k_cell_bbox<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, B, bin_off);
-  PGDG_TRY(launch_pst_kernel<D>(g, P, W, S, B, n_ent, bin_off, sc, st, block));
+  if (parts & 1) {
+    hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S,
+                       B, bin_ent, sc);
+    hipLaunchKernelGGL(k_cell_bbox<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, B, bin_off);
+  }
+  if (parts & 2) PGDG_TRY(launch_pst_kernel<D>(g, P, W, S, B, n_ent, bin_off, sc, st, block));
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
@@ -586,7 +588,7 @@ int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32
                              const DevScalars*, int, bool, hipStream_t);                          \
   template int launch_pst_binned<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \
                                     const BinData&, int64_t, const int*, const int*,              \
-                                    const DevScalars*, hipStream_t, int);                         \
+                                    const DevScalars*, hipStream_t, int, int);                    \
   template int launch_pst_kernel<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \
                                     const BinData&, int64_t, const int*, const DevScalars*,       \
                                     hipStream_t, int);                                            \

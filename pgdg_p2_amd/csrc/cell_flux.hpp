@@ -20,6 +20,25 @@ struct EosT {
   __device__ T rhoe(T p_) const { return (p_ + gpinf) / gm1; }
 };
 
+// Quotient and square root in arithmetic T: correctly rounded for double
+// (the FP64 form's operations), the hardware reciprocal and square root for
+// float (the mixed form; within 2 ulp).
+template <typename T>
+__device__ inline T qdiv(T a, T b) {
+  if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+  else return a / b;
+}
+template <typename T>
+__device__ inline T qrcp(T b) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(b);
+  else return T(1) / b;
+}
+template <typename T>
+__device__ inline T qsqrt(T a) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_sqrtf(a);
+  else return sqrt(a);
+}
+
 // Flux column b of DF = U v^T + P I (+ v_b P in the energy row).  b is a
 // runtime axis: it selects through comparisons rather than array indexing,
 // so the operands stay in registers (a runtime index into a local array
@@ -80,7 +99,9 @@ __device__ inline size_t face_slot(const Geo& g, int side, int ss, int q, int u,
 // neighbour state is replaced by the MGFM ghost state in phase ss's EOS
 // (Newton solve in FP64); kSamePhase: the caller guarantees ss == s1 (no
 // ghost state code).  Fh: the flux the reference multiplies by the basis,
-// the weights and alpha_j.
+// the weights and alpha_j.  Used by the mixed form only (T = float, with the
+// hardware reciprocal and square root); the FP64 form keeps the reference's
+// operations inside its cell kernel.
 template <int D, typename T, bool kSamePhase = false>
 __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss, int s1, int ax, T sgn,
                                   T* Fh) {
@@ -93,17 +114,18 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
   for (int a = 0; a < 3; a++)
     if (a == ax) nrm[a] = sgn;
   const T rho_i = Ui[0], E_i = Ui[U - 1];
+  const T ir_i = qrcp(rho_i);
   T rvi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; a++) rvi[a] = Ui[1 + a];
 #pragma unroll
-  for (int a = 0; a < D; a++) vi[a] = rvi[a] / rho_i;
+  for (int a = 0; a < D; a++) vi[a] = rvi[a] * ir_i;
   T sq = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) sq += rvi[a] * rvi[a];
-  const T e_i = E_i - T(0.5) * sq / rho_i;
+  const T e_i = E_i - T(0.5) * sq * ir_i;
   const T P_i = mt.p(e_i);
-  const T ci = sqrt(mt.g * (mt.pinf + P_i) / rho_i);
+  const T ci = qsqrt(mt.g * (mt.pinf + P_i) * ir_i);
   T vni = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
@@ -113,8 +135,9 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
   T rvj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
 #pragma unroll
   for (int a = 0; a < D; a++) rvj[a] = Uj[1 + a];
+  const T ir_j = qrcp(rho_j);
 #pragma unroll
-  for (int a = 0; a < D; a++) vj[a] = rvj[a] / rho_j;
+  for (int a = 0; a < D; a++) vj[a] = rvj[a] * ir_j;
   T sqj = 0;
 #pragma unroll
   for (int a = 0; a < D; a++) sqj += vj[a] * vj[a];
@@ -133,13 +156,13 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
     for (int a = 0; a < D; a++) vj[a] = vi[a] - vni * nrm[a] + vnj * nrm[a];
 #pragma unroll
     for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
-    rhoe_j = mt.rhoe(P_j);
+    rhoe_j = qdiv(P_j + mt.gpinf, mt.gm1);
     T sq2 = 0;
 #pragma unroll
     for (int a = 0; a < D; a++) sq2 += vj[a] * vj[a];
     E_j = rhoe_j + T(0.5) * rho_j * sq2;
   }
-  const T cj = sqrt(mt.g * (mt.pinf + P_j) / rho_j);
+  const T cj = qsqrt(qdiv(mt.g * (mt.pinf + P_j), rho_j));
   T Ujs[U];
   Ujs[0] = rho_j;
 #pragma unroll
@@ -155,27 +178,27 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
   for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
   const T SL = amin1(vni - ci, vnj - cj);
   const T SR = amax1(vni + ci, vnj + cj);
-  const T S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
-                (rho_i * (SL - vni) - rho_j * (SR - vnj));
+  const T S_s = qdiv((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj),
+                     rho_i * (SL - vni) - rho_j * (SR - vnj));
   if (SL > 0) {
 #pragma unroll
     for (int u = 0; u < U; u++) Fh[u] = DFi[u];
   } else if (SL <= 0 && S_s >= 0) {
-    const T rho_sL = rho_i * (SL - vni) / (SL - S_s);
+    const T rho_sL = qdiv(rho_i * (SL - vni), SL - S_s);
     T UsL[U];
     UsL[0] = rho_sL;
 #pragma unroll
     for (int a = 0; a < D; a++) UsL[1 + a] = rho_sL * (vi[a] + (S_s - vni) * nrm[a]);
-    UsL[U - 1] = rho_sL * (E_i / rho_i + (S_s - vni) * (S_s + (P_i) / (rho_i * (SL - vni))));
+    UsL[U - 1] = rho_sL * (E_i * ir_i + (S_s - vni) * (S_s + qdiv(P_i, rho_i * (SL - vni))));
 #pragma unroll
     for (int u = 0; u < U; u++) Fh[u] = DFi[u] + SL * (UsL[u] - Ui[u]) * sgn;
   } else if (S_s <= 0 && SR >= 0) {
-    const T rho_sR = rho_j * (SR - vnj) / (SR - S_s);
+    const T rho_sR = qdiv(rho_j * (SR - vnj), SR - S_s);
     T UsR[U];
     UsR[0] = rho_sR;
 #pragma unroll
     for (int a = 0; a < D; a++) UsR[1 + a] = rho_sR * (vj[a] + (S_s - vnj) * nrm[a]);
-    UsR[U - 1] = rho_sR * (E_j / rho_j + (S_s - vnj) * (S_s + (P_j) / (rho_j * (SR - vnj))));
+    UsR[U - 1] = rho_sR * (qdiv(E_j, rho_j) + (S_s - vnj) * (S_s + qdiv(P_j, rho_j * (SR - vnj))));
 #pragma unroll
     for (int u = 0; u < U; u++) Fh[u] = DFj[u] + SR * (UsR[u] - Ujs[u]) * sgn;
   } else if (SR <= 0) {

@@ -550,9 +550,12 @@ static_assert(FN_SET_TIMESTEP == PGDG_FN_SET_TIMESTEP && FN_USED == PGDG_FN_USED
 // stage overwrites before anything reads them.  Stage-level calls always
 // write them.
 constexpr int kCopyLater = 0x100;
+// ... and kBinsKept for the stages after the first: the bins (and the
+// mixed form's entry-to-cell map built from them) are those of the first.
+constexpr int kBinsKept = 0x400;
 // RHS argument inside a time step: kFuseRk | stage code (see FN_RHS).
 constexpr int kFuseRk = 0x200;
-static int g2p_code(int md) { return (md == 0 || md == 2) ? (md | kCopyLater) : md; }
+static int g2p_code(int md) { return ((md == 0 || md == 2) ? (md | kCopyLater) : md) | (md != 0 ? kBinsKept : 0); }
 
 static int prof_slot(int fn) {
   switch (fn) {
@@ -652,7 +655,7 @@ int Ctx::stage_impl(int fn, int arg) {
       auto pst = [&](int parts) -> int {
         if (cs.precision == PGDG_PREC_MIXED)
           return launch_pst_mixed<D_>(g, P, W, S, B, MB, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst,
-                                      pst_staging, parts);
+                                      pst_staging, parts, (arg & kBinsKept) != 0);
         return launch_pst_binned<D_>(g, P, W, S, B, n_e, T.bin_off, T.bin_ent, sc, st, tune.pst, parts);
       };
       if (prof.on) {

@@ -584,10 +584,10 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
         for (int b = 0; b < M - 1; b++) dU += sl[u][b] * phi[b];
         T sd = 1;
         if (dU >= 0) {
-          sd = (T)umax[s][u] / (dU + T(1e-20));
+          sd = qdiv((T)umax[s][u], dU + T(1e-20));
           sd = limiter_fn(fabs(sd), lim_fg);
         } else if (dU < 0) {
-          sd = (T)umin[s][u] / (dU + T(1e-20));
+          sd = qdiv((T)umin[s][u], dU + T(1e-20));
           sd = limiter_fn(fabs(sd), lim_fg);
         }
         lim[u] = amin1(lim[u], sd);
@@ -619,7 +619,7 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
         T sq = 0;
 #pragma unroll
         for (int a = 0; a < D; a++) sq += Ui[1 + a] * Ui[1 + a];
-        T e_p = Ui[U - 1] - T(0.5) * sq / Ui[0];
+        T e_p = Ui[U - 1] - qdiv(T(0.5) * sq, Ui[0]);
         T P_p = mt.p(e_p);
         if (P_p < T(-0.05) * mt.pinf || Ui[0] < 0) fgp = 1;
       }
@@ -705,11 +705,11 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
 #pragma unroll
           for (int a = 0; a < D; a++) rv[a] = Up[1 + a];
 #pragma unroll
-          for (int a = 0; a < D; a++) v[a] = rv[a] / Up[0];
+          for (int a = 0; a < D; a++) v[a] = qdiv(rv[a], Up[0]);
           T sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
-          T e_p = Up[U - 1] - T(0.5) * sq / Up[0];
+          T e_p = Up[U - 1] - qdiv(T(0.5) * sq, Up[0]);
           T P_p = mt.p(e_p);
           T col[D][U];
 #pragma unroll
@@ -839,13 +839,13 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
 #pragma unroll
           for (int a = 0; a < D; a++) rvi[a] = Ui[1 + a];
 #pragma unroll
-          for (int a = 0; a < D; a++) vi[a] = rvi[a] / rho_i;
+          for (int a = 0; a < D; a++) vi[a] = qdiv(rvi[a], rho_i);
           T sq = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) sq += rvi[a] * rvi[a];
-          T e_i = E_i - T(0.5) * sq / rho_i;
+          T e_i = E_i - qdiv(T(0.5) * sq, rho_i);
           T P_i = mt.p(e_i);
-          T ci = sqrt(mt.g * (mt.pinf + P_i) / rho_i);
+          T ci = qsqrt(qdiv(mt.g * (mt.pinf + P_i), rho_i));
           T vni = 0;
 #pragma unroll
           for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
@@ -1019,7 +1019,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
 #pragma unroll
             for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
             Ujs[U - 1] = E_j;
-            T cj = sqrt(mt.g * (mt.pinf + P_j) / rho_j);
+            T cj = qsqrt(qdiv(mt.g * (mt.pinf + P_j), rho_j));
             T vnj = 0;
 #pragma unroll
             for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
@@ -1034,7 +1034,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
             } else if (SL <= 0 && SR >= 0) {
 #pragma unroll
               for (int u = 0; u < U; u++)
-                Fh[u] = (SR * DFi[u] - SL * DFj[u] + SL * SR * (Ujs[u] - Ui[u]) * sgn) / (SR - SL);
+                Fh[u] = qdiv(SR * DFi[u] - SL * DFj[u] + SL * SR * (Ujs[u] - Ui[u]) * sgn, SR - SL);
             } else if (SR < 0) {
 #pragma unroll
               for (int u = 0; u < U; u++) Fh[u] = DFj[u];

@@ -99,7 +99,7 @@ template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
                      const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging,
-                     int parts = 3);
+                     int parts = 3, bool bins_unchanged = false);
 template <int D>
 int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                             const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,

@@ -17,28 +17,34 @@
 
 namespace pgdg {
 
-// Bin cell of every entry (binary search over the bin offsets) and its
-// FP32 records.
+// Bin cell of every entry (binary search over the bin offsets; search ==
+// false: the bins have not changed since the last search, reuse it) and
+// its FP32 records.
 template <int D>
 __global__ void k_bin_gather_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B, MixedBins MB,
                                    const int* __restrict__ bin_off, const int* __restrict__ bin_ent,
-                                   const DevScalars* __restrict__ sc) {
+                                   const DevScalars* __restrict__ sc, bool search) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= sc->nent_dev) return;
-  int lo = 0, hi = g.nc - 1;  // last cell with bin_off[c] <= e
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (bin_off[mid] <= e) lo = mid;
-    else hi = mid - 1;
+  int c;
+  if (search) {
+    int lo = 0, hi = g.nc - 1;  // last cell with bin_off[c] <= e
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (bin_off[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    c = lo;
+  } else {
+    c = MB.ecell[e];
   }
-  const int c = lo;
   int ci, cj, ck;
   cell_ijk(g, c, ci, cj, ck);
   const double corner[3] = {g.box_l[0] + ci * g.ps[0], g.box_l[1] + cj * g.ps[1],
                             D == 3 ? g.box_l[2] + ck * g.ps[2] : 0.0};
   const int ref = bin_ent[e];
   B.ref[e] = ref;
-  MB.ecell[e] = c;
+  if (search) MB.ecell[e] = c;
   float ub[3] = {0, 0, 0}, u[3] = {0, 0, 0};
   float vol, rho, cs;
   int ph;
@@ -501,12 +507,12 @@ template <int D>
 int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                      const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
                      const int* bin_ent, const DevScalars* sc, hipStream_t st, int block, int staging,
-                     int parts) {
+                     int parts, bool bins_unchanged) {
   if (n_ent <= 0) return PGDG_OK;
   if (n_ent > MB.ne_cap) return PGDG_ERR_CAPACITY;
   if (parts & 1) {
     hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
-                       P, W, S, B, MB, bin_off, bin_ent, sc);
+                       P, W, S, B, MB, bin_off, bin_ent, sc, !bins_unchanged);
     hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
     PGDG_HIP_TRY(hipGetLastError());
   }
@@ -517,7 +523,7 @@ int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const Stag
 #define PGDG_INST_MIXED(D)                                                                         \
   template int launch_pst_mixed<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,     \
                                    const BinData&, const MixedBins&, int64_t, const int*,          \
-                                   const int*, const DevScalars*, hipStream_t, int, int, int);     \
+                                   const int*, const DevScalars*, hipStream_t, int, int, int, bool);\
   template int launch_pst_mixed_kernel<D>(const Geo&, const PtlDev&, const WallDev&,                \
                                           const StageBufs&, const BinData&, const MixedBins&,       \
                                           int64_t, const int*, const DevScalars*, hipStream_t, int, \

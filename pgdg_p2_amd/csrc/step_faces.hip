@@ -33,23 +33,6 @@ __device__ inline void phase_masks(const Geo& g, const CellDev& C, int c, unsign
   }
 }
 
-// Cell c's phase s at a face point (offset dr from its centre), FP32.
-template <int D, int ORD>
-__device__ inline void point_state(const Geo& g, const CellDev& C, int c, int s, const double* dr, float* Uo) {
-  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1;
-  const int nc = g.nc;
-  double ph64[B];
-  basis_tail<D, ORD>(dr, g.q12, ph64);
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    float acc = 0;
-#pragma unroll
-    for (int b = 0; b < B; b++)
-      acc += (float)C.coef[(((size_t)s * U + u) * M + 1 + b) * nc + c] * (float)ph64[b];
-    Uo[u] = (float)C.coef[(((size_t)s * U + u) * M) * nc + c] + (float)C.lim[(s * U + u) * nc + c] * acc;
-  }
-}
-
 // One lane per cell c, its faces towards +x, +y(, +z).  Same-single-phase
 // faces: one HLLC per point, stored for both sides; other faces: appended
 // to the general list.  Only sides whose cell is in use are stored.
@@ -141,15 +124,37 @@ k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
 
 // The listed faces, each side as its cell sees it: for every present phase
 // ss of the side's cell, the sum over the other cell's phases s1 (alpha != 0)
-// of alpha_s1 * flux(ss, s1), with the MGFM ghost state where s1 != ss.
+// of alpha_s1 * flux(ss, s1), with the MGFM ghost state where s1 != ss.  The
+// two cells' coefficient sets are loaded once per phase into registers; the
+// sum over s1 accumulates in the output slots (this lane's own, in s1 order).
 template <int D, int ORD>
 __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const int32_t* __restrict__ used) {
-  constexpr int U = D + 2, NQ = Basis<D, ORD>::NQ;
+  constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NQ = Basis<D, ORD>::NQ;
   constexpr int NQF = D == 3 ? NQ * NQ : NQ;
   const int nc = g.nc;
+  const size_t ncz = (size_t)nc;
   const int n = *C.fcount;
   const int stride[3] = {g.N[1] * g.N[2], g.N[2], 1};
   const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
+  // the phase-s coefficient set of cell c (FP32)
+  auto load_set = [&](int c, int s, float (*cf)[M], float* lm) {
+    const double* pc = C.coef + (size_t)s * U * M * ncz + c;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      lm[u] = (float)C.lim[((size_t)s * U + u) * ncz + c];
+#pragma unroll
+      for (int m = 0; m < M; m++) cf[u][m] = (float)pc[(size_t)(u * M + m) * ncz];
+    }
+  };
+  auto eval = [&](const float (*cf)[M], const float* lm, const float* ph, float* Uo) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      float acc = 0;
+#pragma unroll
+      for (int b = 0; b < B; b++) acc += cf[u][1 + b] * ph[b];
+      Uo[u] = cf[u][0] + lm[u] * acc;
+    }
+  };
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     const int id = C.flist[t];
     const int ax = id / nc, c = id % nc;
@@ -161,30 +166,41 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
       for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
         if (ss >= g.F) break;
         if (C.ph_state[ss * nc + ci] != 1) continue;
+        float cfi[U][M], lmi[U];
+        load_set(ci, ss, cfi, lmi);
+        bool first = true;
+        for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
+          if (s1 >= g.F) break;
+          const float alph_j = (float)C.alph[s1 * nc + cj];
+          if (alph_j == 0) continue;
+          float cfj[U][M], lmj[U];
+          load_set(cj, s1, cfj, lmj);
 #pragma unroll 1
-        for (int q = 0; q < NQF; q++) {
-          double off[3], gwd;
-          face_point<D, ORD>(g, ax, q, hp, off, gwd);
-          double di[3] = {off[0], off[1], off[2]}, dj[3] = {off[0], off[1], off[2]};
-          di[ax] += side == 0 ? hp[ax] : -hp[ax];
-          dj[ax] -= side == 0 ? hp[ax] : -hp[ax];
-          float Ui[U], acc[U];
-          point_state<D, ORD>(g, C, ci, ss, di, Ui);
-#pragma unroll
-          for (int u = 0; u < U; u++) acc[u] = 0;
-          for (int s1 = 0; s1 < PGDG_MAX_PHASE; s1++) {
-            if (s1 >= g.F) break;
-            const float alph_j = (float)C.alph[s1 * nc + cj];
-            if (alph_j == 0) continue;
-            float Uj[U], Fh[U];
-            point_state<D, ORD>(g, C, cj, s1, dj, Uj);
+          for (int q = 0; q < NQF; q++) {
+            double off[3], gwd;
+            face_point<D, ORD>(g, ax, q, hp, off, gwd);
+            float di[3] = {(float)off[0], (float)off[1], (float)off[2]}, dj[3] = {di[0], di[1], di[2]};
+            di[ax] += (float)(side == 0 ? hp[ax] : -hp[ax]);
+            dj[ax] -= (float)(side == 0 ? hp[ax] : -hp[ax]);
+            float phi[B], phj[B], Ui[U], Uj[U], Fh[U];
+            basis_tail<D, ORD>(di, g.q12, phi);
+            basis_tail<D, ORD>(dj, g.q12, phj);
+            eval(cfi, lmi, phi, Ui);
+            eval(cfj, lmj, phj, Uj);
             iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh);
 #pragma unroll
-            for (int u = 0; u < U; u++) acc[u] += Fh[u] * alph_j;
+            for (int u = 0; u < U; u++) {
+              float* o = &C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, c)];
+              *o = first ? Fh[u] * alph_j : *o + Fh[u] * alph_j;
+            }
           }
-#pragma unroll
-          for (int u = 0; u < U; u++) C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, c)] = acc[u];
+          first = false;
         }
+        if (first)  // no phase of the other cell has a volume fraction
+#pragma unroll 1
+          for (int q = 0; q < NQF; q++)
+#pragma unroll
+            for (int u = 0; u < U; u++) C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, c)] = 0.f;
       }
     }
   }

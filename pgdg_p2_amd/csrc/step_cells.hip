@@ -387,6 +387,12 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
   double xc[3];
   cell_center<D>(g, i, j, k, xc);
   const int F = g.F;
+  // float form: phases absent from the cell get no limiter (nothing in the
+  // mixed step reads an absent phase's limiter); the FP64 form computes all,
+  // as the reference
+  unsigned need = 0;
+  for (int s = 0; s < PGDG_MAX_PHASE; s++)
+    if (s < F && (sizeof(T) == 8 || C.ph_state[s * nc + c] == 1)) need |= 1u << s;
   double umax[PGDG_MAX_PHASE][U], umin[PGDG_MAX_PHASE][U], pi[PGDG_MAX_PHASE][U];
   double crho[PGDG_MAX_PHASE], calph[PGDG_MAX_PHASE], crv[PGDG_MAX_PHASE][3];
   for (int s = 0; s < PGDG_MAX_PHASE; s++) {
@@ -464,6 +470,7 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
       int nb = fg.nbr;
       for (int s = 0; s < PGDG_MAX_PHASE; s++) {
         if (s >= F) break;
+        if (!((need >> s) & 1u)) continue;
         double pj[U];
 #pragma unroll
         for (int u = 0; u < U; u++) pj[u] = 0;
@@ -549,6 +556,7 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
   // ratio of the allowed range to the slope increment at neighbour centres
   for (int s = 0; s < PGDG_MAX_PHASE; s++) {
     if (s >= F) break;
+    if (!((need >> s) & 1u)) continue;
     T sl[U][M - 1];
 #pragma unroll
     for (int u = 0; u < U; u++)

@@ -321,8 +321,9 @@ typedef struct pgdg_ctx pgdg_ctx;
 
 /* Allocates the device state for `c` with room for particle_cap fluid
  * particles and n_wall wall particles (replaces the state ownership of
- * MPM::MPM / MPM::Initialize_case, S3:mpm.h:160-194).  Supported: order 1,
- * rk_stages 2, dim 2 (PGDG_SEM_T2) or 3 (PGDG_SEM_S3), 2..3 phases.       */
+ * MPM::MPM / MPM::Initialize_case, S3:mpm.h:160-194).  Supported: order 1
+ * or 2, rk_stages 2 or 3, dim 2 (PGDG_SEM_T2) or 3 (PGDG_SEM_S3), 2..3
+ * phases.                                                                 */
 int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_ctx** out);
 int pgdg_destroy(pgdg_ctx* ctx);
 
@@ -393,6 +394,46 @@ int pgdg_tuning_read(pgdg_ctx* ctx, int* blocks);
  * the same sums bit for bit; the switch exists to compare them.           */
 int pgdg_pst_staging_enable(pgdg_ctx* ctx, int on);
 int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
+
+/* ======================================================================== *
+ *  Output and diagnostics (the reference's Output_plt_file and             *
+ *  Output_point)                                                           *
+ * ======================================================================== */
+/* Neighbour-list sizes of the fluid particles (particle order), as the
+ * reference's particle->neighbor.size() after a step: the status-2 list of
+ * the last RK stage (entries, walls included, within h * 2 of the stage-start
+ * position; S3:level_infor.cpp:153-195).  n_neighbor: host [n].           */
+int pgdg_neighbor_counts(pgdg_ctx* ctx, int32_t* n_neighbor);
+/* Cells in use (Used_Ele_cen, S3:mpm.cpp:184-263): host [nc], 1 or 0.      */
+int pgdg_used_cells(pgdg_ctx* ctx, int32_t* used);
+
+/* Host-only writers (no GPU), from host state:
+ * pgdg_plt_write_particles  the particle file of MPM::Output_plt_file
+ *   (S3:mpm_ini.cpp:832-946; T2:mpm_ini.cpp:662-805): Tecplot binary
+ *   #!TDV112, variables x, y[, z], phase, vx, vy, vol, rho, P, |tv|, |v|,
+ *   n_neighbor in block order.
+ * pgdg_plt_write_cells      the cell file (S3:mpm_ini.cpp:946-978;
+ *   T2:mpm_ini.cpp:807-838): ASCII, one line per cell in i, j[, k] order.
+ * pgdg_point_diagnostics    MPM::Output_point's sums (S3:mpm_ini.cpp:774-827:
+ *   phase-2 front positions and mass; T2:mpm_ini.cpp:599-654: particle and
+ *   grid mass and energy, particles per cell in use), in the reference's
+ *   summation order; pgdg_point_append writes its lines (pointa.txt and
+ *   mass.txt for S3, point_p.txt and point_g.txt for T2) into dir,
+ *   truncating the files when first != 0.                                   */
+typedef struct pgdg_point_diag {
+  double  time;
+  double  dw, jet, ups, mass2;      /* S3                                  */
+  int64_t n_fluid;
+  double  p_mass[2], p_energy;      /* T2: particle sums                   */
+  double  g_mass[2], g_energy;      /* T2: grid sums over the cells in use */
+  double  p2g;                      /* fluid particles per cell in use     */
+} pgdg_point_diag;
+int pgdg_plt_write_particles(const char* path, const pgdg_case* c, const pgdg_particles* p,
+                             const int32_t* n_neighbor, double run_time);
+int pgdg_plt_write_cells(const char* path, const pgdg_case* c, const pgdg_cells* cl, double run_time);
+int pgdg_point_diagnostics(const pgdg_case* c, const pgdg_particles* p, const pgdg_cells* cl,
+                           const int32_t* used, double time, pgdg_point_diag* out);
+int pgdg_point_append(const char* dir, const pgdg_case* c, const pgdg_point_diag* d, int first);
 
 #ifdef __cplusplus
 }

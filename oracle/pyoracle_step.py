@@ -20,6 +20,8 @@ def declare(L: C.CDLL) -> None:
                               C.POINTER(Scalars)]
     L.oracle_step.argtypes = [C.c_void_p, C.c_int]
     L.oracle_call.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.oracle_neighbor_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    L.oracle_used_cells.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
 
 
 class OracleSim:
@@ -53,6 +55,22 @@ class OracleSim:
     def call(self, fn: str, arg: int = 0):
         rc = self.L.oracle_call(self.h, FN[fn], arg)
         assert rc == 0, rc
+
+    def neighbor_counts(self, n: int):
+        """particle->neighbor.size() of the n fluid particles, particle order."""
+        import numpy as np
+        out = np.zeros(n, dtype=np.int32)
+        rc = self.L.oracle_neighbor_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        assert rc == 0, rc
+        return out
+
+    def used_cells(self, nc: int):
+        """1 for each cell in the used-cell list, else 0 (cell order)."""
+        import numpy as np
+        out = np.zeros(nc, dtype=np.int32)
+        rc = self.L.oracle_used_cells(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        assert rc == 0, rc
+        return out
 
     def __del__(self):
         try:

@@ -154,6 +154,8 @@ struct Base {
   virtual int save(pgdg_particles*, pgdg_cells*, pgdg_scalars*) = 0;
   virtual int step() = 0;
   virtual int call(int fn, int arg) = 0;
+  virtual int neighbor_counts(int32_t* out) = 0;
+  virtual int used_cells(int32_t* out) = 0;
 };
 
 // Reference-style references to particles: >= 0 a fluid particle object
@@ -1751,6 +1753,20 @@ struct Sim : Base {
     for (int a = 0; a < D; a++) q.tv[a] += dv[a];
   }
 
+  // particle->neighbor.size() of each fluid particle, in particle order
+  // (the list the last Refresh_neighbor_info built)
+  int neighbor_counts(int32_t* out) override {
+    for (size_t i = 0; i < fluid.size(); i++) out[i] = (int32_t)pool[fluid[i]].nb.size();
+    return PGDG_OK;
+  }
+
+  // 1 for each cell in Used_Ele_cen, else 0 (cell order)
+  int used_cells(int32_t* out) override {
+    for (size_t ci = 0; ci < cells.size(); ci++) out[ci] = 0;
+    for (int ci : used) out[ci] = 1;
+    return PGDG_OK;
+  }
+
   // ---- single-stage entry for unit tests ------------------------------------
   int call(int fn, int arg) override {
     switch (fn) {
@@ -1801,6 +1817,10 @@ int oracle_step(void* h, int nsteps) {
   return 0;
 }
 int oracle_call(void* h, int fn, int arg) { return static_cast<oracle::Base*>(h)->call(fn, arg); }
+int oracle_neighbor_counts(void* h, int32_t* out) {
+  return static_cast<oracle::Base*>(h)->neighbor_counts(out);
+}
+int oracle_used_cells(void* h, int32_t* out) { return static_cast<oracle::Base*>(h)->used_cells(out); }
 void oracle_root_dim(const double* x, double* y, int64_t n, int dim) {
   for (int64_t i = 0; i < n; i++) y[i] = oracle::root_dim(dim, x[i]);
 }

@@ -26,6 +26,15 @@ def declare(L: C.CDLL) -> None:
     L.pgdg_graphs_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_autotune_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_pst_staging_enable.argtypes = [_CTX, C.c_int]
+    L.pgdg_neighbor_counts.argtypes = [_CTX, C.POINTER(C.c_int32)]
+    L.pgdg_used_cells.argtypes = [_CTX, C.POINTER(C.c_int32)]
+    from .output import PointDiag
+    L.pgdg_plt_write_particles.argtypes = [C.c_char_p, C.POINTER(Case), C.POINTER(Particles),
+                                           C.POINTER(C.c_int32), C.c_double]
+    L.pgdg_plt_write_cells.argtypes = [C.c_char_p, C.POINTER(Case), C.POINTER(Cells), C.c_double]
+    L.pgdg_point_diagnostics.argtypes = [C.POINTER(Case), C.POINTER(Particles), C.POINTER(Cells),
+                                         C.POINTER(C.c_int32), C.c_double, C.POINTER(PointDiag)]
+    L.pgdg_point_append.argtypes = [C.c_char_p, C.POINTER(Case), C.POINTER(PointDiag), C.c_int]
     L.pgdg_tuning_read.argtypes = [_CTX, C.POINTER(C.c_int)]
     L.pgdg_profile_read.argtypes = [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
     for name in _STAGES:

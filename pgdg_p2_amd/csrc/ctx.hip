@@ -906,6 +906,35 @@ int pgdg_autotune_enable(pgdg_ctx* x, int on) {
   return PGDG_OK;
 }
 
+int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
+  if (!x || !n_neighbor) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.loaded) return PGDG_ERR_STATE;
+  if (c.np == 0) return PGDG_OK;
+  PGDG_TRY(c.sync_counts());
+  int32_t* d = nullptr;
+  PGDG_HIP_TRY(hipMalloc((void**)&d, sizeof(int32_t) * (size_t)c.np));
+  int rc = c.D == 2 ? pgdg::launch_neighbor_counts<2>(c.g, c.PA.view(), c.W, c.S, c.B, c.T.n_ent, c.T.bin_off,
+                                                       c.T.bin_ent, c.sc, c.st, d)
+                    : pgdg::launch_neighbor_counts<3>(c.g, c.PA.view(), c.W, c.S, c.B, c.T.n_ent, c.T.bin_off,
+                                                       c.T.bin_ent, c.sc, c.st, d);
+  if (rc == PGDG_OK && hipMemcpyAsync(n_neighbor, d, sizeof(int32_t) * (size_t)c.np, hipMemcpyDeviceToHost,
+                                      c.st) != hipSuccess)
+    rc = PGDG_ERR_HIP;
+  if (rc == PGDG_OK && hipStreamSynchronize(c.st) != hipSuccess) rc = PGDG_ERR_HIP;
+  (void)hipFree(d);
+  return rc;
+}
+
+int pgdg_used_cells(pgdg_ctx* x, int32_t* used) {
+  if (!x || !used) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.loaded) return PGDG_ERR_STATE;
+  PGDG_HIP_TRY(hipMemcpyAsync(used, c.used, sizeof(int32_t) * (size_t)c.nc, hipMemcpyDeviceToHost, c.st));
+  PGDG_HIP_TRY(hipStreamSynchronize(c.st));
+  return PGDG_OK;
+}
+
 int pgdg_pst_staging_enable(pgdg_ctx* x, int on) {
   if (!x) return PGDG_ERR_ARG;
   x->c.pst_staging = on != 0;

@@ -79,6 +79,11 @@ template <int D>
 int launch_pst_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                       const BinData& B, int64_t n_ent, const int* bin_off, const DevScalars* sc,
                       hipStream_t st, int block);
+// Neighbour-list sizes per particle (the status-2 list of the last stage).
+template <int D>
+int launch_neighbor_counts(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                           const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
+                           const DevScalars* sc, hipStream_t st, int32_t* cnt);
 
 // Mixed-precision particle shifting: FP32 records in bin-entry order with
 // positions relative to the lower corner of the entry's bin cell.

@@ -442,6 +442,59 @@ int launch_pst_binned(const Geo& g, const PtlDev& P, const WallDev& W, const Sta
   return PGDG_OK;
 }
 
+// Neighbour-list sizes (Refresh_neighbor_info, status 2, S3:level_infor.cpp:
+// 153-195) over the bin-ordered stage-start positions: the list test of
+// k_pst_binned, counted instead of summed.  cnt is indexed by particle.
+template <int D>
+__global__ void k_count_neighbors(Geo g, PtlDev P, StageBufs S, BinData B, const int* __restrict__ bin_off,
+                                  const DevScalars* __restrict__ sc, int32_t* __restrict__ cnt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= sc->nent_dev) return;
+  const int i = B.ref[e];
+  if (i < 0) return;  // wall or removed entry
+  const int64_t nb = B.ne_cap;
+  double xbi[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) xbi[a] = B.xb[a * nb + e];
+  const double rlim = S.hb[i] * kCutOff;
+  const double r2 = rlim * rlim;
+  const double r2_in = r2 * (1.0 - 1e-12), r2_out = r2 * (1.0 + 1e-12);
+  const int pos = cell_of<D>(g, xbi);
+  int pi, pj, pk;
+  cell_ijk(g, pos, pi, pj, pk);
+  const int r = g.radius, rk = D == 3 ? r : 0;
+  int n = 0;
+  for (int t = max(pi - r, 0); t <= min(pi + r, g.N[0] - 1); t++)
+    for (int s = max(pj - r, 0); s <= min(pj + r, g.N[1] - 1); s++)
+      for (int m = max(pk - rk, 0); m <= min(pk + rk, g.N[2] - 1); m++) {
+        const int cc = (t * g.N[1] + s) * g.N[2] + m;
+        for (int f = bin_off[cc]; f < bin_off[cc + 1]; f++) {
+          double sqb = 0;
+#pragma unroll
+          for (int a = 0; a < D; a++) {
+            const double d = xbi[a] - B.xb[a * nb + f];
+            sqb += d * d;
+          }
+          if (sqb <= r2_in || (sqb <= r2_out && sqrt(sqb) <= rlim)) n++;
+        }
+      }
+  cnt[i] = n;
+}
+
+template <int D>
+int launch_neighbor_counts(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
+                           const BinData& B, int64_t n_ent, const int* bin_off, const int* bin_ent,
+                           const DevScalars* sc, hipStream_t st, int32_t* cnt) {
+  if (n_ent <= 0) return PGDG_OK;
+  // the bin-ordered stage-start positions (the records of the FP64 form)
+  hipLaunchKernelGGL(k_bin_gather<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, P, W, S, B,
+                     bin_ent, sc);
+  hipLaunchKernelGGL(k_count_neighbors<D>, dim3(grid_for(n_ent, 256, 1LL << 30)), dim3(256), 0, st, g, P,
+                     S, B, bin_off, sc, cnt);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Set_timestep reduction + Solver dt truncation
 // ---------------------------------------------------------------------------
@@ -584,6 +637,9 @@ int launch_timestep(const Geo& g, const PtlDev& P, const CellDev& C, const int32
   template int launch_stage_save<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, int,     \
                                     const DevScalars*, hipStream_t);                              \
   template int launch_in_ele<D>(const Geo&, const PtlDev&, const StageBufs&, int64_t, hipStream_t); \
+  template int launch_neighbor_counts<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&, \
+                                         const BinData&, int64_t, const int*, const int*,           \
+                                         const DevScalars*, hipStream_t, int32_t*);                 \
   template int launch_g2p<D>(const Geo&, const PtlDev&, const CellDev&, const StageBufs&, int64_t, \
                              const DevScalars*, int, bool, hipStream_t);                          \
   template int launch_pst_binned<D>(const Geo&, const PtlDev&, const WallDev&, const StageBufs&,   \

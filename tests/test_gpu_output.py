@@ -1,0 +1,62 @@
+"""GPU parity of what MPM::Output_plt_file and MPM::Output_point read from
+the run (S3:mpm_ini.cpp:774-980, T2:mpm_ini.cpp:599-840): the neighbour-list
+size of every fluid particle (the last Refresh_neighbor_info), the used-cell
+list, and the files written from the GPU state, byte for byte against the
+same files written from the oracle's state after the same steps."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle_step import OracleSim
+from pgdg_p2_amd import cases, output
+
+
+def _run(case, st, steps, label, tmp_path, mixed=False):
+    from pgdg_p2_amd.step import GpuSim
+    ref = OracleSim(case, st)
+    gpu = GpuSim(case, st)
+    a, b = st.copy(), st.copy()
+    nc = int(np.prod([case.n[x] if x < case.dim else 1 for x in range(3)]))
+    for k in range(steps):
+        print(f"{label}: step {k}", flush=True)
+        ref.step(1)
+        gpu.step(1)
+        ref.save(a)
+        gpu.save(b)
+        assert a.n == b.n
+        nb_ref = ref.neighbor_counts(a.n)
+        nb_gpu = output.neighbor_counts(gpu)
+        bad = np.flatnonzero(nb_ref != nb_gpu)
+        assert bad.size == 0, f"{label} step {k}: {bad[:8]} {nb_ref[bad[:8]]} {nb_gpu[bad[:8]]}"
+        assert nb_gpu.max() > 0
+        used_ref, used_gpu = ref.used_cells(nc), output.used_cells(gpu)
+        assert np.array_equal(used_ref, used_gpu), f"{label} step {k}"
+    # files from both states (the states are bit-identical, so the bytes are)
+    for tag, s, nb, used in (("ref", a, nb_ref, used_ref), ("gpu", b, nb_gpu, used_gpu)):
+        d = os.path.join(tmp_path, tag)
+        os.makedirs(d)
+        output.write_particles_plt(os.path.join(d, "sp.plt"), case, s, nb, s.s.run_time)
+        output.write_cells_plt(os.path.join(d, "ele.plt"), case, s, s.s.run_time)
+        output.point_append(d, case, output.point_diagnostics(case, s, used, s.s.run_time), True)
+    for f in sorted(os.listdir(os.path.join(tmp_path, "ref"))):
+        ra = open(os.path.join(tmp_path, "ref", f), "rb").read()
+        rb = open(os.path.join(tmp_path, "gpu", f), "rb").read()
+        assert ra == rb, f"{label}: {f} differs"
+    # the context's own writer (download + both files)
+    output.output_plt(gpu, st.copy(), str(tmp_path), 7)
+    assert open(os.path.join(tmp_path, "simulation_sp.7.0.plt"), "rb").read() == \
+        open(os.path.join(tmp_path, "gpu", "sp.plt"), "rb").read()
+    gpu.close()
+
+
+@pytest.mark.gpu
+def test_output_triple_point_2d(cuda, tmp_path):
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
+    _run(case, st, 4, "out-tp2d", tmp_path)
+
+
+@pytest.mark.gpu
+def test_output_shock_bubble_3d(cuda, tmp_path):
+    case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25)
+    _run(case, st, 3, "out-sb3d", tmp_path)

@@ -108,9 +108,9 @@ def test_step_parity_per_cell_bounds_lattice_1ppc(cuda):
 
 
 @pytest.mark.gpu
-def test_step_parity_cfg0_p2_rk3_100_steps(cuda):
+def test_step_parity_p2_rk3_triple_point_112x48_100_steps(cuda):
     # the plain-method workload at its full size: 112x48 P2 cells, 3x3
     # jittered particles per cell, SSP-RK3, CFL 0.2, 100 steps
     case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25,
                                      scheme=dict(order=2, rk_stages=3, cfl=0.2))
-    run_pair(case, st, 100, "cfg0-p2rk3")
+    run_pair(case, st, 100, "tp2d-112x48-p2rk3")

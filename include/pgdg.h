@@ -407,6 +407,18 @@ int pgdg_neighbor_counts(pgdg_ctx* ctx, int32_t* n_neighbor);
 /* Cells in use (Used_Ele_cen, S3:mpm.cpp:184-263): host [nc], 1 or 0.      */
 int pgdg_used_cells(pgdg_ctx* ctx, int32_t* used);
 
+/* Checkpoint / restart.  The reference has none: it creates a restart
+ * directory and never writes it (S3:Makefile:95, 115).  A checkpoint file
+ * holds the particle, wall and cell arrays, the time-loop scalars, the
+ * used-cell flags and the neighbour lists the next step's split and merge
+ * read, so that pgdg_checkpoint_read into a context created for the same
+ * case continues bit for bit as the written context would have.  The file
+ * is in the host's byte order; read rejects a file of another case
+ * (PGDG_ERR_ARG) or one that exceeds the context's capacity
+ * (PGDG_ERR_CAPACITY).                                                   */
+int pgdg_checkpoint_write(pgdg_ctx* ctx, const char* path);
+int pgdg_checkpoint_read(pgdg_ctx* ctx, const char* path);
+
 /* Host-only writers (no GPU), from host state:
  * pgdg_plt_write_particles  the particle file of MPM::Output_plt_file
  *   (S3:mpm_ini.cpp:832-946; T2:mpm_ini.cpp:662-805): Tecplot binary

@@ -235,3 +235,43 @@ def shock_bubble_3d(nx=256, ny=128, nz=128, ppc_axis=2, jitter=0.25, seed=0x5EED
     st.w["vol"][:] = wv
     _init_cells(st, case)
     return case, st
+
+
+def shock_cylinder_3d(ny=30, ppc_axis=1, jitter=0.0, seed=0x5EED0005, cap_factor=2.5, scheme=None):
+    """The reference's 3-D cylinder case (C3:mpm_ini.cpp:33-300) in its own
+    units: channel 0.35 x 0.0445 x 0.01 with p_g = 0.0445 / ny (the
+    reference: ny = 30, giving 235 x 30 x 6 cells); post-shock air
+    (rho 1.7756, u 110.62, p 156980) for x < 0.1 and at the inlet, air
+    (rho 1.29, p 1e5) elsewhere, and a helium cylinder (gamma 1.648,
+    rho 0.2347, p 1e5) of radius 0.025 along z around (0.15, 0) on the
+    yL symmetry plane.  Side tags: yL, zL and zR symmetry, yR wall, xL
+    inlet, xR outlet (C3:mpm_ini.cpp:390-545; the cylinder's zR face is
+    the symmetry tag where the spheric case has a wall)."""
+    dom = (0.350, 0.0445, 0.01)
+    p_g = dom[1] / ny
+    n = tuple(max(int(dom[a] / p_g), 1) for a in range(3))
+    bc = [BC_SYMMETRY, BC_OUTLET, BC_WALL, BC_INLET, BC_SYMMETRY, BC_SYMMETRY]
+    case = _base_case(3, 2, n, (0.0, 0.0, 0.0), dom, SEM_S3, bc, p_g, ppc_axis, scheme)
+    case.gamma[0], case.pinf[0], case.ini_rho[0] = 1.4, 0.0, 1.29
+    case.gamma[1], case.pinf[1], case.ini_rho[1] = 1.648, 0.0, 0.2347
+    case.inlet_rho, case.inlet_p = 1.7756, 156980.0
+    case.inlet_u[0], case.inlet_u[1], case.inlet_u[2] = 110.62, 0.0, 0.0
+    pos = _lattice(case, jitter, seed)
+    npart = pos.shape[1]
+    phase = np.ones(npart, dtype=np.int32)
+    rho = np.full(npart, 1.29)
+    p = np.full(npart, 100000.0)
+    vel = np.zeros((3, npart))
+    post = pos[0] < 0.1
+    rho[post], p[post], vel[0, post] = 1.7756, 156980.0, 110.62
+    he = (~post) & ~((pos[0] - 0.15) ** 2 + pos[1] ** 2 > 0.025 ** 2)
+    phase[he], rho[he] = 2, 0.2347
+    spacing = [(case.box_r[a] - case.box_l[a]) / case.n[a] for a in range(3)]
+    wx, wn, wv = _walls_nodes(case, spacing, case.p_sz * case.p_sz)
+    st = HostState(case, int(npart * cap_factor) + 64, wx.shape[1])
+    _fill_particles(st, case, pos, phase, rho, p, vel)
+    st.w["x"][:] = wx
+    st.w["norm"][:] = wn
+    st.w["vol"][:] = wv
+    _init_cells(st, case)
+    return case, st

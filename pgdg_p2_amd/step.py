@@ -57,6 +57,18 @@ class GpuSim:
         """Run the RK stages as captured HIP graphs (default) or launch by launch."""
         check(self.L.pgdg_graphs_enable(self.h, 1 if on else 0), "pgdg_graphs_enable")
 
+    def checkpoint(self, path: str):
+        """Write the context's state (arrays, scalars, used cells, carried
+        neighbour lists) to `path` (pgdg_checkpoint_write)."""
+        import os
+        check(self.L.pgdg_checkpoint_write(self.h, os.fsencode(path)), "pgdg_checkpoint_write")
+
+    def restore(self, path: str):
+        """Replace the context's state by a checkpoint of the same case
+        (pgdg_checkpoint_read); the next step continues the written run."""
+        import os
+        check(self.L.pgdg_checkpoint_read(self.h, os.fsencode(path)), "pgdg_checkpoint_read")
+
     def pst_staging(self, on: bool):
         """Mixed form, 3-D: particle-shifting candidates staged in LDS (default)
         or read by every lane from global memory (same sums bit for bit)."""

@@ -114,3 +114,78 @@ def test_step_parity_p2_rk3_triple_point_112x48_100_steps(cuda):
     case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25,
                                      scheme=dict(order=2, rk_stages=3, cfl=0.2))
     run_pair(case, st, 100, "tp2d-112x48-p2rk3")
+
+
+@pytest.mark.gpu
+def test_step_parity_shock_cylinder_3d(cuda):
+    # the reference's cylinder case (symmetry on both z sides) in its own
+    # units, coarsened to 78x10x2 cells at one particle per cell; inlet
+    # copies enter every step
+    case, st = cases.shock_cylinder_3d(ny=10, ppc_axis=1)
+    a, events = run_pair(case, st, 30, "c3d")
+    assert events > 0
+
+
+@pytest.mark.gpu
+def test_step_parity_p2_rk3_bounds_shock_cylinder_3d(cuda):
+    case, st = cases.shock_cylinder_3d(ny=10, ppc_axis=2, jitter=0.25,
+                                       scheme=dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16))
+    a, events = run_pair(case, st, 10, "c3d-p2rk3")
+    assert events > 0
+
+
+def _restart_pair(case, st, before, after, label, tmp_path):
+    # one context runs before+after steps; a second context restores the
+    # first one's checkpoint after `before` steps and runs `after` more
+    from pgdg_p2_amd.step import GpuSim
+    path = str(tmp_path / f"{label}.ckpt")
+    gpu = GpuSim(case, st)
+    print(f"{label}: {before} steps, checkpoint", flush=True)
+    gpu.step(before)
+    gpu.checkpoint(path)
+    gpu.step(after)
+    a = gpu.save(st.copy())
+    gpu.close()
+    print(f"{label}: restore, {after} steps", flush=True)
+    gpu2 = GpuSim(case, st)
+    gpu2.restore(path)
+    gpu2.step(after)
+    b = gpu2.save(st.copy())
+    gpu2.close()
+    bad = compare_states(a, b)
+    assert not bad, f"{label}: {bad[:6]}"
+    assert a.s.iterate_num == b.s.iterate_num == before + after
+    return a
+
+
+@pytest.mark.gpu
+def test_restart_continues_bit_for_bit_triple_point_lattice(cuda, tmp_path):
+    # merges start after ~20 steps: the restart falls inside the merge phase
+    case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=1, jitter=0.0)
+    _restart_pair(case, st, 25, 15, "restart-tp2d", tmp_path)
+
+
+@pytest.mark.gpu
+def test_restart_continues_bit_for_bit_mixed_p2_shock_cylinder(cuda, tmp_path):
+    case, st = cases.shock_cylinder_3d(ny=10, ppc_axis=2, jitter=0.25,
+                                       scheme=dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16))
+    case.precision = 1
+    _restart_pair(case, st, 6, 6, "restart-c3d-mixed", tmp_path)
+
+
+@pytest.mark.gpu
+def test_restart_rejects_checkpoint_of_another_case(cuda, tmp_path):
+    from pgdg_p2_amd.step import GpuSim
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
+    gpu = GpuSim(case, st)
+    gpu.step(1)
+    path = str(tmp_path / "tp.ckpt")
+    gpu.checkpoint(path)
+    gpu.close()
+    case2, st2 = cases.triple_point_2d(nx=32, ny=12, ppc_axis=3, jitter=0.25)
+    gpu2 = GpuSim(case2, st2)
+    with pytest.raises(Exception):
+        gpu2.restore(path)
+    with pytest.raises(Exception):
+        gpu2.restore(str(tmp_path / "missing.ckpt"))
+    gpu2.close()

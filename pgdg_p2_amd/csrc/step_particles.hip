@@ -58,7 +58,10 @@ template <int D, int ORD>
 __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S,
                                              const DevScalars* __restrict__ sc, int md, bool copy) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-chunked numbering: the particles of one cell sit a lattice row or
+  // plane apart in particle order, so runs of neighbouring workgroups on one
+  // XCD read each cell's coefficients into one L2
+  int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= sc->np_dev) return;
   const int64_t cap = P.cap;
   const int nc = g.nc;
@@ -607,11 +610,9 @@ int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs&
                const DevScalars* sc, int md, bool copy, hipStream_t st) {
   if (n <= 0) return PGDG_OK;
   if (g.ord == 2)
-    hipLaunchKernelGGL((k_g2p<D, 2>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
-                       md, copy);
+    hipLaunchKernelGGL((k_g2p<D, 2>), dim3(grid_xcd(n, 256)), dim3(256), 0, st, g, P, C, S, sc, md, copy);
   else
-    hipLaunchKernelGGL((k_g2p<D, 1>), dim3(grid_for(n, 256, 1LL << 30)), dim3(256), 0, st, g, P, C, S, sc,
-                       md, copy);
+    hipLaunchKernelGGL((k_g2p<D, 1>), dim3(grid_xcd(n, 256)), dim3(256), 0, st, g, P, C, S, sc, md, copy);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

@@ -402,7 +402,9 @@ int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 /* Neighbour-list sizes of the fluid particles (particle order), as the
  * reference's particle->neighbor.size() after a step: the status-2 list of
  * the last RK stage (entries, walls included, within h * 2 of the stage-start
- * position; S3:level_infor.cpp:153-195).  n_neighbor: host [n].           */
+ * position; S3:level_infor.cpp:153-195).  Zero for every particle until
+ * the first RK stage after pgdg_upload_state or pgdg_checkpoint_read (the
+ * reference's lists are empty after its set-up).  n_neighbor: host [n].  */
 int pgdg_neighbor_counts(pgdg_ctx* ctx, int32_t* n_neighbor);
 /* Cells in use (Used_Ele_cen, S3:mpm.cpp:184-263): host [nc], 1 or 0.      */
 int pgdg_used_cells(pgdg_ctx* ctx, int32_t* used);

@@ -149,6 +149,10 @@ struct Ctx {
   // mixed form: candidate entries of the particle-shifting sum staged in LDS
   // (3-D); off = every lane walks its candidates in global memory
   int pst_staging = 1;
+  // a status-2 list refresh (the stage-start save) has run since the last
+  // upload: before it every particle's list is empty, as after the
+  // reference's set-up
+  bool lists_built = false;
   template <typename F>
   int pick_block(F launch, int* best) {
     static const int cand[3] = {64, 128, 256};
@@ -453,6 +457,7 @@ int Ctx::upload(const pgdg_particles* p, const pgdg_walls* w, const pgdg_cells* 
   T.pool_used = 0;
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   loaded = true;
+  lists_built = false;
   return PGDG_OK;
 }
 
@@ -617,6 +622,7 @@ int Ctx::stage_impl(int fn, int arg) {
       return launch_in_ele<D_>(g, P, S, np, st);
     case FN_NEIGHBORS:
       if (!capturing) PGDG_TRY(sync_counts());
+      lists_built = true;
       return launch_stage_save<D_>(g, P, S, capturing ? PA.cap : np, arg, sc, st);
     case FN_P2G:
       return launch_p2g<D_>(g, C, P, T.bin_off, T.bin_ent, np + W.n, used, PB, st);
@@ -912,6 +918,10 @@ int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
   Ctx& c = x->c;
   if (!c.loaded) return PGDG_ERR_STATE;
   if (c.np == 0) return PGDG_OK;
+  if (!c.lists_built) {
+    memset(n_neighbor, 0, sizeof(int32_t) * (size_t)c.np);
+    return PGDG_OK;
+  }
   PGDG_TRY(c.sync_counts());
   int32_t* d = nullptr;
   PGDG_HIP_TRY(hipMalloc((void**)&d, sizeof(int32_t) * (size_t)c.np));

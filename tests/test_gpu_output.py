@@ -60,3 +60,30 @@ def test_output_triple_point_2d(cuda, tmp_path):
 def test_output_shock_bubble_3d(cuda, tmp_path):
     case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=2, jitter=0.25)
     _run(case, st, 3, "out-sb3d", tmp_path)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["tp2d", "c3d"])
+def test_run_case_outputs_equal_oracle(cuda, tmp_path, which):
+    # the reference's outer loop (output 0, intervals ending on the output
+    # instant, Output_point after each interval's last step) on the GPU and
+    # on the oracle: every file byte for byte
+    from pgdg_p2_amd.run import run_case
+    from pgdg_p2_amd.step import GpuSim
+    if which == "tp2d":
+        case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
+        case.output_dt = 0.0714435  # about 3.5 first-step time steps
+    else:
+        case, st = cases.shock_cylinder_3d(ny=10, ppc_axis=1)
+        case.output_dt = 5.813e-06
+    print(f"run_case {which}", flush=True)
+    ref = OracleSim(case, st)
+    t_ref = run_case(ref, st.copy(), 3, str(tmp_path / "ref"))
+    gpu = GpuSim(case, st)
+    t_gpu = run_case(gpu, st.copy(), 3, str(tmp_path / "gpu"))
+    gpu.close()
+    assert t_ref == t_gpu
+    names = sorted(os.listdir(tmp_path / "ref"))
+    assert names == sorted(os.listdir(tmp_path / "gpu")) and len(names) == 10
+    for f in names:
+        assert (tmp_path / "ref" / f).read_bytes() == (tmp_path / "gpu" / f).read_bytes(), f

@@ -193,3 +193,26 @@ def test_point_sums_and_lines(tmp_path, dim):
         b = open(os.path.join(tmp_path, "point_g.txt")).read().split("\n")
         assert a[0] == a[1] == f"{_g(0.125)} {_g(pm1)} {_g(pm2)} {_g(pe)} {_g(n / used.sum())} "
         assert b[0] == b[1] == f"{_g(0.125)} {_g(gm1)} {_g(gm2)} {_g(ge)} "
+
+
+def test_run_case_intervals_on_the_oracle(tmp_path):
+    # the outer loop on the oracle: files 0..3, one Output_point line per
+    # interval, each written at the interval's end time
+    from oracle.pyoracle_step import OracleSim
+    from pgdg_p2_amd.run import run_case
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
+    case.output_dt = 0.05
+    ref = OracleSim(case, st)
+    t = run_case(ref, st.copy(), 3, str(tmp_path))
+    assert t == (0.05 + 0.05) + 0.05
+    names = sorted(os.listdir(tmp_path))
+    assert names == sorted([f"simulation_sp.{n}.0.plt" for n in range(4)] +
+                           [f"ele_file.{n}.0.plt" for n in range(4)] + ["point_p.txt", "point_g.txt"])
+    times = [float(l.split()[0]) for l in open(os.path.join(tmp_path, "point_p.txt")).read().split("\n") if l]
+    assert times == [float(_g(0.05)), float(_g(0.1)), float(_g(0.15))]
+    for n, want in ((0, 0.0), (3, None)):
+        head = open(os.path.join(tmp_path, f"ele_file.{n}.0.plt")).read().split("\n")[1]
+        if want is not None:
+            assert head == f'ZONE T="GRID 1",SOLUTIONTIME={_g(want)}'
+        else:
+            assert head.startswith('ZONE T="GRID 1",SOLUTIONTIME=0.15')

@@ -45,7 +45,8 @@ extern "C" {
 
 int pgdg_plt_write_particles(const char* path, const pgdg_case* c, const pgdg_particles* p,
                              const int32_t* n_neighbor, double run_time) {
-  if (!path || !c || !p || !n_neighbor || p->n < 0 || (c->dim != 2 && c->dim != 3)) return PGDG_ERR_ARG;
+  if (!path || !c || !p || !n_neighbor || p->n < 0 || p->n > p->cap || (c->dim != 2 && c->dim != 3))
+    return PGDG_ERR_ARG;
   const int D = c->dim;
   std::ofstream out(path, std::ios::binary);
   if (!out) return PGDG_ERR_ARG;
@@ -157,7 +158,9 @@ int pgdg_plt_write_cells(const char* path, const pgdg_case* c, const pgdg_cells*
 
 int pgdg_point_diagnostics(const pgdg_case* c, const pgdg_particles* p, const pgdg_cells* cl,
                            const int32_t* used, double time, pgdg_point_diag* o) {
-  if (!c || !p || !cl || !used || !o || (c->dim != 2 && c->dim != 3) || c->nphase < 2) return PGDG_ERR_ARG;
+  if (!c || !p || !cl || !used || !o || p->n < 0 || p->n > p->cap || (c->dim != 2 && c->dim != 3) ||
+      c->nphase < 2)
+    return PGDG_ERR_ARG;
   const int D = c->dim, F = c->nphase;
   const int64_t cap = p->cap, n = p->n;
   const int n0 = c->n[0], n1 = c->n[1], n2 = D == 3 ? c->n[2] : 1;

@@ -216,3 +216,27 @@ def test_run_case_intervals_on_the_oracle(tmp_path):
             assert head == f'ZONE T="GRID 1",SOLUTIONTIME={_g(want)}'
         else:
             assert head.startswith('ZONE T="GRID 1",SOLUTIONTIME=0.15')
+
+
+def test_writers_reject_bad_arguments(tmp_path):
+    import ctypes as C
+    from pgdg_p2_amd import lib
+    case, st = _state(3)
+    nb = np.zeros(st.n, dtype=np.int32)
+    missing = os.path.join(tmp_path, "no_such_dir", "x.plt")
+    with pytest.raises(Exception):
+        output.write_particles_plt(missing, case, st, nb, 0.0)
+    with pytest.raises(Exception):
+        output.write_cells_plt(missing, case, st, 0.0)
+    P = st.particles()
+    P.n = P.cap + 1  # more particles than the arrays hold
+    rc = lib().pgdg_plt_write_particles(os.fsencode(os.path.join(tmp_path, "a.plt")), C.byref(case), C.byref(P),
+                                        nb.ctypes.data_as(C.POINTER(C.c_int32)), 0.0)
+    assert rc == -1
+    d = output.PointDiag()
+    used = np.ones(st.c["P"].size, dtype=np.int32)
+    rc = lib().pgdg_point_diagnostics(C.byref(case), C.byref(P), C.byref(st.cells()),
+                                      used.ctypes.data_as(C.POINTER(C.c_int32)), 0.0, C.byref(d))
+    assert rc == -1
+    with pytest.raises(Exception):
+        output.point_append(os.path.join(tmp_path, "no_such_dir"), case, d, True)

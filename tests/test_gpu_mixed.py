@@ -97,3 +97,11 @@ def test_mixed_pst_staged_equals_per_lane_walk(cuda):
     finally:
         ga.close()
         gb.close()
+
+
+@pytest.mark.gpu
+def test_mixed_step_p2_rk3_shock_cylinder_3d(cuda):
+    # the reference's cylinder case in its own units (157x20x4 cells, 8
+    # particles per cell, inlet copies every step)
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    _compare(lambda: cases.shock_cylinder_3d(ny=20, ppc_axis=2, jitter=0.25, scheme=sch), 12, "c3d-p2-mixed")

@@ -1096,6 +1096,19 @@ int pgdg_checkpoint_read(pgdg_ctx* x, const char* path) {
   }
   fclose(f);
   if (rc != PGDG_OK) return rc;
+  // values the kernels index with: phases, list extents within the pool
+  for (int64_t i = 0; i < hd.np; i++) {
+    const int32_t ph = h.P.phase[i];
+    if (ph < 1 || ph > c.F) return PGDG_ERR_ARG;
+    const int n = h.lcnt[i];
+    const int64_t o = h.loff[i];
+    if (n < 0 || (n > 0 && (o < 0 || o + n > hd.pool_used))) return PGDG_ERR_ARG;
+  }
+  // a fluid particle, a removed one (-1) or a wall (-(w + 2))
+  for (int64_t q = 0; q < hd.pool_used; q++) {
+    const int r = h.pool[q];
+    if (!((r >= -1 && r < hd.np) || (r <= -2 && -(int64_t)r - 2 < hd.nwall))) return PGDG_ERR_ARG;
+  }
   c.drop_graphs();
   PGDG_TRY(c.upload(&h.P, &h.W, &h.C, &h.S));
   PGDG_HIP_TRY(hipMemcpy(c.used, h.used.data(), sizeof(int32_t) * c.nc, hipMemcpyHostToDevice));

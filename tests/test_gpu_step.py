@@ -189,3 +189,30 @@ def test_restart_rejects_checkpoint_of_another_case(cuda, tmp_path):
     with pytest.raises(Exception):
         gpu2.restore(str(tmp_path / "missing.ckpt"))
     gpu2.close()
+
+
+@pytest.mark.gpu
+def test_restart_rejects_corrupted_checkpoint(cuda, tmp_path):
+    # a phase outside 1..F or a truncated file is refused before upload
+    import struct
+    from pgdg_p2_amd.step import GpuSim
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
+    gpu = GpuSim(case, st)
+    gpu.step(2)
+    path = tmp_path / "tp.ckpt"
+    gpu.checkpoint(str(path))
+    raw = bytearray(path.read_bytes())
+    # header: magic, 12 int32 (dim ... M, pad), then np, nwall, nc, pool_used
+    np_ = struct.unpack_from("<q", raw, 8 + 48)[0]
+    phase_at = 8 + 48 + 32 + 8 * (19 + case.U * case.M) * np_
+    assert struct.unpack_from("<i", raw, phase_at)[0] in (1, 2)
+    bad = bytearray(raw)
+    struct.pack_into("<i", bad, phase_at, 7)
+    (tmp_path / "bad.ckpt").write_bytes(bytes(bad))
+    (tmp_path / "short.ckpt").write_bytes(bytes(raw[:-4]))
+    for f in ("bad.ckpt", "short.ckpt"):
+        with pytest.raises(Exception):
+            gpu.restore(str(tmp_path / f))
+    gpu.restore(str(path))  # the intact file still loads
+    gpu.step(1)
+    gpu.close()

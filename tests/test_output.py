@@ -240,3 +240,12 @@ def test_writers_reject_bad_arguments(tmp_path):
     assert rc == -1
     with pytest.raises(Exception):
         output.point_append(os.path.join(tmp_path, "no_such_dir"), case, d, True)
+
+
+def test_run_case_needs_an_output_interval(tmp_path):
+    from oracle.pyoracle_step import OracleSim
+    from pgdg_p2_amd.run import run_case
+    case, st = cases.triple_point_2d(nx=8, ny=4, ppc_axis=1, jitter=0.0)
+    ref = OracleSim(case, st)  # output_dt left at its "no output" default
+    with pytest.raises(ValueError):
+        run_case(ref, st.copy(), 1, str(tmp_path))

@@ -156,6 +156,8 @@ struct Base {
   virtual int call(int fn, int arg) = 0;
   virtual int neighbor_counts(int32_t* out) = 0;
   virtual int used_cells(int32_t* out) = 0;
+  virtual int64_t bins_out(int32_t* off, int32_t* ent, int64_t cap) = 0;
+  virtual int list_out(int64_t i, int32_t* out, int cap) = 0;
 };
 
 // Reference-style references to particles: >= 0 a fluid particle object
@@ -1311,6 +1313,89 @@ struct Sim : Base {
     for (int b = 0; b < D; b++) DF[U - 1][b] += v[b] * P;
   }
 
+  // HLLC flux of an interior face point (S3:particle_flow.cpp:232-272):
+  // Davis wave speeds, contact speed S*, star states, four-branch select.
+  // Left state i, right state j (conserved rho, rho v, E plus P, c and v),
+  // face normal n.  Fh is the full U x D flux tensor; the caller contracts
+  // it with n.
+  static void hllc(const double* n, double rho_i, const double* rvi, double E_i, double P_i,
+                   double ci_, const double* vi, double rho_j, const double* rvj, double E_j,
+                   double P_j, double cj, const double* vj, double Fh[U][D]) {
+    double Ui[U], Uj[U];
+    Ui[0] = rho_i;
+    for (int a = 0; a < D; a++) Ui[1 + a] = rvi[a];
+    Ui[U - 1] = E_i;
+    Uj[0] = rho_j;
+    for (int a = 0; a < D; a++) Uj[1 + a] = rvj[a];
+    Uj[U - 1] = E_j;
+    double DFi[U][D], DFj[U][D];
+    flux(rho_i, rvi, E_i, vi, P_i, DFi);
+    flux(rho_j, rvj, E_j, vj, P_j, DFj);
+    double vni = 0;
+    for (int a = 0; a < D; a++) vni += vi[a] * n[a];
+    double vnj = 0;
+    for (int a = 0; a < D; a++) vnj += vj[a] * n[a];
+    double SL = AMIN1(vni - ci_, vnj - cj);
+    double SR = AMAX1(vni + ci_, vnj + cj);
+    double S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
+                 (rho_i * (SL - vni) - rho_j * (SR - vnj));
+    double rho_sL = rho_i * (SL - vni) / (SL - S_s);
+    double rho_sR = rho_j * (SR - vnj) / (SR - S_s);
+    double UsL[U], UsR[U];
+    UsL[0] = rho_sL;
+    UsR[0] = rho_sR;
+    for (int a = 0; a < D; a++) {
+      UsL[1 + a] = rho_sL * (vi[a] + (S_s - vni) * n[a]);
+      UsR[1 + a] = rho_sR * (vj[a] + (S_s - vnj) * n[a]);
+    }
+    UsL[U - 1] = rho_sL * (E_i / rho_i + (S_s - vni) * (S_s + (P_i) / (rho_i * (SL - vni))));
+    UsR[U - 1] = rho_sR * (E_j / rho_j + (S_s - vnj) * (S_s + (P_j) / (rho_j * (SR - vnj))));
+    std::memset(Fh, 0, sizeof(double) * U * D);
+    if (SL > 0) {
+      std::memcpy(Fh, DFi, sizeof(DFi));
+    } else if (SL <= 0 && S_s >= 0) {
+      for (int u = 0; u < U; u++)
+        for (int bb = 0; bb < D; bb++) Fh[u][bb] = DFi[u][bb] + SL * (UsL[u] - Ui[u]) * n[bb];
+    } else if (S_s <= 0 && SR >= 0) {
+      for (int u = 0; u < U; u++)
+        for (int bb = 0; bb < D; bb++) Fh[u][bb] = DFj[u][bb] + SR * (UsR[u] - Uj[u]) * n[bb];
+    } else if (SR <= 0) {
+      std::memcpy(Fh, DFj, sizeof(DFj));
+    }
+  }
+  // HLL flux of a boundary face point against its ghost state
+  // (S3:particle_flow.cpp:367-381).
+  static void hll(const double* n, double rho_i, const double* rvi, double E_i, double P_i,
+                  double ci_, const double* vi, double rho_j, const double* rvj, double E_j,
+                  double P_j, double cj, const double* vj, double Fh[U][D]) {
+    double Ui[U], Uj[U];
+    Ui[0] = rho_i;
+    for (int a = 0; a < D; a++) Ui[1 + a] = rvi[a];
+    Ui[U - 1] = E_i;
+    Uj[0] = rho_j;
+    for (int a = 0; a < D; a++) Uj[1 + a] = rvj[a];
+    Uj[U - 1] = E_j;
+    double DFi[U][D], DFj[U][D];
+    flux(rho_i, rvi, E_i, vi, P_i, DFi);
+    flux(rho_j, rvj, E_j, vj, P_j, DFj);
+    double vni = 0;
+    for (int a = 0; a < D; a++) vni += vi[a] * n[a];
+    double vnj = 0;
+    for (int a = 0; a < D; a++) vnj += vj[a] * n[a];
+    double SL = AMIN1(vni - ci_, vnj - cj);
+    double SR = AMAX1(vni + ci_, vnj + cj);
+    std::memset(Fh, 0, sizeof(double) * U * D);
+    if (SL > 0) {
+      std::memcpy(Fh, DFi, sizeof(DFi));
+    } else if (SL <= 0 && SR >= 0) {
+      for (int u = 0; u < U; u++)
+        for (int bb = 0; bb < D; bb++)
+          Fh[u][bb] = (SR * DFi[u][bb] - SL * DFj[u][bb] + SL * SR * (Uj[u] - Ui[u]) * n[bb]) / (SR - SL);
+    } else if (SR < 0) {
+      std::memcpy(Fh, DFj, sizeof(DFj));
+    }
+  }
+
   void grid_acceleration() {
     for (int ci : used)
       for (int s = 0; s < F; s++)
@@ -1402,12 +1487,6 @@ struct Sim : Base {
           double ci_ = std::sqrt(g * (pinf + P_i) / ti.rho);
           double vni = 0;
           for (int a = 0; a < D; a++) vni += vi[a] * n[a];
-          double Ui[U];
-          Ui[0] = ti.rho;
-          for (int a = 0; a < D; a++) Ui[1 + a] = ti.rv[a];
-          Ui[U - 1] = ti.E;
-          double DFi[U][D];
-          flux(ti.rho, ti.rv, ti.E, vi, P_i, DFi);
           bool wall_like = s3() ? (b == 1) : (b != 0);
           if (b == 0) {
             const Cell& nb = cells[fc.nbr];
@@ -1442,46 +1521,9 @@ struct Sim : Base {
                 E_j = rhoe_j + 0.5 * rho_j * sq_d(vj);
               }
               double cj = std::sqrt(g * (pinf + P_j) / rho_j);
-              double Uj[U];
-              Uj[0] = rho_j;
-              for (int a = 0; a < D; a++) Uj[1 + a] = rvj[a];
-              Uj[U - 1] = E_j;
-              double DFj[U][D];
-              flux(rho_j, rvj, E_j, vj, P_j, DFj);
-              vni = 0;
-              for (int a = 0; a < D; a++) vni += vi[a] * n[a];
-              vnj = 0;
-              for (int a = 0; a < D; a++) vnj += vj[a] * n[a];
               double alph_j = nb.alph[s1];
-              double rho_i = ti.rho;
-              double SL = AMIN1(vni - ci_, vnj - cj);
-              double SR = AMAX1(vni + ci_, vnj + cj);
-              double S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
-                           (rho_i * (SL - vni) - rho_j * (SR - vnj));
-              double rho_sL = rho_i * (SL - vni) / (SL - S_s);
-              double rho_sR = rho_j * (SR - vnj) / (SR - S_s);
-              double UsL[U], UsR[U];
-              UsL[0] = rho_sL;
-              UsR[0] = rho_sR;
-              for (int a = 0; a < D; a++) {
-                UsL[1 + a] = rho_sL * (vi[a] + (S_s - vni) * n[a]);
-                UsR[1 + a] = rho_sR * (vj[a] + (S_s - vnj) * n[a]);
-              }
-              UsL[U - 1] = rho_sL * (ti.E / rho_i + (S_s - vni) * (S_s + (P_i) / (rho_i * (SL - vni))));
-              UsR[U - 1] = rho_sR * (E_j / rho_j + (S_s - vnj) * (S_s + (P_j) / (rho_j * (SR - vnj))));
               double Fh[U][D];
-              std::memset(Fh, 0, sizeof(Fh));
-              if (SL > 0) {
-                std::memcpy(Fh, DFi, sizeof(Fh));
-              } else if (SL <= 0 && S_s >= 0) {
-                for (int u = 0; u < U; u++)
-                  for (int bb = 0; bb < D; bb++) Fh[u][bb] = DFi[u][bb] + SL * (UsL[u] - Ui[u]) * n[bb];
-              } else if (S_s <= 0 && SR >= 0) {
-                for (int u = 0; u < U; u++)
-                  for (int bb = 0; bb < D; bb++) Fh[u][bb] = DFj[u][bb] + SR * (UsR[u] - Uj[u]) * n[bb];
-              } else if (SR <= 0) {
-                std::memcpy(Fh, DFj, sizeof(Fh));
-              }
+              hllc(n, ti.rho, ti.rv, ti.E, P_i, ci_, vi, rho_j, rvj, E_j, P_j, cj, vj, Fh);
               for (int bi = 0; bi < M; bi++)
                 for (int u = 0; u < U; u++)
                   for (int bb = 0; bb < D; bb++) DFij[bi][u][bb] += Fh[u][bb] * Nb[bi] * gw * alph_j;
@@ -1512,28 +1554,9 @@ struct Sim : Base {
               vj[2] = -vi[2];
               for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
             }
-            double Uj[U];
-            Uj[0] = rho_j;
-            for (int a = 0; a < D; a++) Uj[1 + a] = rvj[a];
-            Uj[U - 1] = E_j;
             double cj = std::sqrt(g * (pinf + P_j) / rho_j);
-            double vnj = 0;
-            for (int a = 0; a < D; a++) vnj += vj[a] * n[a];
-            double DFj[U][D];
-            flux(rho_j, rvj, E_j, vj, P_j, DFj);
-            double SL = AMIN1(vni - ci_, vnj - cj);
-            double SR = AMAX1(vni + ci_, vnj + cj);
             double Fh[U][D];
-            std::memset(Fh, 0, sizeof(Fh));
-            if (SL > 0) {
-              std::memcpy(Fh, DFi, sizeof(Fh));
-            } else if (SL <= 0 && SR >= 0) {
-              for (int u = 0; u < U; u++)
-                for (int bb = 0; bb < D; bb++)
-                  Fh[u][bb] = (SR * DFi[u][bb] - SL * DFj[u][bb] + SL * SR * (Uj[u] - Ui[u]) * n[bb]) / (SR - SL);
-            } else if (SR < 0) {
-              std::memcpy(Fh, DFj, sizeof(Fh));
-            }
+            hll(n, ti.rho, ti.rv, ti.E, P_i, ci_, vi, rho_j, rvj, E_j, P_j, cj, vj, Fh);
             for (int bi = 0; bi < M; bi++)
               for (int u = 0; u < U; u++)
                 for (int bb = 0; bb < D; bb++) DFij[bi][u][bb] += Fh[u][bb] * Nb[bi] * gw;
@@ -1767,6 +1790,45 @@ struct Sim : Base {
     return PGDG_OK;
   }
 
+  // Pool id -> position in particle_fluid (-1 when not a live fluid
+  // particle), for the test views below.
+  std::vector<int> fluid_pos() const {
+    std::vector<int> pos(pool.size(), -1);
+    for (size_t i = 0; i < fluid.size(); i++) pos[fluid[i]] = (int)i;
+    return pos;
+  }
+  int map_ref(const std::vector<int>& pos, int r) const {
+    if (r >= 0) return pos[r];  // a freed object maps to -1
+    return r;                   // wall -(w+2) or NULL -1
+  }
+  // Cell lists (table_cell_list) as CSR over cells in index order; entries
+  // are fluid positions, walls -(w+2).  Returns the entry count, or -1 when
+  // cap is too small.
+  int64_t bins_out(int32_t* off, int32_t* ent, int64_t cap) override {
+    std::vector<int> pos = fluid_pos();
+    int64_t k = 0;
+    for (int c = 0; c < nc; c++) {
+      off[c] = (int32_t)k;
+      for (int r : bins[c]) {
+        if (k >= cap) return -1;
+        ent[k++] = map_ref(pos, r);
+      }
+    }
+    off[nc] = (int32_t)k;
+    return k;
+  }
+  // Neighbour list of the i-th fluid particle (particle->neighbor), entries
+  // mapped as in bins_out.  Returns the list size, or -1 when cap is too
+  // small or i is out of range.
+  int list_out(int64_t i, int32_t* out, int cap) override {
+    if (i < 0 || i >= (int64_t)fluid.size()) return -1;
+    const std::vector<int>& nb = pool[fluid[i]].nb;
+    if ((int)nb.size() > cap) return -1;
+    std::vector<int> pos = fluid_pos();
+    for (size_t t = 0; t < nb.size(); t++) out[t] = map_ref(pos, nb[t]);
+    return (int)nb.size();
+  }
+
   // ---- single-stage entry for unit tests ------------------------------------
   int call(int fn, int arg) override {
     switch (fn) {
@@ -1821,6 +1883,62 @@ int oracle_neighbor_counts(void* h, int32_t* out) {
   return static_cast<oracle::Base*>(h)->neighbor_counts(out);
 }
 int oracle_used_cells(void* h, int32_t* out) { return static_cast<oracle::Base*>(h)->used_cells(out); }
+int64_t oracle_bins(void* h, int32_t* off, int32_t* ent, int64_t cap) {
+  return static_cast<oracle::Base*>(h)->bins_out(off, ent, cap);
+}
+int oracle_list(void* h, int64_t i, int32_t* out, int cap) {
+  return static_cast<oracle::Base*>(h)->list_out(i, out, cap);
+}
+// Single routines of the restatement, for the closed-form tests.
+// Wendland kernel W and (dW/dr)/r at distance r, width h.
+int oracle_kernel(int dim, double dist, double h, double* w, double* dw) {
+  if (dim == 2) {
+    *w = oracle::Sim<2, 1>::W(dist, h);
+    *dw = oracle::Sim<2, 1>::dW(dist, h);
+  } else if (dim == 3) {
+    *w = oracle::Sim<3, 1>::W(dist, h);
+    *dw = oracle::Sim<3, 1>::dW(dist, h);
+  } else {
+    return PGDG_ERR_ARG;
+  }
+  return PGDG_OK;
+}
+double oracle_limiter_fn(double ds, int bd_fg) { return oracle::Sim<3, 1>::limiter_fn(ds, bd_fg); }
+// Face flux of two states (kind 0: HLLC of interior faces, 1: HLL of
+// boundary faces).  stL/stR: rho, rho v[3], E, P, c (7 doubles); the
+// velocity is rho v / rho.  F: U x D row-major (U = dim + 2).
+int oracle_face_flux(int dim, int kind, const double* n, const double* stL, const double* stR,
+                     double* F) {
+  double vi[3] = {0, 0, 0}, vj[3] = {0, 0, 0}, rvi[3] = {0, 0, 0}, rvj[3] = {0, 0, 0};
+  for (int a = 0; a < dim; a++) {
+    rvi[a] = stL[1 + a];
+    rvj[a] = stR[1 + a];
+    vi[a] = rvi[a] / stL[0];
+    vj[a] = rvj[a] / stR[0];
+  }
+  if (dim == 2) {
+    double Fh[4][2];
+    if (kind == 0)
+      oracle::Sim<2, 1>::hllc(n, stL[0], rvi, stL[4], stL[5], stL[6], vi, stR[0], rvj, stR[4], stR[5],
+                             stR[6], vj, Fh);
+    else
+      oracle::Sim<2, 1>::hll(n, stL[0], rvi, stL[4], stL[5], stL[6], vi, stR[0], rvj, stR[4], stR[5],
+                            stR[6], vj, Fh);
+    std::memcpy(F, Fh, sizeof(Fh));
+  } else if (dim == 3) {
+    double Fh[5][3];
+    if (kind == 0)
+      oracle::Sim<3, 1>::hllc(n, stL[0], rvi, stL[4], stL[5], stL[6], vi, stR[0], rvj, stR[4], stR[5],
+                             stR[6], vj, Fh);
+    else
+      oracle::Sim<3, 1>::hll(n, stL[0], rvi, stL[4], stL[5], stL[6], vi, stR[0], rvj, stR[4], stR[5],
+                            stR[6], vj, Fh);
+    std::memcpy(F, Fh, sizeof(Fh));
+  } else {
+    return PGDG_ERR_ARG;
+  }
+  return PGDG_OK;
+}
 void oracle_root_dim(const double* x, double* y, int64_t n, int dim) {
   for (int64_t i = 0; i < n; i++) y[i] = oracle::root_dim(dim, x[i]);
 }

@@ -216,3 +216,44 @@ def test_restart_rejects_corrupted_checkpoint(cuda, tmp_path):
     gpu.restore(str(path))  # the intact file still loads
     gpu.step(1)
     gpu.close()
+
+
+def _forced_pair(scheme, steps, label, precision=0):
+    # 3-D state with particles above the split volume (three in one cell),
+    # close pairs below the merge volume and particles past the outlet: the
+    # first steps split, merge and remove particles (S3:mpm.cpp:1036-1417,
+    # 1536-1706); every field bit-exact after every step
+    from forced_events import shock_bubble_forced
+    from pgdg_p2_amd.step import GpuSim
+    case, st = shock_bubble_forced(scheme=scheme)
+    ref = OracleSim(case, st)
+    gpu = GpuSim(case, st)
+    a, b = st.copy(), st.copy()
+    tot = dict(split=0, merge=0, outlet=0)
+    for k in range(steps):
+        print(f"{label}: step {k}", flush=True)
+        n0 = a.n
+        ref.step(1)
+        gpu.step(1)
+        ref.save(a)
+        gpu.save(b)
+        bad = compare_states(a, b)
+        assert not bad, f"{label} step {k}: {bad[:6]}"
+        s = a.s
+        assert a.n == n0 + 7 * s.n_split + s.n_inlet - s.n_outlet - s.n_merge
+        tot["split"] += s.n_split
+        tot["merge"] += s.n_merge
+        tot["outlet"] += s.n_outlet
+    gpu.close()
+    assert tot["split"] > 0 and tot["merge"] > 0 and tot["outlet"] > 0, tot
+    return tot
+
+
+@pytest.mark.gpu
+def test_step_parity_forced_split_merge_outlet_3d(cuda):
+    _forced_pair(dict(order=1, rk_stages=2), 4, "forced-p1")
+
+
+@pytest.mark.gpu
+def test_step_parity_forced_split_merge_outlet_3d_p2_rk3_bounds(cuda):
+    _forced_pair(dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16), 4, "forced-p2-bounds")

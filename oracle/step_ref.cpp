@@ -44,10 +44,11 @@
 //     U1 = U0 + dt L(U0); U2 = 3/4 U0 + 1/4 (U1 + dt L(U1));
 //     U = 1/3 U0 + 2/3 (U2 + dt L(U2)); particle positions and volumes the same.
 //   * cfl: the time-step factor (0 selects the reference 0.3).
-//   * ppc_min / ppc_max (0: off): a split is skipped when it would take its
-//     cell above ppc_max fluid particles; a particle is a merge candidate
-//     only while its cell holds more than ppc_min (counts of the bins when
-//     the split or merge pass starts).
+//   * ppc_min / ppc_max (0: off): a particle is a merge candidate only while
+//     its cell holds more than ppc_min; a split (merge) is admitted only if
+//     no cell it adds to (removes from) passes ppc_max (ppc_min), counting
+//     the bins when the pass starts plus the net changes of every earlier
+//     candidate of the pass in particle order (split_admit, merge_admit).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -722,6 +723,68 @@ struct Sim : Base {
         }
   }
 
+  // Per-cell bounds (no reference counterpart).  A candidate's effect on a
+  // cell is the net change of its fluid count; candidates are taken in
+  // particle order and each counts the additions (or removals) of every
+  // earlier candidate, admitted or not, so that the admission of one never
+  // depends on another's.
+  //
+  // Split: the 2^D children (the parent takes the last one's place) against
+  // the parent's cell; admitted if every cell it adds to stays within
+  // ppc_max.
+  bool split_admit(const double* xp, double dp, const std::vector<int>& fcnt, std::vector<int>& bud) const {
+    int cell[9], net[9], k = 0;
+    auto add = [&](int c, int d) {
+      for (int t = 0; t < k; t++)
+        if (cell[t] == c) {
+          net[t] += d;
+          return;
+        }
+      cell[k] = c;
+      net[k++] = d;
+    };
+    add(cell_index_of(xp), -1);
+    double pref[3] = {xp[0] - dp / 4., xp[1] - dp / 4., D == 3 ? xp[2] - dp / 4. : 0.};
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < (D == 3 ? 2 : 1); ++c) {
+          double x[3] = {pref[0] + dp / 2. * a, pref[1] + dp / 2. * b, 0.};
+          if (D == 3) x[2] = pref[2] + dp / 2. * c;
+          add(cell_index_of(x), 1);
+        }
+    bool ok = true;
+    for (int t = 0; t < k; t++)
+      if (net[t] > 0 && fcnt[cell[t]] + bud[cell[t]] + net[t] > cs.ppc_max) ok = false;
+    for (int t = 0; t < k; t++)
+      if (net[t] > 0) bud[cell[t]] += net[t];
+    return ok;
+  }
+  // Merge: survivor s and partner p leave their cells, the merged particle
+  // enters the cell of the volume-weighted position; admitted if every cell
+  // it removes from keeps at least ppc_min.
+  bool merge_admit(int sid, int pid, const std::vector<int>& fcnt, std::vector<int>& bud) const {
+    const Ptl& a = pool[sid];
+    const Ptl& b = pool[pid];
+    double vol = a.vol + b.vol;
+    double xn[3] = {a.x[0], a.x[1], a.x[2]};
+    for (int t = 0; t < D; t++) xn[t] = (a.x[t] * a.vol + b.x[t] * b.vol) / vol;
+    int cell[3] = {cell_index_of(a.x), cell_index_of(b.x), cell_index_of(xn)};
+    int net[3] = {-1, -1, 1};
+    for (int t = 1; t < 3; t++)
+      for (int u = 0; u < t; u++)
+        if (cell[t] == cell[u] && net[u] != 0) {
+          net[u] += net[t];
+          net[t] = 0;
+          break;
+        }
+    bool ok = true;
+    for (int t = 0; t < 3; t++)
+      if (net[t] < 0 && fcnt[cell[t]] - bud[cell[t]] + net[t] < cs.ppc_min) ok = false;
+    for (int t = 0; t < 3; t++)
+      if (net[t] < 0) bud[cell[t]] -= net[t];
+    return ok;
+  }
+
   void split_fast() {
     const int child_N = D == 3 ? 8 : 4;
     double vol_max = (D == 3 ? (cs.p_sz * cs.p_sz) * cs.p_sz : cs.p_sz * cs.p_sz) * 1.6;
@@ -741,16 +804,19 @@ struct Sim : Base {
     std::unordered_set<int> outlet_set;
     bool io = s3();
     // 1. split / inlet / outlet, single-thread order
-    std::vector<int> fcnt;
-    if (cs.ppc_max > 0) fluid_counts(fcnt);
+    std::vector<int> fcnt, bud;
+    if (cs.ppc_max > 0) {
+      fluid_counts(fcnt);
+      bud.assign(nc, 0);
+    }
     std::vector<int> snapshot = fluid;
     for (int id : snapshot) {
       pool[id].copy_p = -1;
       pool[id].mrg_mark = 0;
       double dp = root(pool[id].vol);
       bool f1 = pool[id].vol > vol_max;
-      // per-cell bound: no split that would take the cell above ppc_max
-      if (f1 && cs.ppc_max > 0) f1 = fcnt[cell_index_of(pool[id].x)] + ((1 << D) - 1) <= cs.ppc_max;
+      // per-cell bound: no split that would take a cell above ppc_max
+      if (f1 && cs.ppc_max > 0) f1 = split_admit(pool[id].x, dp, fcnt, bud);
       bool f2 = io && pool[id].x[0] - cs.box_l[0] >= dp && pool[id].mark_in == 1;
       bool f3 = io && pool[id].x[0] > cs.box_r[0];
       if (f1) {
@@ -908,8 +974,11 @@ struct Sim : Base {
     std::vector<Rec> recs;
     // per-cell bound: merge candidates only in cells holding more than
     // ppc_min fluid particles when the pass starts
-    std::vector<int> fcnt;
-    if (cs.ppc_min > 0) fluid_counts(fcnt);
+    std::vector<int> fcnt, bud;
+    if (cs.ppc_min > 0) {
+      fluid_counts(fcnt);
+      bud.assign(nc, 0);
+    }
     auto small = [&](const Ptl& q) {
       if (!(q.vol < vol_min)) return false;
       return cs.ppc_min <= 0 || fcnt[cell_index_of(q.x)] > cs.ppc_min;
@@ -939,9 +1008,13 @@ struct Sim : Base {
       }
       cp.copy_p = cp1;
     }
+    std::vector<char> seen(pool.size(), 0);  // partner of an earlier pair
     for (int id : fluid) {
       Ptl& cp = pool[id];
-      if (cp.mrg == 1 && cp.copy_p >= 0 && pool[cp.copy_p].copy_p == id && cp.mrg_mark == 0) {
+      if (cp.mrg == 1 && cp.copy_p >= 0 && pool[cp.copy_p].copy_p == id && cp.mrg_mark == 0 && !seen[id]) {
+        seen[cp.copy_p] = 1;
+        // per-cell bound: no merge that would take a cell below ppc_min
+        if (cs.ppc_min > 0 && !merge_admit(id, cp.copy_p, fcnt, bud)) continue;
         Rec rec;
         rec.s = id;
         rec.p = cp.copy_p;

@@ -9,8 +9,10 @@
 //   [neighbour lists, MLS_P2G (stage 0), Calculate_limiter,
 //    Grid_acceleration, Update_grid, Update_particle].
 // All work is enqueued on the context's stream; dt and the loop scalars stay
-// on the device.  Split and merge read a few counts back to size their
-// appended/removed particle sets.
+// on the device.  Binning, split and merge keep their counts on the device
+// too (Topo); the host reads them back once per step, after the last merge
+// pass, when the removed particles are compacted, and once more at the end
+// of the step to size the neighbour-list pool.
 #include <stdio.h>
 #include <string.h>
 
@@ -56,7 +58,7 @@ __global__ void k_used_mark(Geo g, const int32_t* ph_state, int32_t* used, int D
 // the context stream; pgdg_profile_read sums the elapsed times per slot.
 enum {
   PS_TIMESTEP = 0, PS_USED, PS_BIN, PS_SPLIT, PS_MERGE, PS_MEMBERSHIP, PS_STAGE_SAVE, PS_P2G,
-  PS_LIMITER, PS_RHS, PS_UPDATE_GRID, PS_G2P, PS_PST, PS_LISTS, PS_PST_PREP, PS_NSLOT
+  PS_LIMITER, PS_RHS, PS_UPDATE_GRID, PS_G2P, PS_PST, PS_LISTS, PS_PST_PREP, PS_COMPACT, PS_NSLOT
 };
 
 struct Profiler {
@@ -153,6 +155,9 @@ struct Ctx {
   // upload: before it every particle's list is empty, as after the
   // reference's set-up
   bool lists_built = false;
+  // the cell bins have been built since the last upload (the stages that
+  // read them refuse to run before)
+  bool bins_valid = false;
   template <typename F>
   int pick_block(F launch, int* best) {
     static const int cand[3] = {64, 128, 256};
@@ -453,11 +458,11 @@ int Ctx::upload(const pgdg_particles* p, const pgdg_walls* w, const pgdg_cells* 
   iterate_num = s->iterate_num;
   // no neighbour lists exist at upload time (reference: empty lists)
   hipLaunchKernelGGL(k_used_clear, dim3((nc + 255) / 256), dim3(256), 0, st, used, nc);
-  PGDG_HIP_TRY(hipMemsetAsync(T.lcnt, 0, sizeof(int) * PA.cap, st));
-  T.pool_used = 0;
+  PGDG_TRY(T.reset(np));
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   loaded = true;
   lists_built = false;
+  bins_valid = false;
   return PGDG_OK;
 }
 
@@ -540,7 +545,8 @@ int Ctx::set_timestep() {
 enum {
   FN_SET_TIMESTEP = 0, FN_USED = 1, FN_BIN = 2, FN_SPLIT = 3, FN_MERGE = 4, FN_MEMBERSHIP = 5,
   FN_NEIGHBORS = 6, FN_P2G = 7, FN_LIMITER = 8, FN_RHS = 9, FN_UPDATE_GRID = 10,
-  FN_UPDATE_PARTICLE = 11, FN_RUN = 12
+  FN_UPDATE_PARTICLE = 11, FN_RUN = 12,
+  FN_TOPO_FINISH = 13  // internal: compaction after the split/merge passes
 };
 static_assert(FN_SET_TIMESTEP == PGDG_FN_SET_TIMESTEP && FN_USED == PGDG_FN_USED &&
                   FN_BIN == PGDG_FN_BIN && FN_SPLIT == PGDG_FN_SPLIT && FN_MERGE == PGDG_FN_MERGE &&
@@ -559,6 +565,9 @@ constexpr int kCopyLater = 0x100;
 // ... and kBinsKept for the stages after the first: the bins (and the
 // mixed form's entry-to-cell map built from them) are those of the first.
 constexpr int kBinsKept = 0x400;
+// Split/merge argument inside a time step: leave the removed particles for
+// the compaction after the last merge pass.
+constexpr int kDefer = 0x800;
 // RHS argument inside a time step: kFuseRk | stage code (see FN_RHS).
 constexpr int kFuseRk = 0x200;
 static int g2p_code(int md) { return ((md == 0 || md == 2) ? (md | kCopyLater) : md) | (md != 0 ? kBinsKept : 0); }
@@ -576,6 +585,7 @@ static int prof_slot(int fn) {
     case 8: return PS_LIMITER;
     case 9: return PS_RHS;
     case 10: return PS_UPDATE_GRID;
+    case 13: return PS_COMPACT;
     default: return -1;
   }
 }
@@ -603,19 +613,23 @@ int Ctx::stage_impl(int fn, int arg) {
       PGDG_HIP_TRY(hipGetLastError());
       return PGDG_OK;
     case FN_BIN:
-      return T.build_bins<D_>(g, P, W, np);
-    case FN_SPLIT: {
-      int64_t cnt3[3];
-      PGDG_TRY(T.split<D_>(g, PA, W, &np, cnt3));
-      n_split = cnt3[0];
-      n_inlet = cnt3[1];
-      n_outlet = cnt3[2];
+      PGDG_TRY(T.build_bins<D_>(g, P, W, np));
+      bins_valid = true;
       return PGDG_OK;
-    }
-    case FN_MERGE: {
-      int64_t m = 0;
-      PGDG_TRY(T.merge<D_>(g, PA, W, &np, &m));
-      n_merge += m;
+    case FN_SPLIT:
+      if (!bins_valid) return PGDG_ERR_STATE;
+      PGDG_TRY(T.split<D_>(g, PA, W));
+      return (arg & kDefer) ? PGDG_OK : stage_impl<D_>(FN_TOPO_FINISH, 0);
+    case FN_MERGE:
+      if (!bins_valid) return PGDG_ERR_STATE;
+      PGDG_TRY(T.merge<D_>(g, PA, W));
+      return (arg & kDefer) ? PGDG_OK : stage_impl<D_>(FN_TOPO_FINISH, 0);
+    case FN_TOPO_FINISH: {
+      PGDG_TRY(T.finish(PA, &np));
+      n_split = T.hc.n_split;
+      n_inlet = T.hc.n_inlet;
+      n_outlet = T.hc.n_outlet;
+      n_merge = T.hc.n_merge;
       return PGDG_OK;
     }
     case FN_MEMBERSHIP:
@@ -625,6 +639,7 @@ int Ctx::stage_impl(int fn, int arg) {
       lists_built = true;
       return launch_stage_save<D_>(g, P, S, capturing ? PA.cap : np, arg, sc, st);
     case FN_P2G:
+      if (!bins_valid) return PGDG_ERR_STATE;
       return launch_p2g<D_>(g, C, P, T.bin_off, T.bin_ent, np + W.n, used, PB, st);
     case FN_LIMITER:
       if (tuning_now)
@@ -652,6 +667,7 @@ int Ctx::stage_impl(int fn, int arg) {
       return launch_cells<D_>(CELL_UPDATE, g, C, P, T.bin_off, T.bin_ent, used, sc, ua, st);
     }
     case FN_UPDATE_PARTICLE: {
+      if (!bins_valid) return PGDG_ERR_STATE;
       if (!capturing) PGDG_TRY(sync_counts());
       const int64_t n_p = capturing ? PA.cap : np, n_e = capturing ? B.ne_cap : T.n_ent;
       Profiler::Rec r;
@@ -704,10 +720,12 @@ int Ctx::stage_impl(int fn, int arg) {
 template <int D_>
 int Ctx::run_simulation(int64_t n) {
   n_merge = 0;
+  PGDG_TRY(T.reset_step_counts());
   if (n == 1) PGDG_TRY(stage<D_>(FN_USED, 0));
   PGDG_TRY(stage<D_>(FN_BIN, 0));
-  PGDG_TRY(stage<D_>(FN_SPLIT, 0));
-  for (int it = 0; it < 4; it++) PGDG_TRY(stage<D_>(FN_MERGE, 0));
+  PGDG_TRY(stage<D_>(FN_SPLIT, kDefer));
+  for (int it = 0; it < 4; it++) PGDG_TRY(stage<D_>(FN_MERGE, kDefer));
+  PGDG_TRY(stage<D_>(FN_TOPO_FINISH, 0));
   PGDG_TRY(stage<D_>(FN_MEMBERSHIP, 0));
   // stage codes of the cell and particle updates: Heun (reference) 0, 1;
   // SSP-RK3 0, 2, 3
@@ -758,7 +776,7 @@ int Ctx::run_simulation(int64_t n) {
 template <int D_>
 int Ctx::rk_stage(int md) {
   const void* k1 = PA.view().x;
-  const void* k2 = T.bin_ent;
+  const void* k2 = T.bin_ent;  // fixed; kept in the key for clarity
   for (int q = 0; q < ngraphs; q++)
     if (graphs[q].md == md && graphs[q].k1 == k1 && graphs[q].k2 == k2) {
       PGDG_HIP_TRY(hipGraphLaunch(graphs[q].exec, st));
@@ -876,6 +894,8 @@ int pgdg_stage(pgdg_ctx* x, int fn, int arg) {
   if ((fn == pgdg::FN_UPDATE_GRID || fn == pgdg::FN_UPDATE_PARTICLE) && (arg < 0 || arg > 3))
     return PGDG_ERR_ARG;  // stage codes 0-3
   if (fn == pgdg::FN_RHS && arg != 0) return PGDG_ERR_ARG;
+  if ((fn == pgdg::FN_SPLIT || fn == pgdg::FN_MERGE) && arg != 0) return PGDG_ERR_ARG;
+  if (fn < 0 || fn > pgdg::FN_RUN) return PGDG_ERR_ARG;
   return x->c.D == 2 ? x->c.stage<2>(fn, arg) : x->c.stage<3>(fn, arg);
 }
 
@@ -922,6 +942,7 @@ int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
     memset(n_neighbor, 0, sizeof(int32_t) * (size_t)c.np);
     return PGDG_OK;
   }
+  if (!c.bins_valid) return PGDG_ERR_STATE;
   PGDG_TRY(c.sync_counts());
   int32_t* d = nullptr;
   PGDG_HIP_TRY(hipMalloc((void**)&d, sizeof(int32_t) * (size_t)c.np));
@@ -1118,7 +1139,8 @@ int pgdg_checkpoint_read(pgdg_ctx* x, const char* path) {
   }
   if (hd.pool_used > 0)
     PGDG_HIP_TRY(hipMemcpy(c.T.pool, h.pool.data(), sizeof(int) * hd.pool_used, hipMemcpyHostToDevice));
-  c.T.pool_used = hd.pool_used;
+  PGDG_TRY(c.T.set_pool_used(hd.pool_used));
+  PGDG_HIP_TRY(hipStreamSynchronize(c.st));
   return PGDG_OK;
 }
 

@@ -2,6 +2,7 @@
 // neighbour-list pool, split/merge scratch).
 #pragma once
 #include "dev_common.hpp"
+#include "dev_scan.hpp"
 #include "hip_util.hpp"
 
 namespace pgdg {
@@ -61,67 +62,89 @@ struct ParticleArrays {
   int gather(const int* newidx, int64_t n, hipStream_t st);
 };
 
+// Counters of the topology passes, kept on the device so that binning,
+// split and merge run without host round trips.  Particles removed by a
+// pass (outlet, merge partners) stay in the arrays, flagged dead, until
+// Topo::finish compacts them once after the last pass.
+struct TopoCounters {
+  long long n_total;    // particles in the arrays, live and dead
+  long long n_dead;     // removed, not yet compacted
+  long long nent;       // bin entries
+  long long pool_used;  // neighbour-list pool entries in use
+  long long pool_cap;
+  long long n_split, n_inlet, n_outlet, n_merge;  // event counts of the step
+  long long tot_split, tot_inlet, tot_outlet;     // this split pass
+  long long n_pairs;    // merge pairs of this pass
+  long long k_app;      // bin entries appended by this pass
+  long long active;     // this pass changes particles or bins
+  long long win_len;    // nc when active, else 0
+  long long lists_total, list_base, list_ok;
+  long long n_contrib;  // per-cell bound contributions of this pass
+  long long err;        // PGDG_* error code of the passes since the last finish
+};
+
 struct Topo {
   int64_t cap = 0, nwall = 0;
   int nc = 0;
   hipStream_t stream = nullptr;
-  int key_bits = 1;
+  TopoCounters* tc = nullptr;  // device
+  TopoCounters hc{};           // host copy (read by finish)
   // cell bins (CSR)
   int* bin_off = nullptr;
   int* bin_ent = nullptr;
-  int* bin_ent2 = nullptr;
+  int *off_tmp = nullptr, *ent_tmp = nullptr;
   int64_t n_ent = 0;
-  uint32_t *keys = nullptr, *keys2 = nullptr;
-  int* vals = nullptr;
-  int *cnt = nullptr, *cnt2 = nullptr, *off2 = nullptr;
+  uint32_t* keys = nullptr;  // cell of every bin entry being sorted
+  int* order = nullptr;      // bucket-sort output
+  int *cursor = nullptr, *cnt = nullptr;
   int* fcnt = nullptr;  // fluid entries per cell (per-cell split/merge bounds)
   uint8_t* win = nullptr;
   // per-particle scratch
-  uint8_t *rm = nullptr, *removed = nullptr, *src = nullptr, *aff = nullptr, *need_ = nullptr,
-          *type = nullptr;
-  int *ia = nullptr, *ib = nullptr, *ic = nullptr, *id = nullptr;
-  int *sa = nullptr, *sb = nullptr, *sc_ = nullptr, *sd = nullptr;
-  int *app_ref = nullptr, *app_ref2 = nullptr;
-  uint32_t *app_cell = nullptr, *app_cell2 = nullptr;
+  uint8_t *rm = nullptr, *dead = nullptr, *src = nullptr, *aff = nullptr, *need_ = nullptr,
+          *type = nullptr, *reject = nullptr;
+  int *is_a = nullptr, *is_b = nullptr, *is_c = nullptr;  // 0/1 indicators, then their scans
+  int* cand = nullptr;
+  int* app_ref = nullptr;
+  uint32_t* app_cell = nullptr;
+  // per-cell bound contributions (cell, candidate, cost), bucketed by cell
+  uint32_t* ckey = nullptr;
+  int *ccand = nullptr, *ccost = nullptr, *cbase = nullptr, *coff = nullptr;
+  int64_t ccap = 0;
   // neighbour-list pool
-  int *lcnt = nullptr, *lcnt2 = nullptr;
-  int64_t *loff = nullptr, *loff2 = nullptr, *lscan = nullptr, *lwide = nullptr;
+  int *lcnt = nullptr, *lcnt2 = nullptr, *lc32 = nullptr;
+  int64_t *loff = nullptr, *loff2 = nullptr;
+  long long* lscan = nullptr;
   int* pool = nullptr;
   int64_t pool_cap = 0, pool_used = 0;
   int* newidx = nullptr;
-  int64_t* host_small = nullptr;
-  void* tmp = nullptr;
-  size_t tmp_bytes = 0;
+  int* part32 = nullptr;
+  long long* part64 = nullptr;
   size_t bytes_total = 0;
 
   int init(int64_t cap_, int64_t nwall_, int nc_, hipStream_t st);
   void release();
-  int scan_i32(const int* in, int* out, int64_t n);
-  int scan_total(const int* in, int* out, int64_t n, int64_t* total);
+  // counters after an upload: n live particles, no lists
+  int reset(int64_t n);
+  int set_pool_used(int64_t used);
   template <int D>
   int build_bins(const Geo& g, const PtlDev& P, const WallDev& W, int64_t n);
-  int edit_bins(int64_t k_app);
+  int edit_bins();
   template <int D>
   int build_lists(const Geo& g, const double* xq, const double* hq, const double* xall,
-                  const WallDev& W, int64_t n, int status, bool reset_pool);
-  int compact(ParticleArrays& PA, int64_t n, int64_t* n_out);
+                  const WallDev& W, int status);
   template <int D>
-  int mark_in(const Geo& g, const PtlDev& P, int64_t n);
+  int mark_in(const Geo& g, const PtlDev& P);
   template <int D>
-  int split(const Geo& g, ParticleArrays& PA, const WallDev& W, int64_t* n_io, int64_t counts[3]);
+  int split(const Geo& g, ParticleArrays& PA, const WallDev& W);
   template <int D>
-  int merge(const Geo& g, ParticleArrays& PA, const WallDev& W, int64_t* n_io, int64_t* merged);
+  int merge(const Geo& g, ParticleArrays& PA, const WallDev& W);
+  // Waits for the passes, reports their error, compacts the dead particles
+  // and returns the live count and the step's event counts.
+  int finish(ParticleArrays& PA, int64_t* n_out);
+  int reset_step_counts();
   template <int D>
   int end_of_step_lists(const Geo& g, const PtlDev& P, const double* xb, const double* hb,
                         const WallDev& W, int64_t n);
 };
-
-// small helpers used by step_topo.hip
-__global__ void k_widen(const int* in, int64_t n, int64_t* out);
-__global__ void k_keep(const uint8_t* removed, int64_t n, int* keep);
-__global__ void k_split_src(const uint8_t* type, int64_t n, uint8_t* src);
-__global__ void k_mark_refs(const int* refs, int64_t k, uint8_t* flag);
-template <int D>
-__global__ void k_flag_lists(Geo g, PtlDev P, int64_t n, uint8_t* need);
 
 }  // namespace pgdg

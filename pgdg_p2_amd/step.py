@@ -13,7 +13,7 @@ FN = {"set_timestep": 0, "used_state_update": 1, "bin": 2, "split": 3, "merge": 
 
 PROFILE_SLOTS = ["set_timestep", "used_state_update", "bin", "split", "merge", "membership",
                  "stage_save", "p2g", "limiter", "rhs", "update_grid", "g2p", "pst", "lists",
-                 "pst_prep"]
+                 "pst_prep", "compact"]
 
 
 class GpuSim:

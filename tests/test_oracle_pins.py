@@ -584,3 +584,47 @@ def test_step_count_identity_with_forced_events(scheme):
         tot["merge"] += s.n_merge
         tot["outlet"] += s.n_outlet
     assert tot["split"] > 0 and tot["merge"] > 0 and tot["outlet"] > 0, tot
+
+
+# ---------------------------------------------------------------------------
+# per-cell split/merge bounds (no reference routine)
+# ---------------------------------------------------------------------------
+def _fluid_counts(o, nc, cap):
+    off, ent = o.bins(nc, cap)
+    fl = (ent >= 0).astype(np.int64)
+    c = np.zeros(nc, dtype=np.int64)
+    cell = np.repeat(np.arange(nc), np.diff(off))
+    np.add.at(c, cell, fl)
+    return c
+
+
+@pytest.mark.parametrize("lo,hi", [(4, 16), (8, 12)])
+def test_per_cell_bounds_hold_after_every_pass(lo, hi):
+    from forced_events import shock_bubble_forced
+    res = {}
+    for bounded in (False, True):
+        sch = dict(order=2, rk_stages=3, ppc_min=lo if bounded else 0, ppc_max=hi if bounded else 0)
+        case, st = shock_bubble_forced(scheme=sch)
+        o = OracleSim(case, st)
+        cap = st.cap + st.nwall
+        o.call("bin")
+        before = _fluid_counts(o, case.nc, cap)
+        o.call("split")
+        out = st.copy()
+        o.save(out)
+        after = _fluid_counts(o, case.nc, cap)
+        assert out.s.n_inlet == 0
+        if bounded:  # no split takes a cell above ppc_max
+            assert np.all(after <= np.maximum(hi, before))
+        merges = 0
+        for p in range(4):
+            before = after
+            o.call("merge")
+            after = _fluid_counts(o, case.nc, cap)
+            o.save(out)
+            if bounded:  # no merge takes a cell below ppc_min
+                assert np.all(after >= np.minimum(lo, before))
+        res[bounded] = (out.s.n_split, out.s.n_merge)
+    # the bounds refuse some splits and merges of this state
+    assert 0 < res[True][0] < res[False][0], res
+    assert 0 < res[True][1] < res[False][1], res

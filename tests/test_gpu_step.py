@@ -257,3 +257,59 @@ def test_step_parity_forced_split_merge_outlet_3d(cuda):
 @pytest.mark.gpu
 def test_step_parity_forced_split_merge_outlet_3d_p2_rk3_bounds(cuda):
     _forced_pair(dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16), 4, "forced-p2-bounds")
+
+
+def _cell_counts(case, st):
+    from test_oracle_pins import _cells_of
+    c = np.zeros(case.nc, dtype=np.int64)
+    np.add.at(c, _cells_of(case, st.p["x"][:, :st.n]), 1)
+    return c
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lo,hi", [(4, 16), (8, 12)])
+def test_per_cell_bounds_hold_after_every_pass_gpu(cuda, lo, hi):
+    # stage by stage (bin, split, merge x4) against the oracle; after the
+    # split no cell is above max(ppc_max, its count before), after each merge
+    # pass none below min(ppc_min, its count before)
+    from forced_events import shock_bubble_forced
+    from pgdg_p2_amd.step import GpuSim
+    case, st = shock_bubble_forced(scheme=dict(order=2, rk_stages=3, ppc_min=lo, ppc_max=hi))
+    ref = OracleSim(case, st)
+    gpu = GpuSim(case, st)
+    a, b = st.copy(), st.copy()
+    before = _cell_counts(case, st)
+    for name in ["bin", "split", "merge", "merge", "merge", "merge"]:
+        print(f"bounds {lo},{hi}: {name}", flush=True)
+        ref.call(name)
+        gpu.call(name)
+        ref.save(a)
+        gpu.save(b)
+        bad = compare_states(a, b, cells=False)
+        assert not bad, f"{name}: {bad[:6]}"
+        after = _cell_counts(case, b)
+        if name == "split":
+            assert b.s.n_split > 0
+            assert np.all(after <= np.maximum(hi, before))
+        elif name == "merge":
+            assert np.all(after >= np.minimum(lo, before))
+        before = after
+    assert b.s.n_merge > 0
+    gpu.close()
+
+
+@pytest.mark.gpu
+def test_stages_reading_bins_refuse_before_binning(cuda):
+    # after an upload the cell bins do not exist yet: split, merge, P2G and
+    # the particle update return PGDG_ERR_STATE instead of reading them
+    from pgdg_p2_amd.step import FN, GpuSim
+    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
+    gpu = GpuSim(case, st)
+    for name in ("split", "merge", "p2g", "update_particle"):
+        assert gpu.L.pgdg_stage(gpu.h, FN[name], 0) == -5, name
+    gpu.call("bin")
+    gpu.call("split")
+    gpu.load(st)  # a new upload invalidates the bins again
+    assert gpu.L.pgdg_stage(gpu.h, FN["merge"], 0) == -5
+    gpu.step(1)
+    gpu.close()

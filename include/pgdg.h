@@ -7,8 +7,8 @@
  * T2 = 2D_cases/Triple_point_2phase).  Each entry point below replaces one of
  * those member functions; the comment on each names the reference routine
  * (file:line) it stands in for.  The C++ host class `pgdg::MPM`
- * (pgdg_p2_amd/csrc/mpm_host.hpp) keeps the reference method names and
- * forwards to these functions.
+ * (include/pgdg_mpm.hpp) keeps the reference method names and argument
+ * lists and forwards to these functions.
  *
  * Conventions
  *   - every function returns an int status: PGDG_OK (0) or a negative code;
@@ -264,13 +264,23 @@ typedef struct pgdg_case {
   double max_v0;         /* initial max_v (reference 1e-7)                  */
   int    precision;      /* PGDG_PREC_FP64: the reference's FP64 arithmetic
                             and order (bit-identical to the CPU restatement);
-                            PGDG_PREC_MIXED: particle shifting evaluated in
-                            FP32 in cell-local coordinates with FP64 sums   */
+                            PGDG_PREC_MIXED: FP32 DG quadrature and fluxes
+                            (MGFM Newton, mass division and the RK update in
+                            FP64), FP32 limiter slope ratios and positivity
+                            probe, FP32 P2G kernel weights, FP32 particle-
+                            shifting pair terms in cell-local coordinates;
+                            the state, its sums and the topology stay FP64 */
   double cfl;            /* time-step factor; 0 selects the reference 0.3
                             (S3:particle.cpp:560)                          */
   int    ppc_min, ppc_max; /* per-cell particle-count bounds of split and
                             merge (0, 0: the reference's volume criteria
-                            only)                                          */
+                            only).  A split (merge) is admitted only if no
+                            cell it adds particles to (removes them from)
+                            passes ppc_max (ppc_min), counting the cell's
+                            fluid particles when the pass starts plus the
+                            net changes of every earlier candidate of the
+                            pass, in particle order; merge candidates also
+                            need a cell holding more than ppc_min          */
 } pgdg_case;
 
 /* Host SoA views.  Arrays are sized by capacity `cap` (particles) or by the

@@ -80,6 +80,7 @@ class OracleSim:
         self.load(st)
 
     def load(self, st: HostState):
+        self._cap = st.cap
         P, W, Cc = st.particles(), st.walls(), st.cells()
         rc = self.L.oracle_load(self.h, C.byref(P), C.byref(W), C.byref(Cc), C.byref(st.s))
         assert rc == 0, rc
@@ -90,6 +91,13 @@ class OracleSim:
         assert rc == 0, rc
         st.n = P.n
         return st
+
+    def scalars(self):
+        """The time-loop scalars (the state is saved into scratch arrays)."""
+        from pgdg_p2_amd.state import HostState
+        st = HostState(self.case, self._cap, 0)
+        self.save(st)
+        return st.s
 
     def step(self, n=1):
         rc = self.L.oracle_step(self.h, n)

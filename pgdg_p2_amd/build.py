@@ -69,6 +69,18 @@ def build_product(jobs: int = 8) -> Path:
     return LIB
 
 
+HOST_TEST_SRC = ROOT / "tests" / "cpp" / "test_mpm_host.cpp"
+HOST_TEST_BIN = ROOT / "tests" / "cpp" / "mpm_host_test"
+
+
+def build_host_test() -> Path:
+    """The C++ program that drives the step through include/pgdg_mpm.hpp
+    (host code only: g++ against libpgdg.so)."""
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(HOST_TEST_SRC),
+          f"-L{PKG}", "-lpgdg", "-Wl,-rpath,$ORIGIN/../../pgdg_p2_amd", "-o", str(HOST_TEST_BIN)])
+    return HOST_TEST_BIN
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ORACLE), "-j8"])
     return ORACLE / "liboracle.so"
@@ -77,6 +89,7 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_oracle()
     build_product()
+    build_host_test()
 
 
 if __name__ == "__main__":

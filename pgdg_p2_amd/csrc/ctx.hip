@@ -711,6 +711,10 @@ int Ctx::stage_impl(int fn, int arg) {
       return PGDG_OK;
     }
     case FN_RUN:
+      // MPM_run_simulation(n, ...) receives the caller's iteration number
+      // (the Solver increments mpm.iterate_num first, S3:solver.cpp:51-53);
+      // the next Set_timestep reads it
+      iterate_num = arg;
       return run_simulation<D_>(arg);
     default:
       return PGDG_ERR_ARG;

@@ -21,19 +21,14 @@ from .state import HostState
 
 
 def _counts(sim, st: HostState):
-    """Neighbour-list sizes and used-cell flags of a GPU context or of the
-    oracle (the same accessors, for parity runs)."""
+    """Neighbour-list sizes and used-cell flags (any driver with the
+    neighbor_counts / used_cells accessors of pgdg_p2_amd.step.GpuSim)."""
     nc = int(np.prod([sim.case.n[a] if a < sim.case.dim else 1 for a in range(3)]))
-    if hasattr(sim, "neighbor_counts"):  # the oracle
-        return sim.neighbor_counts(st.n), sim.used_cells(nc)
-    return output.neighbor_counts(sim), output.used_cells(sim)
+    return sim.neighbor_counts(st.n), sim.used_cells(nc)
 
 
 def _scalars(sim, st: HostState):
-    if hasattr(sim, "scalars"):
-        return sim.scalars()
-    sim.save(st)
-    return st.s
+    return sim.scalars()
 
 
 def _plt(sim, st: HostState, directory: str, n: int, rank: int, run_time: float):

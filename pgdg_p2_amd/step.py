@@ -111,6 +111,23 @@ class GpuSim:
         check(self.L.pgdg_memory_bytes(self.h, C.byref(b)))
         return b.value
 
+    def neighbor_counts(self, n: int | None = None):
+        """particle->neighbor.size() of every fluid particle (particle order)."""
+        import numpy as np
+        m = self.count()
+        out = np.zeros(max(m, 1), dtype=np.int32)
+        check(self.L.pgdg_neighbor_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))),
+              "pgdg_neighbor_counts")
+        return out[:m]
+
+    def used_cells(self, nc: int | None = None):
+        """1 for each cell in use (Used_Ele_cen), else 0 (cell order)."""
+        import numpy as np
+        out = np.zeros(self.case.nc, dtype=np.int32)
+        check(self.L.pgdg_used_cells(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))),
+              "pgdg_used_cells")
+        return out
+
     def count(self) -> int:
         n = C.c_int64()
         check(self.L.pgdg_particle_count(self.h, C.byref(n)))
@@ -126,22 +143,3 @@ class GpuSim:
             self.close()
         except Exception:
             pass
-
-
-def smoke_step(device=None) -> None:
-    """Run two steps of a small 2-D triple point on the GPU and check them
-    bit for bit against the oracle."""
-    from oracle.pyoracle_step import OracleSim
-    from . import cases
-    from .compare import compare_states
-    case, st = cases.triple_point_2d(nx=28, ny=12, ppc_axis=3, jitter=0.25)
-    ref = OracleSim(case, st)
-    gpu = GpuSim(case, st)
-    ref.step(2)
-    gpu.step(2)
-    a = ref.save(st.copy())
-    b = gpu.save(st.copy())
-    gpu.close()
-    bad = compare_states(a, b)
-    if bad:
-        raise AssertionError(f"GPU step differs from the oracle: {bad[:5]}")

@@ -313,3 +313,32 @@ def test_stages_reading_bins_refuse_before_binning(cuda):
     assert gpu.L.pgdg_stage(gpu.h, FN["merge"], 0) == -5
     gpu.step(1)
     gpu.close()
+
+
+@pytest.mark.gpu
+def test_step_parity_three_phase_p2_rk3_112x48_with_split_merge(cuda):
+    # the three-material triple point (phases 1/2/3: left gamma 1.5,
+    # bottom-right 1.4, top-right 1.67) with P2, SSP-RK3, CFL 0.2, 3x3
+    # jittered particles per cell and per-cell bounds [1, 27]: splits and
+    # merges start after ~25 steps; every step bit-exact
+    case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, nphase=3,
+                                     scheme=dict(order=2, rk_stages=3, cfl=0.2, ppc_min=1,
+                                                 ppc_max=27))
+    from pgdg_p2_amd.step import GpuSim
+    ref = OracleSim(case, st)
+    gpu = GpuSim(case, st)
+    a, b = st.copy(), st.copy()
+    splits = merges = 0
+    for k in range(60):
+        if k % 10 == 0:
+            print(f"tp3-p2rk3: step {k}", flush=True)
+        ref.step(1)
+        gpu.step(1)
+        ref.save(a)
+        gpu.save(b)
+        bad = compare_states(a, b)
+        assert not bad, f"step {k}: {bad[:6]}"
+        splits += a.s.n_split
+        merges += a.s.n_merge
+    gpu.close()
+    assert splits > 50 and merges > 100, (splits, merges)

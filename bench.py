@@ -6,8 +6,12 @@ seeded jitter), air with a helium sphere hit by a Mach 1.22 shock; SSP-RK3
 stages, split/merge with per-cell bounds [4, 16].  Plain FP64 and mixed forms
 are both run; `value` is the mixed form's whole-job cell-steps/s with the state
 resident in HBM.  `--scheme p1` runs the reference's P1 basis and two-stage
-step instead.  Secondary measurements: the batched MGFM solve on 1e7
-interface face pairs (plain and learned initial guess) and the MLS fit.
+step instead.  Secondary measurements: the two forms side by side for 200
+steps on a reduced grid (density error horizon), the 2-D triple point at
+112x48 (plain-method workload), the 2-D three-phase triple point at
+1792x768 after 500 steps of split/merge, the batched MGFM solve on 1e7
+interface face pairs (plain and learned initial guess), the MLS fit, and the
+CPU oracle on a bounded sample (all host worker threads and one thread).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size full|medium|small]
                        [--scheme p2rk3|p1]
@@ -67,6 +71,37 @@ def stage_bytes(stage: str, case, n: int, nwall: int) -> float:
     return 0.0
 
 
+def sec8d_cell_step_bytes(D: int, U: int, M: int, F: int, S: int, ppc: float) -> float:
+    """Algorithmic HBM bytes per cell-step of the whole step (plain FP64
+    state, particle coefficient copy by reference, one present phase):
+    B = S*B_cs + ppc*(S*B_ps + B_pstep)."""
+    b_cs = 8 * (U * M + F + U) + 8 * (3 * U * M + U + F + D * D)
+    b_ps = (8 * (2 * D + 1) + 8) + 8 * (D + 1 + U) + (8 * (2 * D + 3) + 4) + 8 * D
+    b_pstep = (8 * (D + 1) + 4) + (8 * D + 8) + (8 * (D + 1) + 4)
+    return S * b_cs + ppc * (S * b_ps + b_pstep)
+
+
+def sec8d_pst_bytes(D: int) -> float:
+    """Particle shifting per particle-stage: x, v, vol, rho, c, phase read;
+    tv written."""
+    return 8 * (2 * D + 3) + 4 + 8 * D
+
+
+def sec8d_mfma_flops(D: int, U: int, M: int, S: int) -> float:
+    """Quadrature-by-mode contraction flops per cell-step (P2, 3-point rules)."""
+    qv, qf = 3 ** D, 3 ** (D - 1)
+    return S * 2 * U * M * (D * qv + 2 * D * qf)
+
+
+def worker_threads() -> int:
+    """Host worker threads: the process's share of the host's cores (the
+    OMP_NUM_THREADS of the environment, else the core count)."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
 TRAFFIC_PROFILES = {
     "p2rk3": ROOT / "profiles" / "step_shock_bubble_256x128x128_p2_rk3_kernels_and_traffic.txt",
 }
@@ -90,52 +125,180 @@ def profiled_traffic(kernel: str, size: str, scheme: str):
 
 
 def cpu_baseline(scheme, sample=(48, 24, 24), steps=2):
-    """The oracle (scalar C++ restatement of the reference step) on a reduced
-    grid of the same workload, timed on this host."""
+    """The oracle (the C++ FP64 restatement of the reference step; its
+    per-cell and per-particle loops run on worker threads, split and merge
+    sequentially, as in the reference) on a reduced grid of the same
+    workload, timed on this host with all worker threads and with one."""
+    import ctypes as C
+    from oracle.pyoracle import lib as olib
     from oracle.pyoracle_step import OracleSim
     from pgdg_p2_amd import cases
-    case, st = cases.shock_bubble_3d(nx=sample[0], ny=sample[1], nz=sample[2], ppc_axis=2,
-                                     jitter=0.25, scheme=scheme)
-    sim = OracleSim(case, st)
-    sim.step(1)  # first step (particle dt path) excluded
-    t0 = time.perf_counter()
-    sim.step(steps)
-    el = time.perf_counter() - t0
+    L = olib()
+    L.oracle_set_threads.argtypes = [C.c_int]
     nc = sample[0] * sample[1] * sample[2]
-    return {"value": nc * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{sample[0]}x{sample[1]}x{sample[2]} cells x 8 ppc ({st.n} particles), "
-                      f"steps 2..{steps + 1}, same initial condition, oracle FP64 single thread"}
+    res = {}
+    threads = worker_threads()
+    for t in (threads, 1):
+        L.oracle_set_threads(t)
+        case, st = cases.shock_bubble_3d(nx=sample[0], ny=sample[1], nz=sample[2], ppc_axis=2,
+                                         jitter=0.25, scheme=scheme)
+        sim = OracleSim(case, st)
+        sim.step(1)  # first step (particle dt path) excluded
+        t0 = time.perf_counter()
+        sim.step(steps)
+        res[t] = nc * steps / (time.perf_counter() - t0)
+        npart = st.n
+        print(f"[cpu] {t} threads: {res[t]:.4g} cell-steps/s", file=sys.stderr, flush=True)
+    L.oracle_set_threads(1)
+    return {"value": res[threads], "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "value_1_thread": res[1],
+            "sample": f"{sample[0]}x{sample[1]}x{sample[2]} cells x 8 ppc ({npart} particles), "
+                      f"steps 2..{steps + 1} of the same workload, oracle FP64 on {threads} worker "
+                      f"threads (and 1)"}
 
 
-def three_phase_bench(steps=20, warmup=2, nx=1792, ny=768):
-    """The 2-D three-phase triple point at 1792x768 P2 cells (gamma 1.5 / 1.4 /
-    1.67), the reference's one-particle-per-cell lattice, SSP-RK3, FP64 form:
-    cell-steps/s and split + merge events per second over `steps` steps."""
+def three_phase_bench(steps=20, warmup=500, nx=1792, ny=768):
+    """The 2-D three-phase triple point at 1792x768 P2 cells (gamma 1.5 /
+    1.4 / 1.67), 3x3 jittered particles per cell, SSP-RK3, CFL 0.2,
+    per-cell bounds [1, 27], mixed form: `warmup` steps of split/merge
+    spread the particle counts per cell; then cell-steps/s, particles/s and
+    split/merge particles/s over `steps` timed steps, and the distribution
+    of particles per cell."""
+    import numpy as np
     from pgdg_p2_amd import cases
     from pgdg_p2_amd.step import GpuSim
     t0 = time.perf_counter()
-    case, st = cases.triple_point_2d(nx=nx, ny=ny, ppc_axis=1, jitter=0.0, nphase=3,
-                                     scheme=dict(order=2, rk_stages=3))
+    case, st = cases.triple_point_2d(nx=nx, ny=ny, ppc_axis=3, jitter=0.25, nphase=3, cap_factor=2.0,
+                                     scheme=dict(order=2, rk_stages=3, cfl=0.2, ppc_min=1, ppc_max=27))
+    case.precision = 1
     gpu = GpuSim(case, st)
-    print(f"[3-phase] setup {time.perf_counter() - t0:.1f}s: {nx}x{ny} cells, {st.n} particles",
+    n_init = st.n
+    st.p = {}
+    print(f"[3-phase] setup {time.perf_counter() - t0:.1f}s: {nx}x{ny} cells, {n_init} particles",
           file=sys.stderr, flush=True)
-    gpu.step(warmup)
+    t1 = time.perf_counter()
+    ev_w = [0, 0]
+    for k in range(warmup):
+        gpu.step(1)
+        if k % 100 == 99:
+            sc = gpu.scalars()
+            print(f"[3-phase] warm-up step {k + 1}: {gpu.count()} particles", file=sys.stderr, flush=True)
     gpu.sync()
+    t_w = time.perf_counter() - t1
     n0 = gpu.count()
-    ev = 0
+    split = merge = 0
     t = time.perf_counter()
     for _ in range(steps):
         gpu.step(1)
         sc = gpu.scalars()
-        ev += sc.n_split + sc.n_merge
+        split += sc.n_split
+        merge += sc.n_merge
     gpu.sync()
     el = time.perf_counter() - t
     n1 = gpu.count()
+    cnt = gpu.cell_particle_counts()
     gpu.close()
-    return {"cells": nx * ny, "particles_start": n0, "particles_end": n1, "steps": steps,
+    hist = np.bincount(cnt, minlength=28)
+    nz = cnt[cnt > 0]
+    ppc_avg = (n0 + n1) / 2 / (nx * ny)
+    b_cs = sec8d_cell_step_bytes(2, 4, 6, 3, 3, ppc_avg)
+    return {"workload": f"2-D three-phase triple point, {nx}x{ny} P2 cells, 3x3 jittered particles "
+                        "per cell, SSP-RK3, CFL 0.2, per-cell bounds [1,27], mixed form",
+            "cells": nx * ny, "particles_initial": n_init, "particles_start": n0, "particles_end": n1,
+            "warmup_steps": warmup, "warmup_s": t_w, "steps": steps,
             "ms_per_step": el / steps * 1e3, "cell_steps_per_s": nx * ny * steps / el,
-            "split_merge_events": ev, "split_merge_events_per_s": ev / el,
-            "scheme": "P2, SSP-RK3, FP64 form, 1 particle per cell lattice, 3 phases"}
+            "particles_per_s": (n0 + n1) / 2 * steps / el,
+            "splits": split, "merges": merge,
+            "split_merge_particles_per_s": (split * 4 + merge * 2) / el,
+            "ppc": {"min": int(cnt.min()), "max": int(cnt.max()), "mean": float(cnt.mean()),
+                    "empty_cells": int((cnt == 0).sum()),
+                    "p01_p50_p99": [float(np.percentile(nz, q)) for q in (1, 50, 99)],
+                    "histogram_0_to_27_and_more": [int(v) for v in hist[:28]] + [int(hist[28:].sum())]},
+            "roofline_step": {"bound": "hbm", "bytes_per_cell_step": b_cs,
+                              "achieved_GBs": b_cs * nx * ny * steps / el / 1e9,
+                              "frac": b_cs * nx * ny * steps / el / 1e9 / HBM_PEAK_GBS},
+            "counts_vs_oracle": "bit-exact counts and order at 112x48 over 60 steps (GPU test); the "
+                                "oracle is not run at this size"}
+
+
+def triple_point_bench(steps=100, warmup=2, cpu_steps=10):
+    """The plain-method workload: the 2-D triple point on [0,7]x[0,3] with
+    112x48 P2 cells, 3x3 jittered particles per cell, SSP-RK3, CFL 0.2,
+    100 steps, FP64 and mixed forms on the GPU; the oracle on the host
+    worker threads for the first cpu_steps steps."""
+    import ctypes as C
+    from oracle.pyoracle import lib as olib
+    from oracle.pyoracle_step import OracleSim
+    from pgdg_p2_amd import cases
+    from pgdg_p2_amd.step import GpuSim
+    nx, ny = 112, 48
+    res = {"workload": "2-D triple point, 112x48 P2 cells, 3x3 jittered particles per cell, "
+                       "SSP-RK3, CFL 0.2"}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        case, st = cases.triple_point_2d(nx=nx, ny=ny, ppc_axis=3, jitter=0.25,
+                                         scheme=dict(order=2, rk_stages=3, cfl=0.2))
+        case.precision = prec
+        gpu = GpuSim(case, st)
+        gpu.step(warmup)
+        gpu.sync()
+        t0 = time.perf_counter()
+        gpu.step(steps)
+        gpu.sync()
+        el = time.perf_counter() - t0
+        res[name] = {"steps": steps, "ms_per_step": el / steps * 1e3,
+                     "cell_steps_per_s": nx * ny * steps / el, "particles_end": gpu.count()}
+        gpu.close()
+    L = olib()
+    L.oracle_set_threads.argtypes = [C.c_int]
+    threads = worker_threads()
+    L.oracle_set_threads(threads)
+    case, st = cases.triple_point_2d(nx=nx, ny=ny, ppc_axis=3, jitter=0.25,
+                                     scheme=dict(order=2, rk_stages=3, cfl=0.2))
+    sim = OracleSim(case, st)
+    sim.step(1)
+    t0 = time.perf_counter()
+    sim.step(cpu_steps)
+    el = time.perf_counter() - t0
+    L.oracle_set_threads(1)
+    res["cpu_oracle"] = {"cell_steps_per_s": nx * ny * cpu_steps / el, "threads": threads,
+                         "steps": f"2..{cpu_steps + 1}"}
+    res["mixed_speedup"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    return res
+
+
+def error_horizon(size=(64, 32, 32), steps=200, every=50, scheme=None):
+    """FP64 and mixed forms of the shock-bubble step side by side from the
+    same initial state for `steps` steps on a reduced grid: relative L1 and
+    Linf of the cell density (alpha-weighted over phases) every `every`
+    steps, and the particle counts."""
+    import numpy as np
+    from pgdg_p2_amd import cases
+    from pgdg_p2_amd.step import GpuSim
+    nx, ny, nz = size
+    sims = []
+    for prec in (0, 1):
+        case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25,
+                                         scheme=scheme)
+        case.precision = prec
+        sims.append((GpuSim(case, st), st.copy()))
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(every, steps + 1, every):
+        rho = []
+        for g, st in sims:
+            g.step(every)
+            g.save_cells(st)
+            rho.append((st.c["alph"] * st.c["rho"]).sum(axis=0))
+        d = np.abs(rho[1] - rho[0])
+        rows.append({"step": k, "density_L1_rel": float(d.sum() / np.abs(rho[0]).sum()),
+                     "density_Linf_rel": float(d.max() / np.abs(rho[0]).max()),
+                     "particles": [g.count() for g, _ in sims]})
+        print(f"[horizon] step {k}: L1 {rows[-1]['density_L1_rel']:.3g}", file=sys.stderr, flush=True)
+    for g, _ in sims:
+        g.close()
+    return {"grid": f"{nx}x{ny}x{nz}", "steps": steps, "rows": rows,
+            "wall_s": time.perf_counter() - t0,
+            "L1_le_1e-6_at_every_row": all(r["density_L1_rel"] <= 1e-6 for r in rows)}
 
 
 def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
@@ -261,6 +424,10 @@ def main():
     ap.add_argument("--no-mgfm", action="store_true")
     ap.add_argument("--no-mls", action="store_true")
     ap.add_argument("--no-3phase", action="store_true")
+    ap.add_argument("--no-tp", action="store_true", help="skip the 112x48 triple-point leg")
+    ap.add_argument("--no-horizon", action="store_true", help="skip the 200-step error horizon")
+    ap.add_argument("--3phase-warmup", dest="tp3_warmup", type=int, default=500)
+    ap.add_argument("--only-3phase", action="store_true")
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--only-mgfm", action="store_true")
@@ -292,6 +459,9 @@ def main():
         return
     if args.only_mgfm:
         print(json.dumps({"mgfm": mgfm_bench()}))
+        return
+    if args.only_3phase:
+        print(json.dumps({"three_phase_2d": three_phase_bench(warmup=args.tp3_warmup)}))
         return
 
     from pgdg_p2_amd import cases
@@ -350,7 +520,11 @@ def main():
     value = nc * args.steps / el
     dom = max(stages, key=lambda k: stages[k]["ms_total"])
     per_launch_ms = stages[dom]["ms_total"] / stages[dom]["launches"]
-    alg = stage_bytes(dom, case, R["n_mid"], walls)
+    alg_full = stage_bytes(dom, case, R["n_mid"], walls)
+    # algorithmic bytes by the per-unit figure of the particle-shifting
+    # stage: 100 B per particle-stage in 3-D (x, v, vol, rho, c, phase read;
+    # tv written), times the particles of the launch
+    alg = sec8d_pst_bytes(case.dim) * R["n_mid"] if dom == "pst" else alg_full
     achieved = alg / (per_launch_ms * 1e-3) / 1e9 if alg > 0 else None
     order = case.order
     tb = R["tuning"] or {}
@@ -358,15 +532,29 @@ def main():
              "rhs": f"pgdg::k_rhs<3, {order}, {tb.get('rhs', 64)}, float>",
              "p2g": f"pgdg::k_p2g<3, {order}>", "g2p": f"pgdg::k_g2p<3, {order}>"}.get(dom, "")
     traffic = profiled_traffic(kname, args.size, args.scheme)
+    S = 3 if case.rk_stages == 3 else 2
+    ppc_avg = (R["n_start"] + R["n_end"]) / 2 / nc
+    b_step = sec8d_cell_step_bytes(case.dim, case.U, case.M, case.nphase, S, ppc_avg)
+    f_step = sec8d_mfma_flops(case.dim, case.U, case.M, S)
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic, "alg_bytes_per_launch": alg, "avg_launch_ms": per_launch_ms}
+            "traffic": traffic, "alg_bytes_per_launch": alg,
+            "alg_bytes_per_unit": "100 B per particle-stage (particle shifting)" if dom == "pst" else None,
+            "alg_bytes_per_launch_with_stage_inputs": alg_full,
+            "avg_launch_ms": per_launch_ms,
+            "step": {"bytes_per_cell_step": b_step, "mfma_flops_per_cell_step": f_step,
+                     "achieved_GBs": b_step * value / 1e9,
+                     "frac": b_step * value / 1e9 / HBM_PEAK_GBS,
+                     "fp64_form_frac": b_step * nc * args.steps / plain["el"] / 1e9 / HBM_PEAK_GBS}}
     if traffic is not None:
         roof["traffic_source"] = str(TRAFFIC_PROFILES[args.scheme].relative_to(ROOT))
         roof["traffic_kernel"] = kname
 
     line = {
         "metric": "P2 DG cell-steps/s + MGFM Newton its/face, mixed vs FP64 plain at equal L1 err",
+        "metric_note": ("value: cell-steps/s of the mixed form; its density deviation from the FP64 "
+                        "form is in mixed_vs_plain (this run) and error_horizon (200 steps); Newton "
+                        "iterations per face are in mgfm"),
         "value": value,
         "unit": "cell-steps/s",
         "n_gpus": 1,
@@ -403,15 +591,20 @@ def main():
         "autotuned_workgroups": {"mixed": R["tuning"], "fp64": plain["tuning"]},
         "stage_graphs": "RK stages replayed as HIP graphs in the timed steps",
     }
+    if not args.no_horizon:
+        line["error_horizon"] = error_horizon(scheme=SCHEMES[args.scheme])
+    if not args.no_tp:
+        line["triple_point_112x48"] = triple_point_bench()
     if not args.no_mgfm:
         line["mgfm"] = mgfm_bench()
     if not args.no_3phase:
-        line["three_phase_2d"] = three_phase_bench()
+        line["three_phase_2d"] = three_phase_bench(warmup=args.tp3_warmup)
     if not args.no_mls:
         line["mls"] = mls_bench(args.mls_n)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(SCHEMES[args.scheme])
         line["cpu_baseline"]["host_cpus"] = os.cpu_count()
+        line["cpu_baseline"]["gpu_vs_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line))
 
 

@@ -418,6 +418,11 @@ int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 int pgdg_neighbor_counts(pgdg_ctx* ctx, int32_t* n_neighbor);
 /* Cells in use (Used_Ele_cen, S3:mpm.cpp:184-263): host [nc], 1 or 0.      */
 int pgdg_used_cells(pgdg_ctx* ctx, int32_t* used);
+/* Fluid particles in each cell's list (the cell lists of the last binning
+ * and split/merge edits; the reference sums them as particles per cell in
+ * Output_point, T2:mpm_ini.cpp:599-654): host [nc].  PGDG_ERR_STATE before
+ * the first binning after an upload.                                      */
+int pgdg_cell_particle_counts(pgdg_ctx* ctx, int32_t* counts);
 
 /* Checkpoint / restart.  The reference has none: it creates a restart
  * directory and never writes it (S3:Makefile:95, 115).  A checkpoint file

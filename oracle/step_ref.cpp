@@ -54,6 +54,7 @@
 #include <cstring>
 #include <functional>
 #include <unordered_set>
+#include <thread>
 #include <vector>
 
 #include "pgdg.h"
@@ -71,6 +72,27 @@ void MGFM_T2(double ul, double ur, double pl, double pr, double rhol, double rho
 static const double CELL_RATIO = 3.2;
 static const double CUT_OFF = 2.;
 static const double PI_REF = 3.1415926;  // S3:glbparam.h:36
+
+// Worker threads of the per-cell and per-particle loops (1: sequential).
+// Every parallel loop writes only the item it visits and reads state no
+// iteration of the same loop writes, so results do not depend on the count.
+static int g_threads = 1;
+template <class Fn>
+static void par_range(int64_t n, Fn fn) {
+  const int T = g_threads;
+  if (T <= 1 || n < 64) {
+    fn((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + T - 1) / T;
+  for (int t = 0; t < T; t++) {
+    const int64_t lo = t * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    if (lo >= hi) break;
+    th.emplace_back(fn, lo, hi);
+  }
+  for (auto& x : th) x.join();
+}
 
 static inline double AMAX1(double a, double b) { return a >= b ? a : b; }
 static inline double AMIN1(double a, double b) { return a <= b ? a : b; }
@@ -565,7 +587,7 @@ struct Sim : Base {
     const int ns = cs.rk_stages == 3 ? 3 : 2;
     for (int st = 0; st < ns; st++) {
       const int md = ns == 3 ? codes3[st] : codes2[st];
-      for (int id : fluid) refresh_neighbors(id, 2);
+      refresh_all(2);
       if (md == 0) mls_p2g();
       calculate_limiter();
       grid_acceleration();
@@ -1082,7 +1104,9 @@ struct Sim : Base {
   // ---- MLS_P2G (S3:mpm.cpp:274-382) -------------------------------------------
   void mls_p2g() {
     double hi = cs.p_g * 1 * 3.2 / CUT_OFF;
-    for (int ci : used) {
+    par_range((int64_t)used.size(), [&](int64_t b0_, int64_t b1_) {
+      for (int64_t k_ = b0_; k_ < b1_; k_++) {
+      const int ci = used[k_];
       Cell& cp = cells[ci];
       double mass_t[PGDG_MAX_PHASE], p_avg[PGDG_MAX_PHASE][U], Cf[PGDG_MAX_PHASE][U][M - 1];
       std::memset(mass_t, 0, sizeof(mass_t));
@@ -1158,6 +1182,7 @@ struct Sim : Base {
         }
       }
     }
+    });
   }
 
   // ---- Calculate_limiter (S3:mpm.cpp:392-738; T2:mpm.cpp:887-1225) ---------
@@ -1170,7 +1195,9 @@ struct Sim : Base {
   }
 
   void calculate_limiter() {
-    for (int ci : used) {
+    par_range((int64_t)used.size(), [&](int64_t b0_, int64_t b1_) {
+      for (int64_t k_ = b0_; k_ < b1_; k_++) {
+      const int ci = used[k_];
       Cell& cp = cells[ci];
       int i = cp.idx[0], j = cp.idx[1], k = cp.idx[2];
       double u_max[PGDG_MAX_PHASE][U], u_min[PGDG_MAX_PHASE][U], pi[PGDG_MAX_PHASE][U];
@@ -1353,6 +1380,7 @@ struct Sim : Base {
           for (int u = 0; u < U; u++) cp.lim[s][u] = 0.;
       }
     }
+    });
   }
 
   // ---- Grid_acceleration (S3:particle_flow.cpp:16-405) ---------------------
@@ -1470,9 +1498,13 @@ struct Sim : Base {
   }
 
   void grid_acceleration() {
-    for (int ci : used)
-      for (int s = 0; s < F; s++)
-        if (cells[ci].ph_state[s] == 1) cell_rhs(ci, s);
+    par_range((int64_t)used.size(), [&](int64_t b0, int64_t b1) {
+      for (int64_t k = b0; k < b1; k++) {
+        const int ci = used[k];
+        for (int s = 0; s < F; s++)
+          if (cells[ci].ph_state[s] == 1) cell_rhs(ci, s);
+      }
+    });
   }
 
   void cell_rhs(int ci, int ss) {
@@ -1650,7 +1682,9 @@ struct Sim : Base {
 
   // ---- Update_grid (S3:mpm.cpp:768-830; T2:mpm.cpp:1254-1320) -------------
   void update_grid(int md) {
-    for (int ci : used) {
+    par_range((int64_t)used.size(), [&](int64_t b0_, int64_t b1_) {
+      for (int64_t k_ = b0_; k_ < b1_; k_++) {
+      const int ci = used[k_];
       Cell& cp = cells[ci];
       double rv_a[3] = {0, 0, 0};
       double rho_a = 0, P_a = 0;
@@ -1700,12 +1734,22 @@ struct Sim : Base {
       for (int a = 0; a < 3; a++) cp.v[a] = rv_a[a] / rho_a;
       cp.P = P_a;
     }
+    });
   }
 
   // ---- Update_particle: two passes (S3:mpm.cpp:839-853) ----------------------
   void update_particle(int md) {
-    for (int id : fluid) update_state(id, md);
-    for (int id : fluid) update_pst(id);
+    par_range((int64_t)fluid.size(), [&](int64_t b0, int64_t b1) {
+      for (int64_t k = b0; k < b1; k++) update_state(fluid[k], md);
+    });
+    par_range((int64_t)fluid.size(), [&](int64_t b0, int64_t b1) {
+      for (int64_t k = b0; k < b1; k++) update_pst(fluid[k]);
+    });
+  }
+  void refresh_all(int status) {
+    par_range((int64_t)fluid.size(), [&](int64_t b0, int64_t b1) {
+      for (int64_t k = b0; k < b1; k++) refresh_neighbors(fluid[k], status);
+    });
   }
 
   // Update_state_onestep (S3:particle_flow.cpp:413-477)
@@ -1911,7 +1955,7 @@ struct Sim : Base {
       case 3: split_fast(); return PGDG_OK;
       case 4: merge_fast(); return PGDG_OK;
       case 5: membership(); return PGDG_OK;
-      case 6: for (int id : fluid) refresh_neighbors(id, arg); return PGDG_OK;
+      case 6: refresh_all(arg); return PGDG_OK;
       case 7: mls_p2g(); return PGDG_OK;
       case 8: calculate_limiter(); return PGDG_OK;
       case 9: grid_acceleration(); return PGDG_OK;
@@ -1956,6 +2000,9 @@ int oracle_neighbor_counts(void* h, int32_t* out) {
   return static_cast<oracle::Base*>(h)->neighbor_counts(out);
 }
 int oracle_used_cells(void* h, int32_t* out) { return static_cast<oracle::Base*>(h)->used_cells(out); }
+// Worker threads of the step's per-cell and per-particle loops.
+void oracle_set_threads(int n) { oracle::g_threads = n > 0 ? n : 1; }
+int oracle_get_threads(void) { return oracle::g_threads; }
 int64_t oracle_bins(void* h, int32_t* off, int32_t* ent, int64_t cap) {
   return static_cast<oracle::Base*>(h)->bins_out(off, ent, cap);
 }

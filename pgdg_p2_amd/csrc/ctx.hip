@@ -962,6 +962,17 @@ int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
   return rc;
 }
 
+int pgdg_cell_particle_counts(pgdg_ctx* x, int32_t* counts) {
+  if (!x || !counts) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.loaded) return PGDG_ERR_STATE;
+  if (!c.bins_valid) return PGDG_ERR_STATE;
+  PGDG_TRY(c.T.fluid_counts());
+  PGDG_HIP_TRY(hipMemcpyAsync(counts, c.T.fcnt, sizeof(int32_t) * (size_t)c.nc, hipMemcpyDeviceToHost, c.st));
+  PGDG_HIP_TRY(hipStreamSynchronize(c.st));
+  return PGDG_OK;
+}
+
 int pgdg_used_cells(pgdg_ctx* x, int32_t* used) {
   if (!x || !used) return PGDG_ERR_ARG;
   Ctx& c = x->c;

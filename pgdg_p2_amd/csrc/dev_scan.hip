@@ -162,17 +162,18 @@ __global__ void k_bucket_scatter(const uint32_t* __restrict__ key, Len len, int*
 
 // Insertion sort of each bucket's element indices (ascending).
 __global__ void k_bucket_sort(const int* __restrict__ off, Len nb_len, int* order) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb_len.get()) return;
-  const int a = off[c], b = off[c + 1];
-  for (int i = a + 1; i < b; i++) {
-    const int v = order[i];
-    int j = i - 1;
-    while (j >= a && order[j] > v) {
-      order[j + 1] = order[j];
-      j--;
+  const long long nb = nb_len.get();
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < nb; c += (long long)gridDim.x * blockDim.x) {
+    const int a = off[c], b = off[c + 1];
+    for (int i = a + 1; i < b; i++) {
+      const int v = order[i];
+      int j = i - 1;
+      while (j >= a && order[j] > v) {
+        order[j + 1] = order[j];
+        j--;
+      }
+      order[j + 1] = v;
     }
-    order[j + 1] = v;
   }
 }
 
@@ -213,7 +214,7 @@ int bucket_stable(const uint32_t* key, Len K, int nb, int* off, int* cursor, int
   PGDG_TRY(scan_excl<int>(cursor, off, nb1, nullptr, false, part, st));
   hipLaunchKernelGGL(k_copy_i32, dim3(kFillBlocks), dim3(256), 0, st, (const int*)off, cursor, nb1);
   hipLaunchKernelGGL(k_bucket_scatter, dim3(kFillBlocks), dim3(256), 0, st, key, K, cursor, order);
-  hipLaunchKernelGGL(k_bucket_sort, dim3((nb + 255) / 256), dim3(256), 0, st, (const int*)off,
+  hipLaunchKernelGGL(k_bucket_sort, dim3(kFillBlocks * 2), dim3(256), 0, st, (const int*)off,
                      len_host(nb, gate), order);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;

@@ -139,33 +139,35 @@ __global__ void k_set_nent(TopoCounters* tc, long long k) {
 // ---------------------------------------------------------------------------
 __global__ void k_edit_count(const int* off, const int* ent, int nc, const uint8_t* rm, int* cnt,
                              const int* aoff, const long long* gate) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc || *gate == 0) return;
-  int k = 0;
-  for (int e = off[c]; e < off[c + 1]; e++) {
-    int r = ent[e];
-    if (r >= 0 && rm[r]) continue;
-    k++;
+  if (*gate == 0) return;
+  PGDG_LOOP(c, nc) {
+    int k = 0;
+    for (int e = off[c]; e < off[c + 1]; e++) {
+      int r = ent[e];
+      if (r >= 0 && rm[r]) continue;
+      k++;
+    }
+    cnt[c] = k + aoff[c + 1] - aoff[c];
   }
-  cnt[c] = k + aoff[c + 1] - aoff[c];
 }
 
 __global__ void k_edit_fill(const int* off, const int* ent, int nc, const uint8_t* rm, const int* noff,
                             int* nent, const int* aoff, const int* aorder, const int* app_ref,
                             const long long* gate, long long ent_cap, TopoCounters* tc) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc || *gate == 0) return;
-  int o = noff[c];
-  if (o + (off[c + 1] - off[c]) + (aoff[c + 1] - aoff[c]) > ent_cap) {  // inconsistent counts
-    tc->err = PGDG_ERR_STATE;
-    return;
+  if (*gate == 0) return;
+  PGDG_LOOP(c, nc) {
+    int o = noff[c];
+    if (o + (off[c + 1] - off[c]) + (aoff[c + 1] - aoff[c]) > ent_cap) {  // inconsistent counts
+      tc->err = PGDG_ERR_STATE;
+      continue;
+    }
+    for (int e = off[c]; e < off[c + 1]; e++) {
+      int r = ent[e];
+      if (r >= 0 && rm[r]) continue;
+      nent[o++] = r;
+    }
+    for (int t = aoff[c]; t < aoff[c + 1]; t++) nent[o++] = app_ref[aorder[t]];
   }
-  for (int e = off[c]; e < off[c + 1]; e++) {
-    int r = ent[e];
-    if (r >= 0 && rm[r]) continue;
-    nent[o++] = r;
-  }
-  for (int t = aoff[c]; t < aoff[c + 1]; t++) nent[o++] = app_ref[aorder[t]];
 }
 
 __global__ void k_copy_len(const int* __restrict__ src, int* dst, Len len) {
@@ -382,14 +384,15 @@ __global__ void k_bound_walk(const int* __restrict__ coff, const int* __restrict
                              const int* __restrict__ ccand, const int* __restrict__ ccost,
                              const int* __restrict__ fcnt, int nc, int lim, int upper, uint8_t* reject,
                              const long long* gate) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc || *gate == 0) return;
-  int run = 0;
-  for (int t = coff[c]; t < coff[c + 1]; t++) {
-    const int e = corder[t];
-    run += ccost[e];
-    const bool bad = upper ? fcnt[c] + run > lim : fcnt[c] - run < lim;
-    if (bad) reject[ccand[e]] = 1;
+  if (*gate == 0) return;
+  PGDG_LOOP(c, nc) {
+    int run = 0;
+    for (int t = coff[c]; t < coff[c + 1]; t++) {
+      const int e = corder[t];
+      run += ccost[e];
+      const bool bad = upper ? fcnt[c] + run > lim : fcnt[c] - run < lim;
+      if (bad) reject[ccand[e]] = 1;
+    }
   }
 }
 
@@ -404,6 +407,20 @@ __global__ void k_contrib_check(TopoCounters* tc, long long ccap) {
 // split
 // ---------------------------------------------------------------------------
 enum { T_NONE = 0, T_SPLIT = 1, T_INLET = 2, T_OUTLET = 3 };
+
+// Event totals of a pass: per-lane counts added once per lane that has any
+// (sums, so the totals do not depend on the order of the atomics).
+__device__ inline void add_counts(TopoCounters* tc, long long cs, long long ci, long long co) {
+  if (cs) atomicAdd((unsigned long long*)&tc->tot_split, (unsigned long long)cs);
+  if (ci) atomicAdd((unsigned long long*)&tc->tot_inlet, (unsigned long long)ci);
+  if (co) atomicAdd((unsigned long long*)&tc->tot_outlet, (unsigned long long)co);
+}
+
+__global__ void k_pass_zero(TopoCounters* tc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  tc->tot_split = tc->tot_inlet = tc->tot_outlet = 0;
+  tc->n_pairs = 0;
+}
 enum { F_SPLIT = 1, F_INLET = 2, F_OUTLET = 4 };
 
 __device__ inline uint8_t split_type(int flags) {
@@ -415,16 +432,13 @@ __device__ inline uint8_t split_type(int flags) {
 // flag is provisional: flags are kept in `type` and lc32 counts the cells a
 // split candidate adds to.
 template <int D>
-__global__ void k_split_classify(Geo g, PtlDev P, const TopoCounters* tc, uint8_t* type, int* is_s,
-                                 int* is_i, int* is_o, int* ncontrib, int bounds, uint8_t* rm, uint8_t* aff,
-                                 uint8_t* src, uint8_t* reject) {
+__global__ void k_split_classify(Geo g, PtlDev P, TopoCounters* tc, uint8_t* type, int* is_s,
+                                 int* is_i, int* is_o, int* ncontrib, int bounds, uint8_t* src,
+                                 uint8_t* reject) {
   const long long n = tc->n_total;
   const bool io = g.sem == PGDG_SEM_S3;
+  long long cs = 0, ci = 0, co = 0;
   PGDG_LOOP(i, n) {
-    rm[i] = 0;
-    aff[i] = 0;
-    src[i] = 0;
-    reject[i] = 0;
     const double vol = P.vol[i];
     const double dp = root_d<D>(vol);
     const double x0 = P.x[i];
@@ -434,6 +448,7 @@ __global__ void k_split_classify(Geo g, PtlDev P, const TopoCounters* tc, uint8_
     const int flags = (f1 ? F_SPLIT : 0) | (f2 ? F_INLET : 0) | (f3 ? F_OUTLET : 0);
     if (bounds) {
       type[i] = (uint8_t)flags;
+      reject[i] = 0;
       int k = 0;
       if (f1) {
         double x[3];
@@ -451,8 +466,12 @@ __global__ void k_split_classify(Geo g, PtlDev P, const TopoCounters* tc, uint8_
       is_i[i] = t == T_INLET;
       is_o[i] = t == T_OUTLET;
       src[i] = (t == T_SPLIT || t == T_OUTLET) ? 1 : 0;
+      cs += t == T_SPLIT;
+      ci += t == T_INLET;
+      co += t == T_OUTLET;
     }
   }
+  add_counts(tc, cs, ci, co);
 }
 
 template <int D>
@@ -479,9 +498,10 @@ __global__ void k_split_contrib(Geo g, PtlDev P, const TopoCounters* tc, const u
   }
 }
 
-__global__ void k_split_final(const TopoCounters* tc, uint8_t* type, const uint8_t* reject, int* is_s, int* is_i,
+__global__ void k_split_final(TopoCounters* tc, uint8_t* type, const uint8_t* reject, int* is_s, int* is_i,
                               int* is_o, uint8_t* src) {
   const long long n = tc->n_total;
+  long long cs = 0, ci = 0, co = 0;
   PGDG_LOOP(i, n) {
     int flags = type[i];
     if (reject[i]) flags &= ~F_SPLIT;
@@ -491,7 +511,11 @@ __global__ void k_split_final(const TopoCounters* tc, uint8_t* type, const uint8
     is_i[i] = t == T_INLET;
     is_o[i] = t == T_OUTLET;
     src[i] = (t == T_SPLIT || t == T_OUTLET) ? 1 : 0;
+    cs += t == T_SPLIT;
+    ci += t == T_INLET;
+    co += t == T_OUTLET;
   }
+  add_counts(tc, cs, ci, co);
 }
 
 // Totals of the pass: new particles, appended entries, capacity.
@@ -823,10 +847,7 @@ __global__ void __launch_bounds__(128) k_merge_candidate(Geo g, PtlDev P, const 
   const long long n = tc->n_total;
   const int64_t cap = P.cap;
   PGDG_LOOP(i, n) {
-    rm[i] = 0;
-    aff[i] = 0;
-    src[i] = 0;
-    reject[i] = 0;
+    if (g.ppc_min > 0) reject[i] = 0;
     int best = -1;
     if (!dead[i] && merge_small<D>(g, P, i, fcnt)) {
       double dss = g.dss0;
@@ -858,13 +879,15 @@ __global__ void __launch_bounds__(128) k_merge_candidate(Geo g, PtlDev P, const 
 // Survivors: mutual candidates, the lower index survives.  With per-cell
 // bounds, ncontrib counts the cells a pair removes from.
 template <int D>
-__global__ void k_merge_mark(Geo g, PtlDev P, const TopoCounters* tc, const int* cand, int* surv, int bounds,
+__global__ void k_merge_mark(Geo g, PtlDev P, TopoCounters* tc, const int* cand, int* surv, int bounds,
                              int* ncontrib) {
   const long long n = tc->n_total;
+  long long cnt = 0;
   PGDG_LOOP(i, n) {
     int c = cand[i];
     const int s = (c >= 0 && cand[c] == (int)i && (int)i < c) ? 1 : 0;
     surv[i] = s;
+    cnt += s;
     if (bounds) {
       int k = 0;
       if (s) {
@@ -875,6 +898,7 @@ __global__ void k_merge_mark(Geo g, PtlDev P, const TopoCounters* tc, const int*
       ncontrib[i] = k;
     }
   }
+  if (!bounds && cnt) atomicAdd((unsigned long long*)&tc->n_pairs, (unsigned long long)cnt);
 }
 
 template <int D>
@@ -898,11 +922,14 @@ __global__ void k_merge_contrib(Geo g, PtlDev P, const TopoCounters* tc, const i
   }
 }
 
-__global__ void k_merge_final(const TopoCounters* tc, const uint8_t* reject, int* surv) {
+__global__ void k_merge_final(TopoCounters* tc, const uint8_t* reject, int* surv) {
   const long long n = tc->n_total;
+  long long cnt = 0;
   PGDG_LOOP(i, n) {
     if (reject[i]) surv[i] = 0;
+    cnt += surv[i];
   }
+  if (cnt) atomicAdd((unsigned long long*)&tc->n_pairs, (unsigned long long)cnt);
 }
 
 __global__ void k_merge_begin(TopoCounters* tc, int nc) {
@@ -1084,6 +1111,13 @@ int Topo::set_pool_used(int64_t used) {
   return PGDG_OK;
 }
 
+int Topo::fluid_counts() {
+  hipLaunchKernelGGL(k_fluid_count, dim3(g1(nc)), dim3(256), 0, stream, (const int*)bin_off, (const int*)bin_ent,
+                     nc, fcnt);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
+}
+
 int Topo::reset_step_counts() {
   hipLaunchKernelGGL(k_tc_step, dim3(1), dim3(64), 0, stream, tc);
   PGDG_HIP_TRY(hipGetLastError());
@@ -1108,10 +1142,10 @@ int Topo::build_bins(const Geo& g, const PtlDev& P, const WallDev& W, int64_t n)
 int Topo::edit_bins() {
   const long long* gate = &tc->active;
   PGDG_TRY(bucket_stable(app_cell, len_dev(&tc->k_app), nc, coff, cursor, order, part32, stream, gate));
-  hipLaunchKernelGGL(k_edit_count, dim3(g1(nc)), dim3(256), 0, stream, (const int*)bin_off,
+  hipLaunchKernelGGL(k_edit_count, dim3(kLoopBlocks), dim3(256), 0, stream, (const int*)bin_off,
                      (const int*)bin_ent, nc, (const uint8_t*)rm, cnt, (const int*)coff, gate);
   PGDG_TRY(scan_excl<int>(cnt, off_tmp, len_host(nc, gate), &tc->nent, true, part32, stream));
-  hipLaunchKernelGGL(k_edit_fill, dim3(g1(nc)), dim3(256), 0, stream, (const int*)bin_off,
+  hipLaunchKernelGGL(k_edit_fill, dim3(kLoopBlocks), dim3(256), 0, stream, (const int*)bin_off,
                      (const int*)bin_ent, nc, (const uint8_t*)rm, (const int*)off_tmp, ent_tmp,
                      (const int*)coff, (const int*)order, (const int*)app_ref, gate, (long long)(cap + nwall), tc);
   hipLaunchKernelGGL(k_copy_len, dim3(kLoopBlocks), dim3(256), 0, stream, (const int*)off_tmp, bin_off,
@@ -1147,9 +1181,9 @@ int Topo::build_lists(const Geo& g, const double* xq, const double* hq, const do
 }
 
 template <int D>
-int Topo::mark_in(const Geo& g, const PtlDev& P) {
+int Topo::mark_in(const Geo& g, const PtlDev& P, const long long* gate) {
   if (g.sem != PGDG_SEM_S3) return PGDG_OK;
-  hipLaunchKernelGGL(k_mark_in<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P, len_dev(&tc->n_total),
+  hipLaunchKernelGGL(k_mark_in<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P, len_dev(&tc->n_total, gate),
                      (const uint8_t*)dead);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
@@ -1164,8 +1198,9 @@ int Topo::split(const Geo& g, ParticleArrays& PA, const WallDev& W) {
   if (bounds)
     hipLaunchKernelGGL(k_fluid_count, dim3(g1(nc)), dim3(256), 0, stream, (const int*)bin_off,
                        (const int*)bin_ent, nc, fcnt);
-  hipLaunchKernelGGL(k_split_classify<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P,
-                     (const TopoCounters*)tc, type, is_a, is_b, is_c, lc32, bounds ? 1 : 0, rm, aff, src, reject);
+  hipLaunchKernelGGL(k_pass_zero, dim3(1), dim3(64), 0, stream, tc);
+  hipLaunchKernelGGL(k_split_classify<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P, tc, type, is_a, is_b,
+                     is_c, lc32, bounds ? 1 : 0, src, reject);
   if (bounds) {
     // admissions against ppc_max, per cell in particle order
     PGDG_TRY(scan_excl<int>(lc32, cbase, N, &tc->n_contrib, false, part32, stream));
@@ -1175,18 +1210,20 @@ int Topo::split(const Geo& g, ParticleArrays& PA, const WallDev& W) {
     hipLaunchKernelGGL(k_contrib_check, dim3(1), dim3(64), 0, stream, tc, (long long)ccap);
     PGDG_TRY(bucket_stable(ckey, len_dev(&tc->n_contrib), nc, coff, cursor, order, part32, stream,
                            &tc->n_contrib));
-    hipLaunchKernelGGL(k_bound_walk, dim3(g1(nc)), dim3(256), 0, stream, (const int*)coff, (const int*)order,
+    hipLaunchKernelGGL(k_bound_walk, dim3(kLoopBlocks), dim3(256), 0, stream, (const int*)coff, (const int*)order,
                        (const int*)ccand, (const int*)ccost, (const int*)fcnt, nc, g.ppc_max, 1, reject,
                        (const long long*)&tc->n_contrib);
-    hipLaunchKernelGGL(k_split_final, dim3(kLoopBlocks), dim3(256), 0, stream, (const TopoCounters*)tc, type,
+    hipLaunchKernelGGL(k_split_final, dim3(kLoopBlocks), dim3(256), 0, stream, tc, type,
                        (const uint8_t*)reject, is_a, is_b, is_c, src);
   }
-  PGDG_TRY(scan_excl<int>(is_a, is_a, N, &tc->tot_split, false, part32, stream));
-  PGDG_TRY(scan_excl<int>(is_b, is_b, N, &tc->tot_inlet, false, part32, stream));
-  PGDG_TRY(scan_excl<int>(is_c, is_c, N, &tc->tot_outlet, false, part32, stream));
+  // split and inlet bases (the outlet total is all the pass needs)
+  PGDG_TRY(scan_excl<int>(is_a, is_a, len_dev(&tc->n_total, &tc->tot_split), nullptr, false, part32, stream));
+  PGDG_TRY(scan_excl<int>(is_b, is_b, len_dev(&tc->n_total, &tc->tot_inlet), nullptr, false, part32, stream));
   hipLaunchKernelGGL(k_split_begin, dim3(1), dim3(64), 0, stream, tc, (long long)cap, nch, nc);
   const long long* gate = &tc->active;
   PGDG_TRY(fill_u8(win, len_dev(&tc->win_len), 0, stream));
+  PGDG_TRY(fill_u8(rm, len_dev(&tc->n_total, gate), 0, stream));
+  PGDG_TRY(fill_u8(aff, len_dev(&tc->n_total, gate), 0, stream));
   // old lists of split parents and outlet particles -> affected
   hipLaunchKernelGGL(k_scatter_lists, dim3(kLoopBlocks), dim3(256), 0, stream, len_dev(&tc->n_total, gate),
                      (const uint8_t*)src, (const int*)lcnt, (const int64_t*)loff, (const int*)pool, aff);
@@ -1218,12 +1255,13 @@ int Topo::merge(const Geo& g, ParticleArrays& PA, const WallDev& W) {
   if (bounds)
     hipLaunchKernelGGL(k_fluid_count, dim3(g1(nc)), dim3(256), 0, stream, (const int*)bin_off,
                        (const int*)bin_ent, nc, fcnt);
+  hipLaunchKernelGGL(k_pass_zero, dim3(1), dim3(64), 0, stream, tc);
   hipLaunchKernelGGL(k_merge_candidate<D>, dim3(kLoopBlocks), dim3(128), 0, stream, g, P,
                      (const TopoCounters*)tc, (const int*)lcnt, (const int64_t*)loff, (const int*)pool, cand,
                      (const int*)fcnt, (const uint8_t*)dead, rm, aff, src, reject);
   // survivor indicator in is_b, its exclusive scan in is_a
-  hipLaunchKernelGGL(k_merge_mark<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P, (const TopoCounters*)tc,
-                     (const int*)cand, is_b, bounds ? 1 : 0, lc32);
+  hipLaunchKernelGGL(k_merge_mark<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P, tc, (const int*)cand, is_b,
+                     bounds ? 1 : 0, lc32);
   if (bounds) {
     // admissions against ppc_min, per cell in survivor order
     PGDG_TRY(scan_excl<int>(lc32, cbase, N, &tc->n_contrib, false, part32, stream));
@@ -1233,16 +1271,18 @@ int Topo::merge(const Geo& g, ParticleArrays& PA, const WallDev& W) {
     hipLaunchKernelGGL(k_contrib_check, dim3(1), dim3(64), 0, stream, tc, (long long)ccap);
     PGDG_TRY(bucket_stable(ckey, len_dev(&tc->n_contrib), nc, coff, cursor, order, part32, stream,
                            &tc->n_contrib));
-    hipLaunchKernelGGL(k_bound_walk, dim3(g1(nc)), dim3(256), 0, stream, (const int*)coff, (const int*)order,
+    hipLaunchKernelGGL(k_bound_walk, dim3(kLoopBlocks), dim3(256), 0, stream, (const int*)coff, (const int*)order,
                        (const int*)ccand, (const int*)ccost, (const int*)fcnt, nc, g.ppc_min, 0, reject,
                        (const long long*)&tc->n_contrib);
-    hipLaunchKernelGGL(k_merge_final, dim3(kLoopBlocks), dim3(256), 0, stream, (const TopoCounters*)tc,
-                       (const uint8_t*)reject, is_b);
+    hipLaunchKernelGGL(k_merge_final, dim3(kLoopBlocks), dim3(256), 0, stream, tc, (const uint8_t*)reject, is_b);
   }
-  PGDG_TRY(scan_excl<int>(is_b, is_a, N, &tc->n_pairs, false, part32, stream));
+  PGDG_TRY(scan_excl<int>(is_b, is_a, len_dev(&tc->n_total, &tc->n_pairs), nullptr, false, part32, stream));
   hipLaunchKernelGGL(k_merge_begin, dim3(1), dim3(64), 0, stream, tc, nc);
   const long long* gate = &tc->active;
   PGDG_TRY(fill_u8(win, len_dev(&tc->win_len), 0, stream));
+  PGDG_TRY(fill_u8(rm, len_dev(&tc->n_total, gate), 0, stream));
+  PGDG_TRY(fill_u8(aff, len_dev(&tc->n_total, gate), 0, stream));
+  PGDG_TRY(fill_u8(src, len_dev(&tc->n_total, gate), 0, stream));
   hipLaunchKernelGGL(k_merge_apply<D>, dim3(kLoopBlocks), dim3(256), 0, stream, g, P, (const TopoCounters*)tc,
                      (const int*)cand, (const int*)is_b, (const int*)is_a, rm, dead, src, app_ref, app_cell);
   // old lists of survivors and partners, survivors themselves, windows
@@ -1258,7 +1298,9 @@ int Topo::merge(const Geo& g, ParticleArrays& PA, const WallDev& W) {
                      (const uint8_t*)win, (const uint8_t*)aff, (const uint8_t*)dead, need_);
   PGDG_TRY(build_lists<D>(g, P.x, nullptr, P.x, W, 3));
   PGDG_HIP_TRY(hipGetLastError());
-  return mark_in<D>(g, P);
+  // positions and volumes change only for merged particles: an inactive
+  // pass leaves mark_in as the last pass computed it
+  return mark_in<D>(g, P, gate);
 }
 
 int Topo::finish(ParticleArrays& PA, int64_t* n_out) {
@@ -1350,7 +1392,7 @@ int Topo::end_of_step_lists(const Geo& g, const PtlDev& P, const double* xb, con
   template int Topo::merge<D>(const Geo&, ParticleArrays&, const WallDev&);                       \
   template int Topo::end_of_step_lists<D>(const Geo&, const PtlDev&, const double*, const double*, \
                                           const WallDev&, int64_t);                               \
-  template int Topo::mark_in<D>(const Geo&, const PtlDev&);
+  template int Topo::mark_in<D>(const Geo&, const PtlDev&, const long long*);
 PGDG_INST(2)
 PGDG_INST(3)
 #undef PGDG_INST

@@ -133,7 +133,7 @@ struct Topo {
   int build_lists(const Geo& g, const double* xq, const double* hq, const double* xall,
                   const WallDev& W, int status);
   template <int D>
-  int mark_in(const Geo& g, const PtlDev& P);
+  int mark_in(const Geo& g, const PtlDev& P, const long long* gate = nullptr);
   template <int D>
   int split(const Geo& g, ParticleArrays& PA, const WallDev& W);
   template <int D>
@@ -142,6 +142,8 @@ struct Topo {
   // and returns the live count and the step's event counts.
   int finish(ParticleArrays& PA, int64_t* n_out);
   int reset_step_counts();
+  // fluid particles of every cell's bin into fcnt (device)
+  int fluid_counts();
   template <int D>
   int end_of_step_lists(const Geo& g, const PtlDev& P, const double* xb, const double* hb,
                         const WallDev& W, int64_t n);

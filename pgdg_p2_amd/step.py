@@ -128,6 +128,14 @@ class GpuSim:
               "pgdg_used_cells")
         return out
 
+    def cell_particle_counts(self):
+        """Fluid particles in each cell's list (cell order)."""
+        import numpy as np
+        out = np.zeros(self.case.nc, dtype=np.int32)
+        check(self.L.pgdg_cell_particle_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))),
+              "pgdg_cell_particle_counts")
+        return out
+
     def count(self) -> int:
         n = C.c_int64()
         check(self.L.pgdg_particle_count(self.h, C.byref(n)))

@@ -1109,9 +1109,11 @@ struct Sim : Base {
       const int ci = used[k_];
       Cell& cp = cells[ci];
       double mass_t[PGDG_MAX_PHASE], p_avg[PGDG_MAX_PHASE][U], Cf[PGDG_MAX_PHASE][U][M - 1];
+      double q_avg[PGDG_MAX_PHASE][U];  // order 2: the particles' own values
       std::memset(mass_t, 0, sizeof(mass_t));
       std::memset(p_avg, 0, sizeof(p_avg));
       std::memset(Cf, 0, sizeof(Cf));
+      std::memset(q_avg, 0, sizeof(q_avg));
       int i = cp.idx[0], j = cp.idx[1], k = cp.idx[2];
       int kc = D == 3 ? 1 : 0;
       for (int t = IMAX(i - 1, 0); t <= IMIN(i + 1, N[0] - 1); t++)
@@ -1129,6 +1131,8 @@ struct Sim : Base {
                 pj[0] = nb.rho;
                 for (int a = 0; a < D; a++) pj[1 + a] = nb.rhov[a];
                 pj[U - 1] = nb.E;
+                double pq[U];
+                for (int u = 0; u < U; u++) pq[u] = pj[u];
                 // the particle's polynomial at the cell centre
                 const double nx[3] = {-xij[0], -xij[1], -xij[2]};
                 double psi[M - 1];
@@ -1141,6 +1145,8 @@ struct Sim : Base {
                 int ph = nb.phase - 1;
                 if (ph >= 0 && ph < F) {
                   for (int u = 0; u < U; u++) p_avg[ph][u] += pj[u] * ww;
+                  if (ORD == 2)
+                    for (int u = 0; u < U; u++) q_avg[ph][u] += pq[u] * ww;
                   for (int u = 0; u < U; u++)
                     for (int m = 0; m < M - 1; m++) Cf[ph][u][m] += nb.coef[u][1 + m] * ww;
                   mass_t[ph] += ww;
@@ -1158,9 +1164,24 @@ struct Sim : Base {
           for (int u = 0; u < U; u++) p_avg[ss][u] /= (mass_t[ss] + 1e-20);
           for (int u = 0; u < U; u++)
             for (int m = 0; m < M - 1; m++) Cf[ss][u][m] /= (mass_t[ss] + 1e-20);
-          if (ORD == 2)  // centre value -> mean of the zero-mean basis
+          if (ORD == 2) {  // centre value -> mean of the zero-mean basis
             for (int u = 0; u < U; u++)
               for (int a = 0; a < D; a++) p_avg[ss][u] += Cf[ss][u][(D == 2 ? 3 : 6) + a] * q12[a];
+            // positivity: a mean extrapolated from the particles' quadratic
+            // copies with non-positive density or pressure below
+            // -0.05 Pp0 (the bound of Update_state_onestep's fallback,
+            // S3:particle_flow.cpp:460-469) is replaced by the weighted
+            // average of the particles' own values, with zero slopes
+            double sq = 0;
+            for (int a = 0; a < D; a++) sq += p_avg[ss][1 + a] * p_avg[ss][1 + a];
+            double e_m = p_avg[ss][U - 1] - 0.5 * sq / p_avg[ss][0];
+            double P_m = eos_p(e_m, ss);
+            if (!(p_avg[ss][0] > 0) || P_m < -0.05 * cs.pinf[ss]) {
+              for (int u = 0; u < U; u++) p_avg[ss][u] = q_avg[ss][u] / (mass_t[ss] + 1e-20);
+              for (int u = 0; u < U; u++)
+                for (int m = 0; m < M - 1; m++) Cf[ss][u][m] = 0;
+            }
+          }
           cp.coef[ss][0][0] = cp.rho[ss] = p_avg[ss][0];
           for (int a = 0; a < D; a++) cp.coef[ss][1 + a][0] = cp.rhov[ss][a] = p_avg[ss][1 + a];
           cp.coef[ss][U - 1][0] = cp.E[ss] = p_avg[ss][U - 1];

@@ -75,9 +75,13 @@ __device__ inline double p2g_weight(const Geo& g, double sq, double vol) {
 // phases' coefficients from the weighted averages (pav [phase][U], cf
 // [phase][U][B]; read only for appearing phases), absent phases cleared,
 // velocity and pressure.  APPEAR false: the caller knows no phase appears.
+// Order 2: qav [phase][U] are the weighted averages of the particles' own
+// values; an appearing mean with non-positive density or pressure below
+// -0.05 Pp0 (the bound of Update_state_onestep's fallback,
+// S3:particle_flow.cpp:460-469) is replaced by them, with zero slopes.
 template <int D, int ORD, bool APPEAR>
 __device__ inline void p2g_finish(const Geo& g, const CellDev& C, int c, const double* mass_t,
-                                  const double* pav, const double* cf) {
+                                  const double* pav, const double* cf, const double* qav = nullptr) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1;
   const int nc = g.nc;
   double mass_total = 0;
@@ -99,11 +103,23 @@ __device__ inline void p2g_finish(const Geo& g, const CellDev& C, int c, const d
       double pa[U];
 #pragma unroll
       for (int u = 0; u < U; u++) pa[u] = pv[u] / (mass_t[ss] + 1e-20);
-      if constexpr (ORD == 2)  // centre value -> mean of the zero-mean basis
+      bool keep_tail = true;
+      if constexpr (ORD == 2) {  // centre value -> mean of the zero-mean basis
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
           for (int a = 0; a < D; a++) pa[u] += cs[u * B + (D == 2 ? 3 : 6) + a] / (mass_t[ss] + 1e-20) * g.q12[a];
+        double sq = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++) sq += pa[1 + a] * pa[1 + a];
+        const double e_m = pa[U - 1] - 0.5 * sq / pa[0];
+        const double P_m = eos_p(e_m, Eos{g.gamma[ss], g.pinf[ss]});
+        if (!(pa[0] > 0) || P_m < -0.05 * g.pinf[ss]) {
+          keep_tail = false;
+#pragma unroll
+          for (int u = 0; u < U; u++) pa[u] = qav[ss * U + u] / (mass_t[ss] + 1e-20);
+        }
+      }
       double rho = pa[0];
       C.coef[CIDX(ss, 0, 0)] = rho;
       C.rho[ss * nc + c] = rho;
@@ -121,7 +137,7 @@ __device__ inline void p2g_finish(const Geo& g, const CellDev& C, int c, const d
 #pragma unroll
       for (int u = 0; u < U; u++)
 #pragma unroll
-        for (int b = 0; b < B; b++) C.coef[CIDX(ss, u, 1 + b)] = cs[u * B + b] / (mass_t[ss] + 1e-20);
+        for (int b = 0; b < B; b++) C.coef[CIDX(ss, u, 1 + b)] = keep_tail ? cs[u * B + b] / (mass_t[ss] + 1e-20) : 0.0;
 #pragma unroll
       for (int a = 0; a < 3; a++) cv[a] = rv[a] / rho;
       double sq = 0;
@@ -255,8 +271,11 @@ __global__ void __launch_bounds__(64) k_p2g_appear(Geo g, CellDev C, PtlDev P, P
   }
   double pav[PGDG_MAX_PHASE][U];
   double cf[PGDG_MAX_PHASE][U][B];
+  double qav[PGDG_MAX_PHASE][ORD == 2 ? U : 1];
 #pragma unroll
   for (int s = 0; s < PGDG_MAX_PHASE; s++) {
+#pragma unroll
+    for (int u = 0; u < (ORD == 2 ? U : 1); u++) qav[s][u] = 0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
       pav[s][u] = 0;
@@ -292,6 +311,9 @@ __global__ void __launch_bounds__(64) k_p2g_appear(Geo g, CellDev C, PtlDev P, P
 #pragma unroll
           for (int a = 0; a < D; a++) pj[1 + a] = P.rhov[a * cap + p];
           pj[U - 1] = P.E[p];
+          double pq[ORD == 2 ? U : 1];
+#pragma unroll
+          for (int u = 0; u < (ORD == 2 ? U : 1); u++) pq[u] = pj[u];
           // the particle's polynomial at the cell centre
           const double nx[3] = {-xij[0], -xij[1], D == 3 ? -xij[2] : 0.0};
           double psi[B];
@@ -312,6 +334,9 @@ __global__ void __launch_bounds__(64) k_p2g_appear(Geo g, CellDev C, PtlDev P, P
             if (s < g.F && ph == s) {
 #pragma unroll
               for (int u = 0; u < U; u++) pav[s][u] += pj[u] * ww;
+              if constexpr (ORD == 2)
+#pragma unroll
+                for (int u = 0; u < U; u++) qav[s][u] += pq[u] * ww;
 #pragma unroll
               for (int u = 0; u < U; u++)
 #pragma unroll
@@ -320,7 +345,7 @@ __global__ void __launch_bounds__(64) k_p2g_appear(Geo g, CellDev C, PtlDev P, P
           }
         }
       }
-  p2g_finish<D, ORD, true>(g, C, c, mass_t, &pav[0][0], &cf[0][0][0]);
+  p2g_finish<D, ORD, true>(g, C, c, mass_t, &pav[0][0], &cf[0][0][0], &qav[0][0]);
 }
 
 // ---------------------------------------------------------------------------

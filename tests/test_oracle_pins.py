@@ -628,3 +628,42 @@ def test_per_cell_bounds_hold_after_every_pass(lo, hi):
     # the bounds refuse some splits and merges of this state
     assert 0 < res[True][0] < res[False][0], res
     assert 0 < res[True][1] < res[False][1], res
+
+
+# ---------------------------------------------------------------------------
+# P2 appearance positivity (no reference routine)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("cxx,fallback", [(-4000.0, True), (-1.0, False)])
+def test_p2_appearance_falls_back_to_particle_average(cxx, fallback):
+    # every phase appears in the first MLS_P2G (ph_state_b = 0); with a large
+    # negative curvature of the particles' density copies the quadratic
+    # extrapolation to the cell centre is negative, and the cell takes the
+    # plain weighted average of the particles' own values with zero slopes
+    case, st = cases.triple_point_2d(nx=8, ny=4, ppc_axis=3, jitter=0.2,
+                                     scheme=dict(order=2, rk_stages=3))
+    n, U, M = st.n, case.U, case.M
+    coef = st.p["coef"].reshape(U, M, -1)
+    coef[:, :, :n] = 0.0
+    coef[0, 4, :n] = cxx  # x^2 mode of the density row
+    o = OracleSim(case, st)
+    o.call("used_state_update")
+    o.call("bin")
+    o.call("membership")
+    o.call("p2g")
+    out = st.copy()
+    o.save(out)
+    c = out.c
+    ph = c["ph_state"][0] == 1
+    assert ph.sum() > 10
+    tails = c["coef"][0, :, 1:, :][:, :, ph]
+    rho = c["coef"][0, 0, 0, ph]
+    # phase 1 carries rho = 1 in the left block and bottom right
+    if fallback:
+        assert np.all(tails == 0.0)
+        assert np.all(rho > 0)
+        # the plain average of particle densities 1 (and 1 where mixed with
+        # nothing else of phase 1)
+        assert np.allclose(rho, 1.0, rtol=0, atol=1e-12)
+    else:
+        assert np.any(tails != 0.0)
+        assert np.all(rho > 0) and not np.allclose(rho, 1.0, rtol=0, atol=1e-12)

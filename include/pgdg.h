@@ -416,6 +416,14 @@ int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
  * the first RK stage after pgdg_upload_state or pgdg_checkpoint_read (the
  * reference's lists are empty after its set-up).  n_neighbor: host [n].  */
 int pgdg_neighbor_counts(pgdg_ctx* ctx, int32_t* n_neighbor);
+/* pgdg_neighbor_counts without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * counts are staged in a fixed device array of 2^21 int32 (8 MiB) per GPU,
+ * held until the process ends; PGDG_ERR_CAPACITY when the context holds
+ * more than 2^21 fluid particles.  Calls that overlap on one GPU share the
+ * array and race.                                                         */
+int pgdg_neighbor_counts_noalloc(pgdg_ctx* ctx, int32_t* n_neighbor);
 /* Cells in use (Used_Ele_cen, S3:mpm.cpp:184-263): host [nc], 1 or 0.      */
 int pgdg_used_cells(pgdg_ctx* ctx, int32_t* used);
 /* Fluid particles in each cell's list (the cell lists of the last binning

@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -937,9 +938,15 @@ int pgdg_autotune_enable(pgdg_ctx* x, int on) {
   return PGDG_OK;
 }
 
-int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
-  if (!x || !n_neighbor) return PGDG_ERR_ARG;
-  Ctx& c = x->c;
+}  // extern "C"
+
+namespace {
+
+// Neighbour-list sizes counted into the device buffer d (np entries, every
+// one written) and copied to the host.  Returns 1 in *done when the context
+// answered without counting (no particles, no lists yet).
+int neighbor_counts_prepare(Ctx& c, int32_t* n_neighbor, int* done) {
+  *done = 1;
   if (!c.loaded) return PGDG_ERR_STATE;
   if (c.np == 0) return PGDG_OK;
   if (!c.lists_built) {
@@ -947,9 +954,12 @@ int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
     return PGDG_OK;
   }
   if (!c.bins_valid) return PGDG_ERR_STATE;
+  *done = 0;
+  return PGDG_OK;
+}
+
+int neighbor_counts_into(Ctx& c, int32_t* d, int32_t* n_neighbor) {
   PGDG_TRY(c.sync_counts());
-  int32_t* d = nullptr;
-  PGDG_HIP_TRY(hipMalloc((void**)&d, sizeof(int32_t) * (size_t)c.np));
   int rc = c.D == 2 ? pgdg::launch_neighbor_counts<2>(c.g, c.PA.view(), c.W, c.S, c.B, c.T.n_ent, c.T.bin_off,
                                                        c.T.bin_ent, c.sc, c.st, d)
                     : pgdg::launch_neighbor_counts<3>(c.g, c.PA.view(), c.W, c.S, c.B, c.T.n_ent, c.T.bin_off,
@@ -958,8 +968,69 @@ int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
                                       c.st) != hipSuccess)
     rc = PGDG_ERR_HIP;
   if (rc == PGDG_OK && hipStreamSynchronize(c.st) != hipSuccess) rc = PGDG_ERR_HIP;
+  return rc;
+}
+
+// Scratch of pgdg_neighbor_counts_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code uses the address
+// hipGetSymbolAddress returns for the current device, never the array's
+// name.  Calls that overlap on one GPU share it and race.
+constexpr int64_t kNoAllocMaxParticles = (int64_t)1 << 21;  // 8 MiB of int32
+__device__ __align__(256) int32_t scratch[kNoAllocMaxParticles];
+constexpr int kMaxDevices = 64;
+int32_t* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(int32_t** out) {
+  int dev = 0;
+  PGDG_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return PGDG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    PGDG_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return PGDG_ERR_HIP;
+    g_scratch[dev] = static_cast<int32_t*>(p);
+  }
+  *out = g_scratch[dev];
+  return PGDG_OK;
+}
+
+__global__ void k_zero_i32(int32_t* a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgdg_neighbor_counts(pgdg_ctx* x, int32_t* n_neighbor) {
+  if (!x || !n_neighbor) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  int done = 0;
+  PGDG_TRY(neighbor_counts_prepare(c, n_neighbor, &done));
+  if (done) return PGDG_OK;
+  int32_t* d = nullptr;
+  PGDG_HIP_TRY(hipMalloc((void**)&d, sizeof(int32_t) * (size_t)c.np));
+  const int rc = neighbor_counts_into(c, d, n_neighbor);
   (void)hipFree(d);
   return rc;
+}
+
+int pgdg_neighbor_counts_noalloc(pgdg_ctx* x, int32_t* n_neighbor) {
+  if (!x || !n_neighbor) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  if (c.np > kNoAllocMaxParticles) return PGDG_ERR_CAPACITY;
+  int done = 0;
+  PGDG_TRY(neighbor_counts_prepare(c, n_neighbor, &done));
+  if (done) return PGDG_OK;
+  int32_t* d = nullptr;
+  PGDG_TRY(scratch_ptr(&d));
+  // the previous call's counts are still there: start from zero
+  hipLaunchKernelGGL(k_zero_i32, dim3(256), dim3(256), 0, c.st, d, (int64_t)c.np);
+  PGDG_HIP_TRY(hipGetLastError());
+  return neighbor_counts_into(c, d, n_neighbor);
 }
 
 int pgdg_cell_particle_counts(pgdg_ctx* x, int32_t* counts) {

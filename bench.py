@@ -266,20 +266,28 @@ def triple_point_bench(steps=100, warmup=2, cpu_steps=10):
     return res
 
 
-def error_horizon(size=(64, 32, 32), steps=200, every=50, scheme=None):
+def error_horizon(size=(64, 32, 32), steps=200, every=50, scheme=None, perturb=1e-10):
     """FP64 and mixed forms of the shock-bubble step side by side from the
     same initial state for `steps` steps on a reduced grid: relative L1 and
     Linf of the cell density (alpha-weighted over phases) every `every`
-    steps, and the particle counts."""
+    steps, and the particle counts.  A third run, the FP64 form from the
+    same state with every particle's density, energy and mass scaled by
+    1 + perturb * U(-1, 1) (seeded), shows how far the method itself
+    amplifies a perturbation of that size over the same steps."""
     import numpy as np
     from pgdg_p2_amd import cases
     from pgdg_p2_amd.step import GpuSim
     nx, ny, nz = size
     sims = []
-    for prec in (0, 1):
+    for prec, pert in ((0, 0.0), (1, 0.0), (0, perturb)):
         case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25,
                                          scheme=scheme)
         case.precision = prec
+        if pert > 0:
+            n = st.n
+            f = 1.0 + pert * np.random.Generator(np.random.PCG64(0x5EED00A0)).uniform(-1, 1, n)
+            for k in ("rho", "E", "mass"):
+                st.p[k][:n] *= f
         sims.append((GpuSim(case, st), st.copy()))
     rows = []
     t0 = time.perf_counter()
@@ -290,13 +298,18 @@ def error_horizon(size=(64, 32, 32), steps=200, every=50, scheme=None):
             g.save_cells(st)
             rho.append((st.c["alph"] * st.c["rho"]).sum(axis=0))
         d = np.abs(rho[1] - rho[0])
+        dp = np.abs(rho[2] - rho[0])
         rows.append({"step": k, "density_L1_rel": float(d.sum() / np.abs(rho[0]).sum()),
                      "density_Linf_rel": float(d.max() / np.abs(rho[0]).max()),
+                     "fp64_perturbed_L1_rel": float(dp.sum() / np.abs(rho[0]).sum()),
+                     "fp64_perturbed_Linf_rel": float(dp.max() / np.abs(rho[0]).max()),
                      "particles": [g.count() for g, _ in sims]})
         print(f"[horizon] step {k}: L1 {rows[-1]['density_L1_rel']:.3g}", file=sys.stderr, flush=True)
     for g, _ in sims:
         g.close()
-    return {"grid": f"{nx}x{ny}x{nz}", "steps": steps, "rows": rows,
+    last_ok = max([r["step"] for r in rows if r["density_L1_rel"] <= 1e-6], default=0)
+    return {"grid": f"{nx}x{ny}x{nz}", "steps": steps, "rows": rows, "perturbation": perturb,
+            "last_row_with_L1_le_1e-6": last_ok,
             "wall_s": time.perf_counter() - t0,
             "L1_le_1e-6_at_every_row": all(r["density_L1_rel"] <= 1e-6 for r in rows)}
 
@@ -592,7 +605,8 @@ def main():
         "stage_graphs": "RK stages replayed as HIP graphs in the timed steps",
     }
     if not args.no_horizon:
-        line["error_horizon"] = error_horizon(scheme=SCHEMES[args.scheme])
+        line["error_horizon"] = [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
+                                 for sz in ((64, 32, 32), (128, 64, 64))]
     if not args.no_tp:
         line["triple_point_112x48"] = triple_point_bench()
     if not args.no_mgfm:

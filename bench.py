@@ -441,6 +441,7 @@ def main():
     ap.add_argument("--no-horizon", action="store_true", help="skip the 200-step error horizon")
     ap.add_argument("--3phase-warmup", dest="tp3_warmup", type=int, default=500)
     ap.add_argument("--only-3phase", action="store_true")
+    ap.add_argument("--only-horizon", action="store_true")
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--only-mgfm", action="store_true")
@@ -472,6 +473,10 @@ def main():
         return
     if args.only_mgfm:
         print(json.dumps({"mgfm": mgfm_bench()}))
+        return
+    if args.only_horizon:
+        print(json.dumps({"error_horizon": [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
+                                            for sz in ((64, 32, 32), (128, 64, 64))]}))
         return
     if args.only_3phase:
         print(json.dumps({"three_phase_2d": three_phase_bench(warmup=args.tp3_warmup)}))

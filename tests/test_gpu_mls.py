@@ -127,3 +127,24 @@ def test_mls_empty_neighbourhood(cuda):
     np.testing.assert_array_equal(coef, ref)
     np.testing.assert_allclose(cm, ref, rtol=1e-6, atol=0)
 
+
+
+@pytest.mark.gpu
+def test_mls_mixed_error_budget_128(cuda):
+    # the isolated-MLS workload at its full size, 128^3 cells: the mixed
+    # form's centre values against the FP64 form's, per field
+    from pgdg_p2_amd import mls
+    d = workloads.mls_particles(n=128)
+    print(f"mls mixed n=128 np={d['x'].shape[1]}", flush=True)
+    f = mls.MlsFit(d)
+    del d
+    c64, o64 = f.run(mls.PREC_FP64)
+    c32, o32 = f.run(mls.PREC_MIXED)
+    f.close()
+    assert (o32 != o64).sum() == 0
+    v64, v32 = mls.center_values(c64), mls.center_values(c32)
+    for fld in range(v64.shape[0]):
+        l1 = np.abs(v32[fld] - v64[fld]).mean() / np.abs(v64[fld]).mean()
+        linf = np.abs(v32[fld] - v64[fld]).max() / np.abs(v64[fld]).max()
+        print(f"field {fld}: mixed vs fp64 centre values L1 {l1:.2e} Linf {linf:.2e}", flush=True)
+        assert l1 <= 1e-6 and linf <= 1e-5

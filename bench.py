@@ -468,6 +468,8 @@ def main():
         b.free()
         print(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr,
               flush=True)
+    if args.profile_steps < 1:
+        ap.error("--profile-steps must be at least 1 (the per-stage times and the roofline come from them)")
     if args.only_mls:
         print(json.dumps({"mls": mls_bench(args.mls_n)}))
         return

@@ -403,6 +403,16 @@ int pgdg_tuning_read(pgdg_ctx* ctx, int* blocks);
  * on) or let every lane read its candidates from global memory.  Both give
  * the same sums bit for bit; the switch exists to compare them.           */
 int pgdg_pst_staging_enable(pgdg_ctx* ctx, int on);
+/* Mixed form, 3-D P2: the DG right-hand side as 16x16 matrix products on
+ * the matrix cores (k_rhs_mfma) or one lane per cell on the vector ALUs
+ * (default).  The two sum the same point terms in different orders.  No
+ * counterpart in the reference (whose right-hand side is FP64,
+ * S3:particle_flow.cpp:16-405).  Off by default: both kernels move about
+ * 3.5 KB per cell through HBM (face fluxes, coefficients, the fused stage
+ * update), and the matrix-core kernel keeps three wavefronts per SIMD to
+ * hide it against the vector kernel's more (6.0 against 4.5 ms per stage
+ * at 256x128x128 on MI355X).                                             */
+int pgdg_rhs_mfma_enable(pgdg_ctx* ctx, int on);
 int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 
 /* ======================================================================== *

@@ -26,6 +26,7 @@ def declare(L: C.CDLL) -> None:
     L.pgdg_graphs_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_autotune_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_pst_staging_enable.argtypes = [_CTX, C.c_int]
+    L.pgdg_rhs_mfma_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_neighbor_counts.argtypes = [_CTX, C.POINTER(C.c_int32)]
     L.pgdg_neighbor_counts_noalloc.argtypes = [_CTX, C.POINTER(C.c_int32)]
     L.pgdg_used_cells.argtypes = [_CTX, C.POINTER(C.c_int32)]

@@ -210,4 +210,139 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
   }
 }
 
+// Flux of a domain-boundary face point in the float form: the wall
+// pressure, or HLL against the inlet / outlet / symmetry ghost state (the
+// boundary branches of the face integral, S3:particle_flow.cpp:290-394).
+// (j, k): the cell's indices, for the symmetry reflections.
+__device__ inline void bface_flux3(const Geo& g, const EosT<float>& mt, int fb, int j, int k, int ax, float sgn,
+                                   const float* Ui, float* Fh) {
+  constexpr int D = 3, U = 5;
+  const bool wall_like = g.sem == PGDG_SEM_S3 ? (fb == 1) : (fb != 0);
+  float nrm[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) nrm[a] = a == ax ? sgn : 0.f;
+  const float rho_i = Ui[0], E_i = Ui[U - 1];
+  float rvi[3], vi[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) rvi[a] = Ui[1 + a];
+#pragma unroll
+  for (int a = 0; a < D; a++) vi[a] = qdiv(rvi[a], rho_i);
+  float sq = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) sq += rvi[a] * rvi[a];
+  const float e_i = E_i - qdiv(0.5f * sq, rho_i);
+  const float P_i = mt.p(e_i);
+  if (wall_like) {
+    Fh[0] = Fh[U - 1] = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; a++) Fh[1 + a] = a == ax ? P_i : 0.f;
+    return;
+  }
+  const float ci = qsqrt(qdiv(mt.g * (mt.pinf + P_i), rho_i));
+  float vni = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) vni += vi[a] * nrm[a];
+  float DFi[U];
+  flux_col<D>(rho_i, rvi, E_i, vi, P_i, ax, DFi);
+  float rho_j = rho_i, P_j = P_i, E_j = E_i;
+  float vj[3], rvj[3];
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    vj[a] = vi[a];
+    rvj[a] = rho_j * vj[a];
+  }
+  if (fb == -1) {
+    rho_j = (float)g.inlet_rho;
+#pragma unroll
+    for (int a = 0; a < D; a++) vj[a] = (float)g.inlet_u[a];
+#pragma unroll
+    for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+    P_j = (float)g.inlet_p;
+    float sqv = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) sqv += vj[a] * vj[a];
+    E_j = mt.rhoe(P_j) + 0.5f * sqv * rho_j;
+  }
+  if (j == g.N[1] - 1 || j == 0) {
+    vj[1] = -vi[1];
+#pragma unroll
+    for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+  }
+  if (k == g.N[2] - 1 || k == 0) {
+    vj[2] = -vi[2];
+#pragma unroll
+    for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
+  }
+  float Ujs[U];
+  Ujs[0] = rho_j;
+#pragma unroll
+  for (int a = 0; a < D; a++) Ujs[1 + a] = rvj[a];
+  Ujs[U - 1] = E_j;
+  const float cj = qsqrt(qdiv(mt.g * (mt.pinf + P_j), rho_j));
+  float vnj = 0;
+#pragma unroll
+  for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
+  float DFj[U];
+  flux_col<D>(rho_j, rvj, E_j, vj, P_j, ax, DFj);
+  const float SL = amin1(vni - ci, vnj - cj);
+  const float SR = amax1(vni + ci, vnj + cj);
+  if (SL > 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = DFi[u];
+  } else if (SL <= 0 && SR >= 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = qdiv(SR * DFi[u] - SL * DFj[u] + SL * SR * (Ujs[u] - Ui[u]) * sgn, SR - SL);
+  } else if (SR < 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = DFj[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; u++) Fh[u] = 0;
+  }
+}
+
+// Face side f of a cell (the reference order yL, xR, yR, xL, zL, zR):
+// normal axis and sign, and the neighbour's index offsets.
+__device__ inline void side_of(int f, int& ax, float& sgn) {
+  ax = (f == 1 || f == 3) ? 0 : (f < 4 ? 1 : 2);
+  sgn = (f == 1 || f == 2 || f == 5) ? 1.f : -1.f;
+}
+
+// Face point fp = f * NQ^2 + q of a 3-D cell (side f, canonical point q):
+// offset r from the cell centre and the weight w S (-n) of the point in the
+// face integral.
+template <int ORD>
+__device__ inline void face_point_cell3(const Geo& g, int fp, float* r, float& wq) {
+  constexpr int NQ = ORD + 1;
+  const int f = fp / (NQ * NQ), q = fp % (NQ * NQ);
+  int ax;
+  float sgn;
+  side_of(f, ax, sgn);
+  const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
+  double off[3], gwd;
+  face_point<3, ORD>(g, ax, q, hp, off, gwd);
+  off[ax] += (double)sgn * hp[ax];
+#pragma unroll
+  for (int a = 0; a < 3; a++) r[a] = (float)off[a];
+  const double S = ax == 0 ? g.ps[1] * g.ps[2] : (ax == 1 ? g.ps[0] * g.ps[2] : g.ps[0] * g.ps[1]);
+  wq = (float)gwd * (float)(-(double)sgn * S);
+}
+
+// Boundary-face point fluxes of the mixed form, 3-D: plane index of side f
+// of cell (i, j, k) (the two tangential cell indices), the plane stride
+// (the largest of the three plane sizes) and the slot of phase ss, side f,
+// point q, variable u.
+__device__ inline int bplane_index(const Geo& g, int f, int i, int j, int k) {
+  return (f == 1 || f == 3) ? j * g.N[2] + k : (f < 4 ? i * g.N[2] + k : i * g.N[1] + j);
+}
+__device__ inline int bplane_stride(const Geo& g) {
+  const int a = g.N[1] * g.N[2], b = g.N[0] * g.N[2], c = g.N[0] * g.N[1];
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+template <int ORD>
+__device__ inline size_t bflux_slot(const Geo& g, int ss, int f, int q, int u, int pidx) {
+  constexpr int NQF = (ORD + 1) * (ORD + 1);
+  return ((((size_t)ss * 6 + f) * NQF + q) * 5 + u) * (size_t)bplane_stride(g) + pidx;
+}
+
 }  // namespace pgdg

@@ -13,6 +13,7 @@
 // too (Topo); the host reads them back once per step, after the last merge
 // pass, when the removed particles are compacted, and once more at the end
 // of the step to size the neighbour-list pool.
+#include <algorithm>
 #include <stdio.h>
 #include <string.h>
 
@@ -243,6 +244,7 @@ static void fill_geo(const pgdg_case& cs, Geo& g) {
   g.radius = (int)std::ceil(kCellRatio / (cs.p_g / cs.p_sz));  // S3:level_infor.cpp:155
   g.ord = cs.order;
   g.prec = cs.precision;
+  g.mfma_rhs = 0;
   if (cs.order == 1) {
     g.Gp[0] = -1. / std::sqrt(3.);  // S3:mpm_ini.cpp:37-41
     g.Gp[1] = 1. / std::sqrt(3.);
@@ -317,10 +319,17 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
     // face list [D][nc] and its count
     const int nq = cs.order + 1, nqf = D == 3 ? nq * nq : nq;
     const size_t nflux = (size_t)2 * F * nqf * U * D * nc;
-    ff_bytes = sizeof(float) * nflux + sizeof(int32_t) * ((size_t)D * nc + 64);
+    // 3-D: boundary-face point fluxes [F][6][NQF][U][plane], plane the
+    // largest of the three face planes of the grid
+    const size_t plane = D == 3 ? std::max((size_t)cs.n[1] * cs.n[2],
+                                           std::max((size_t)cs.n[0] * cs.n[2], (size_t)cs.n[0] * cs.n[1]))
+                                : 0;
+    const size_t nbflux = (size_t)F * 6 * nqf * U * plane;
+    ff_bytes = sizeof(float) * (nflux + nbflux) + sizeof(int32_t) * ((size_t)D * nc + 64);
     PGDG_HIP_TRY(hipMalloc(&ff_slab, ff_bytes));
     C.fflux = (float*)ff_slab;
-    C.flist = (int32_t*)(C.fflux + nflux);
+    C.bflux = D == 3 ? C.fflux + nflux : nullptr;
+    C.flist = (int32_t*)(C.fflux + nflux + nbflux);
     C.fcount = C.flist + (size_t)D * nc;
   }
   // stage buffers: xb[3], hb, x0[3], vol0
@@ -1233,6 +1242,13 @@ int pgdg_checkpoint_read(pgdg_ctx* x, const char* path) {
 int pgdg_pst_staging_enable(pgdg_ctx* x, int on) {
   if (!x) return PGDG_ERR_ARG;
   x->c.pst_staging = on != 0;
+  x->c.drop_graphs();
+  return PGDG_OK;
+}
+
+int pgdg_rhs_mfma_enable(pgdg_ctx* x, int on) {
+  if (!x) return PGDG_ERR_ARG;
+  x->c.g.mfma_rhs = on != 0;
   x->c.drop_graphs();
   return PGDG_OK;
 }

@@ -49,6 +49,7 @@ struct Geo {
   int ord;              // 1: P1 basis, 2: P2 basis
   int prec;             // PGDG_PREC_FP64 or PGDG_PREC_MIXED
   int ppc_min, ppc_max; // per-cell particle-count bounds of split/merge (0: off)
+  int mfma_rhs;         // mixed form, 3-D P2: RHS on the matrix cores (k_rhs_mfma), 0 by default
 };
 
 // Modes per cell and Gauss points per axis.
@@ -121,6 +122,8 @@ struct CellDev {
   // general face kernel and their count
   float* fflux;
   int32_t *flist, *fcount;
+  // mixed form, 3-D: domain-boundary face point fluxes [F][6][NQF][U][plane]
+  float* bflux;
 };
 
 struct WallDev {

@@ -731,7 +731,10 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
           for (int u = 0; u < U; u++) {
             T acc = 0;
 #pragma unroll
-            for (int b = 0; b < B; b++) acc += cf[u][1 + b] * phv[b];
+            for (int b = 0; b < B; b++) {
+              if constexpr (kFaceBuf) acc = fmaf(cf[u][1 + b], phv[b], acc);
+              else acc += cf[u][1 + b] * phv[b];
+            }
             Up[u] = cf[u][0] + lm[u] * acc;
           }
           T rv[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -747,6 +750,35 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
           T col[D][U];
 #pragma unroll
           for (int b = 0; b < D; b++) flux_col<D>(Up[0], rv, Up[U - 1], v, P_p, b, col[b]);
+          if constexpr (kFaceBuf) {
+            // float form: the point weight once, fused products
+            const T wv = gw * voli;
+#pragma unroll
+            for (int bi = 1; bi <= D; bi++)
+#pragma unroll
+              for (int u = 0; u < U; u++) dc[u][bi] = fmaf(col[bi - 1][u], wv, dc[u][bi]);
+            if constexpr (ORD == 2) {
+              const T x2[3] = {T(2.) * dpT[0], T(2.) * dpT[1], T(2.) * dpT[2]};
+#pragma unroll
+              for (int u = 0; u < U; u++) {
+                T c0 = col[0][u] * wv, c1 = col[1][u] * wv;
+                if constexpr (D == 2) {
+                  dc[u][3] = fmaf(c0, dpT[1], fmaf(c1, dpT[0], dc[u][3]));
+                  dc[u][4] = fmaf(c0, x2[0], dc[u][4]);
+                  dc[u][5] = fmaf(c1, x2[1], dc[u][5]);
+                } else {
+                  T c2 = col[2][u] * wv;
+                  dc[u][4] = fmaf(c0, dpT[1], fmaf(c1, dpT[0], dc[u][4]));
+                  dc[u][5] = fmaf(c0, dpT[2], fmaf(c2, dpT[0], dc[u][5]));
+                  dc[u][6] = fmaf(c1, dpT[2], fmaf(c2, dpT[1], dc[u][6]));
+                  dc[u][7] = fmaf(c0, x2[0], dc[u][7]);
+                  dc[u][8] = fmaf(c1, x2[1], dc[u][8]);
+                  dc[u][9] = fmaf(c2, x2[2], dc[u][9]);
+                }
+              }
+            }
+            return;
+          }
 #pragma unroll
           for (int bi = 1; bi <= D; bi++)
 #pragma unroll
@@ -817,9 +849,9 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
             const T wq = (T)gwd * fs;
 #pragma unroll
             for (int u = 0; u < U; u++) {
-              const T Fh = C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, fc)];
+              const T Fw = C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, fc)] * wq;
 #pragma unroll
-              for (int bi = 0; bi < M; bi++) dc[u][bi] += Fh * Nb[bi] * wq;
+              for (int bi = 0; bi < M; bi++) dc[u][bi] = fmaf(Fw, Nb[bi], dc[u][bi]);
             }
           }
           continue;
@@ -1149,6 +1181,273 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-form DG right-hand side on the matrix cores (3-D, P2 basis)
+// ---------------------------------------------------------------------------
+typedef float rhs_f32x4 __attribute__((ext_vector_type(4)));
+
+// One wavefront takes 64 consecutive cells as four blocks of 16; within a
+// block every step is a 16x16 matrix product over the block's cells
+// (v_mfma_f32_16x16x4_f32, lane l holding row l % 16 / column l % 16 and
+// reduction index l / 16 of the operands, and entries (4 (l / 16) + r,
+// l % 16) of the result):
+//   states  U(point, cell) = sum_m Phi(point, m) Cf(m, cell), Cf the
+//           coefficients with the limiter applied to the slopes, Phi the
+//           basis at the 27 volume Gauss points;
+//   volume  R(m, cell) += sum_b sum_point w dPhi_m/dx_b(point) F_b(U(point));
+//   faces   R(m, cell) += sum_point w S (-n) Phi_m(point) Fh(point) over the
+//           54 face points, Fh the point fluxes of the face kernels
+//           (interior faces: k_face_simple / k_face_general; domain-boundary
+//           faces: k_face_boundary3), loaded one block of 16 points ahead.
+// The volume products take the flux columns straight from the registers the
+// state product left them in: the reduction index of reduction step t in
+// lane l is point 4 (l / 16) + t, which is the point of result entry r = t
+// of that lane, and the constant operand is laid out in that order.  The
+// rates are then divided by the mass matrix and stored, and the stage's
+// coefficient update applied, from the result layout (16 consecutive
+// cells per mode).  The point fluxes are those of k_rhs<float>; the sums
+// run in another order.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_rhs_mfma(Geo g, CellDev C, const int32_t* __restrict__ used,
+                                                 const DevScalars* __restrict__ sc, int md) {
+  constexpr int D = 3, U = 5, M = 10, NQ = 3;
+  const int nc = g.nc;
+  // constant operands of every lane position, shared by the workgroup's
+  // wavefronts: [0, 6) states at the volume points (pb * 3 + s), [6, 30)
+  // volume weights ((b * 2 + pb) * 4 + t), [30, 46) face weights (pb * 4 + t)
+  __shared__ float s_A[46][64];
+  const int lane = threadIdx.x & 63, grp = lane >> 4, col = lane & 15;
+  const int cbase = ((int)xcd_block() * 4 + (int)(threadIdx.x >> 6)) * 64;
+  const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
+  const double vol64 = g.ps[0] * g.ps[1] * g.ps[2];
+  const float voli = (float)vol64;
+  // basis (1, tail) at offset r from the cell centre
+  auto phi_at = [&](const float* r, int m) -> float {
+    float t[M - 1];
+    basis_tail<D, 2>(r, g.q12, t);
+    float v = 1.f;
+#pragma unroll
+    for (int q = 0; q < M - 1; q++)
+      if (m == q + 1) v = t[q];
+    return m < M ? v : 0.f;
+  };
+  auto vol_point = [&](int p, float* r, float& gw) {  // p = (g1 * 3 + g2) * 3 + g3
+    const int g1 = p / 9, g2 = (p / 3) % 3, g3 = p % 3;
+    r[0] = (float)(g.Gp[g1] * hp[0]);
+    r[1] = (float)(g.Gp[g2] * hp[1]);
+    r[2] = (float)(g.Gp[g3] * hp[2]);
+    gw = (float)(g.Gw[g1] * g.Gw[g2] * g.Gw[g3]);
+  };
+  // derivative of basis mode m along b at offset r
+  auto dphi = [&](int m, int b, const float* r) -> float {
+    if (m >= 1 && m <= 3) return m - 1 == b ? 1.f : 0.f;
+    switch (m) {
+      case 4: return b == 0 ? r[1] : (b == 1 ? r[0] : 0.f);
+      case 5: return b == 0 ? r[2] : (b == 2 ? r[0] : 0.f);
+      case 6: return b == 1 ? r[2] : (b == 2 ? r[1] : 0.f);
+      case 7: return b == 0 ? 2.f * r[0] : 0.f;
+      case 8: return b == 1 ? 2.f * r[1] : 0.f;
+      case 9: return b == 2 ? 2.f * r[2] : 0.f;
+      default: return 0.f;
+    }
+  };
+  // constant operands: Phi(point pb * 16 + col, mode 4 s + grp); the
+  // volume weight w dPhi_col/dx_b(point pb * 16 + 4 grp + t) vol; the face
+  // weight w S (-n) Phi_col(face point pb * 16 + 4 grp + t)
+  if (threadIdx.x < 64) {
+#pragma unroll 1
+    for (int pb = 0; pb < 2; pb++) {
+      const int p = pb * 16 + col;
+      float r[3], gw;
+      vol_point(p < 27 ? p : 0, r, gw);
+#pragma unroll
+      for (int s = 0; s < 3; s++) s_A[pb * 3 + s][lane] = p < 27 ? phi_at(r, 4 * s + grp) : 0.f;
+#pragma unroll 1
+      for (int t = 0; t < 4; t++) {
+        const int pp = pb * 16 + 4 * grp + t;
+        float rr[3], w;
+        vol_point(pp < 27 ? pp : 0, rr, w);
+#pragma unroll
+        for (int b = 0; b < 3; b++) s_A[6 + (b * 2 + pb) * 4 + t][lane] = pp < 27 ? (dphi(col, b, rr) * w) * voli : 0.f;
+      }
+    }
+#pragma unroll 1
+    for (int q = 0; q < 16; q++) {
+      const int fp = (q >> 2) * 16 + 4 * grp + (q & 3);
+      float r[3], wq;
+      face_point_cell3<2>(g, fp < 54 ? fp : 0, r, wq);
+      s_A[30 + q][lane] = fp < 54 ? phi_at(r, col) * wq : 0.f;
+    }
+  }
+  __syncthreads();
+  if (cbase >= nc) return;  // the whole wavefront (no barrier follows)
+  // inverse diagonal mass matrix of the result rows m = 4 grp + r
+  double minv[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int m = 4 * grp + r;
+    double mij = 1.;
+    if (m >= 1 && m <= 3) mij = hp[m - 1] * hp[m - 1] / 3.;
+    else if (m >= 4 && m <= 6) {
+      const double t = m == 4 ? hp[0] * hp[1] : (m == 5 ? hp[0] * hp[2] : hp[1] * hp[2]);
+      mij = t * t / 9.;
+    } else if (m >= 7 && m <= 9) {
+      const double t = hp[m - 7] * hp[m - 7];
+      mij = t * t * 4. / 45.;
+    }
+    minv[r] = 1. / (mij * vol64 + 1e-20);
+  }
+  const double dt = sc->dt;
+  const double a0 = md == 0 ? 0. : md == 1 ? 0.5 : md == 2 ? 0.75 : 1. / 3.;
+  const double a1 = md == 0 ? 1. : md == 1 ? 0.5 : md == 2 ? 0.25 : 2. / 3.;
+
+  for (int blk = 0; blk < 4; blk++) {
+    const int c0 = cbase + blk * 16;
+    if (c0 >= nc) break;  // uniform
+    const int c = c0 + col;
+    const bool in_range = c < nc;
+    const bool use_c = in_range && used[c] != 0;
+    int ci, cj, ck;
+    cell_ijk(g, in_range ? c : c0, ci, cj, ck);
+    for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
+      if (ss >= g.F) break;
+      const bool valid = use_c && C.ph_state[ss * nc + c] == 1;
+      if (__ballot(valid) == 0) continue;  // no cell of the block holds the phase
+      const EosT<float> mt(Eos{g.gamma[ss], g.pinf[ss]});
+      // coefficients: mode 4s + grp of cell col (a placeholder state, unit
+      // density and energy, where the cell does not hold the phase)
+      float Bv[U][3];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const float lm = valid ? (float)C.lim[(ss * U + u) * nc + c] : 0.f;
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+          const int m = 4 * s + grp;
+          float v = (m == 0 && (u == 0 || u == U - 1)) ? 1.f : 0.f;
+          if (valid && m < M) {
+            const float cf = (float)C.coef[(((size_t)ss * U + u) * M + m) * nc + c];
+            v = m == 0 ? cf : lm * cf;
+          }
+          Bv[u][s] = v;
+        }
+      }
+      rhs_f32x4 R[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) R[u] = rhs_f32x4{0.f, 0.f, 0.f, 0.f};
+      // face integral: the point fluxes of face block pb (points
+      // pb * 16 + 4 grp + t), interior faces from the face kernels' C.fflux,
+      // domain-boundary faces from k_face_boundary3's C.bflux
+      auto load_faces = [&](int pb, float (&Fo)[4][U]) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int fp = pb * 16 + 4 * grp + t;
+#pragma unroll
+          for (int u = 0; u < U; u++) Fo[t][u] = 0.f;
+          if (fp < 54 && valid) {
+            const int f = fp / (NQ * NQ), q = fp % (NQ * NQ);
+            int ax;
+            float sgn;
+            side_of(f, ax, sgn);
+            const int st = ax == 0 ? g.N[1] * g.N[2] : (ax == 1 ? g.N[2] : 1);
+            const int ia = ax == 0 ? ci : (ax == 1 ? cj : ck);
+            const int na = ia + (sgn > 0 ? 1 : -1);
+            if (na >= 0 && na < g.N[ax]) {
+              const int side = sgn > 0 ? 0 : 1;
+              const int fc = sgn > 0 ? c : c - st;
+#pragma unroll
+              for (int u = 0; u < U; u++) Fo[t][u] = C.fflux[face_slot<D, 2>(g, side, ss, q, u, ax, fc)];
+            } else if (g.bc[f] != 0) {
+              const int pidx = bplane_index(g, f, ci, cj, ck);
+#pragma unroll
+              for (int u = 0; u < U; u++) Fo[t][u] = C.bflux[bflux_slot<2>(g, ss, f, q, u, pidx)];
+            }
+          }
+        }
+      };
+      // volume integral
+#pragma unroll 1
+      for (int pb = 0; pb < 2; pb++) {
+        rhs_f32x4 Up[U];
+        const float Ae0 = s_A[pb * 3][lane], Ae1 = s_A[pb * 3 + 1][lane], Ae2 = s_A[pb * 3 + 2][lane];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          rhs_f32x4 a = {0.f, 0.f, 0.f, 0.f};
+          a = __builtin_amdgcn_mfma_f32_16x16x4f32(Ae0, Bv[u][0], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x4f32(Ae1, Bv[u][1], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x4f32(Ae2, Bv[u][2], a, 0, 0, 0);
+          Up[u] = a;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int p = pb * 16 + 4 * grp + t;
+          float colv[3][U];
+          if (p < 27) {
+            const float rho = Up[0][t], E = Up[U - 1][t];
+            float rv[3], v[3];
+#pragma unroll
+            for (int a = 0; a < D; a++) rv[a] = Up[1 + a][t];
+#pragma unroll
+            for (int a = 0; a < D; a++) v[a] = qdiv(rv[a], rho);
+            float sq = 0;
+#pragma unroll
+            for (int a = 0; a < D; a++) sq += rv[a] * rv[a];
+            const float P = mt.p(E - qdiv(0.5f * sq, rho));
+#pragma unroll
+            for (int b = 0; b < D; b++) flux_col<D>(rho, rv, E, v, P, b, colv[b]);
+          } else {
+#pragma unroll
+            for (int b = 0; b < D; b++)
+#pragma unroll
+              for (int u = 0; u < U; u++) colv[b][u] = 0.f;
+          }
+#pragma unroll
+          for (int b = 0; b < D; b++) {
+            const float Ap = s_A[6 + (b * 2 + pb) * 4 + t][lane];
+#pragma unroll
+            for (int u = 0; u < U; u++) R[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ap, colv[b][u], R[u], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll 1
+      for (int pb = 0; pb < 4; pb++) {
+        float Fc[4][U];
+        load_faces(pb, Fc);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const float Af = s_A[30 + pb * 4 + t][lane];
+#pragma unroll
+          for (int u = 0; u < U; u++) R[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af, Fc[t][u], R[u], 0, 0, 0);
+        }
+      }
+      // rates of modes 4 grp + r of cell col; then the stage's update
+      if (valid) {
+#pragma unroll 1
+        for (int u = 0; u < U; u++) {
+          const double l = md >= 0 ? fabs(C.lim[(ss * U + u) * nc + c]) : 0.;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int m = 4 * grp + r;
+            if (m >= M) continue;
+            const size_t o = (((size_t)ss * U + u) * M + m) * nc + c;
+            double dcf = (double)R[u][r] * minv[r];
+            C.dcoef[o] = dcf;
+            if (md >= 0) {
+              double cf = C.coef[o];
+              if (m >= 1) {
+                dcf *= l;
+                cf *= l;
+              }
+              double nw = a1 * (cf + dcf * dt);
+              if (md == 0) C.coef0[o] = cf;
+              else nw += a0 * C.coef0[o];
+              C.coef[o] = nw;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Update_grid (S3:mpm.cpp:768-830; T2:mpm.cpp:1254-1320)
 // ---------------------------------------------------------------------------
 // md: 0 first stage (Euler), 1 Heun second stage (reference), 2 and 3 the
@@ -1293,6 +1592,11 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
     case CELL_RHS:
       if (g.prec == PGDG_PREC_MIXED) {
         PGDG_TRY(launch_faces<D>(g, C, used, st));
+        if (D == 3 && ORD == 2 && g.mfma_rhs) {
+          hipLaunchKernelGGL(k_rhs_mfma, dim3(grid_xcd(((long long)nc + 63) / 64, 4)), dim3(256), 0, st, g, C,
+                             used, sc, arg);
+          break;
+        }
         switch (block) {
           case 64: hipLaunchKernelGGL((k_rhs<D, ORD, 64, float>), dim3(grid_xcd(nc, 64)), dim3(64), 0, st, g, C, used, sc, arg); break;
           case 128: hipLaunchKernelGGL((k_rhs<D, ORD, 128, float>), dim3(grid_xcd(nc, 128)), dim3(128), 0, st, g, C, used, sc, arg); break;

@@ -12,6 +12,8 @@
 // or a material interface (MGFM ghost states, one-sided) are listed and
 // evaluated per side by a second kernel.  The cell kernel (k_rhs, float)
 // projects the stored point fluxes onto its modes.
+#include <algorithm>
+
 #include "cell_flux.hpp"
 #include "dev_common.hpp"
 #include "hip_util.hpp"
@@ -206,6 +208,79 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
   }
 }
 
+// Domain-boundary faces of the 3-D mixed form (for the matrix-core cell
+// kernel): one lane per boundary face (side f, plane index), every present
+// phase of its cell; the state at each face point from the cell's
+// coefficients (slopes scaled by the limiter), the wall pressure or the HLL
+// flux against the inlet / outlet / symmetry ghost state (bface_flux3), into
+// C.bflux.
+template <int ORD>
+__global__ void __launch_bounds__(256) k_face_boundary3(Geo g, CellDev C, const int32_t* __restrict__ used) {
+  constexpr int D = 3, U = 5, M = Basis<3, ORD>::M, NQF = (ORD + 1) * (ORD + 1);
+  const int nc = g.nc;
+  const int PL = bplane_stride(g);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)6 * PL) return;
+  const int f = (int)(t / PL), pidx = (int)(t % PL);
+  if (g.bc[f] == 0) return;  // this side of the domain has no boundary condition
+  int i, j, k;
+  if (f == 1 || f == 3) {
+    if (pidx >= g.N[1] * g.N[2]) return;
+    i = f == 1 ? g.N[0] - 1 : 0;
+    j = pidx / g.N[2];
+    k = pidx % g.N[2];
+  } else if (f < 4) {
+    if (pidx >= g.N[0] * g.N[2]) return;
+    i = pidx / g.N[2];
+    j = f == 2 ? g.N[1] - 1 : 0;
+    k = pidx % g.N[2];
+  } else {
+    if (pidx >= g.N[0] * g.N[1]) return;
+    i = pidx / g.N[1];
+    j = pidx % g.N[1];
+    k = f == 5 ? g.N[2] - 1 : 0;
+  }
+  const int c = (i * g.N[1] + j) * g.N[2] + k;
+  if (!used[c]) return;
+  int ax;
+  float sgn;
+  side_of(f, ax, sgn);
+  for (int ss = 0; ss < PGDG_MAX_PHASE; ss++) {
+    if (ss >= g.F) break;
+    if (C.ph_state[ss * nc + c] != 1) continue;
+    const EosT<float> mt(Eos{g.gamma[ss], g.pinf[ss]});
+    float cf[U][M];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float lm = (float)C.lim[(ss * U + u) * nc + c];
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const float v = (float)C.coef[(((size_t)ss * U + u) * M + m) * nc + c];
+        cf[u][m] = m == 0 ? v : lm * v;
+      }
+    }
+#pragma unroll 1
+    for (int q = 0; q < NQF; q++) {
+      float r[3], wq;
+      face_point_cell3<ORD>(g, f * NQF + q, r, wq);
+      float ph[M - 1];
+      basis_tail<3, ORD>(r, g.q12, ph);
+      float Ui[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        float a = cf[u][0];
+#pragma unroll
+        for (int b = 0; b < M - 1; b++) a = fmaf(cf[u][1 + b], ph[b], a);
+        Ui[u] = a;
+      }
+      float Fh[U];
+      bface_flux3(g, mt, g.bc[f], j, k, ax, sgn, Ui, Fh);
+#pragma unroll
+      for (int u = 0; u < U; u++) C.bflux[bflux_slot<ORD>(g, ss, f, q, u, pidx)] = Fh[u];
+    }
+  }
+}
+
 template <int D>
 int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st) {
   if (!C.fflux || !C.flist || !C.fcount) return PGDG_ERR_STATE;
@@ -213,6 +288,11 @@ int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_
   if (g.ord == 2) {
     hipLaunchKernelGGL((k_face_simple<D, 2>), dim3(grid_xcd(g.nc, 256)), dim3(256), 0, st, g, C, used);
     hipLaunchKernelGGL((k_face_general<D, 2>), dim3(1024), dim3(128), 0, st, g, C, used);
+    if (D == 3 && g.mfma_rhs) {
+      if (!C.bflux) return PGDG_ERR_STATE;
+      const long long n = 6LL * std::max(g.N[1] * g.N[2], std::max(g.N[0] * g.N[2], g.N[0] * g.N[1]));
+      hipLaunchKernelGGL(k_face_boundary3<2>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, C, used);
+    }
   } else {
     hipLaunchKernelGGL((k_face_simple<D, 1>), dim3(grid_xcd(g.nc, 256)), dim3(256), 0, st, g, C, used);
     hipLaunchKernelGGL((k_face_general<D, 1>), dim3(1024), dim3(128), 0, st, g, C, used);

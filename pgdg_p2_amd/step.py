@@ -74,6 +74,11 @@ class GpuSim:
         or read by every lane from global memory (same sums bit for bit)."""
         check(self.L.pgdg_pst_staging_enable(self.h, 1 if on else 0), "pgdg_pst_staging_enable")
 
+    def rhs_mfma(self, on: bool):
+        """Mixed form, 3-D P2: DG right-hand side as matrix products on the
+        matrix cores or one lane per cell (default; pgdg_rhs_mfma_enable)."""
+        check(self.L.pgdg_rhs_mfma_enable(self.h, 1 if on else 0), "pgdg_rhs_mfma_enable")
+
     def autotune(self, on: bool):
         check(self.L.pgdg_autotune_enable(self.h, 1 if on else 0), "pgdg_autotune_enable")
 

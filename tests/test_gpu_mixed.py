@@ -71,29 +71,88 @@ def test_mixed_step_p2_rk3_shock_bubble_3d(cuda):
     _compare(lambda: cases.shock_bubble_3d(nx=32, ny=16, nz=16, scheme=sch), 12, "sb3d-p2-mixed")
 
 
-@pytest.mark.gpu
-def test_mixed_pst_staged_equals_per_lane_walk(cuda):
-    # 64x32x32 P2 cells, 8 particles per cell: a workgroup's targets lie on
-    # one or two cell lines (candidates staged in LDS) or, where cells hold
-    # fewer particles, on more lines (every lane walks its candidates in
-    # global memory); both must give the per-lane sums bit for bit
+def _staged_vs_walk(case, st, steps, tag):
     from pgdg_p2_amd.compare import compare_states
     from pgdg_p2_amd.step import GpuSim
-    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
-    case, st = cases.shock_bubble_3d(nx=64, ny=32, nz=32, scheme=sch)
     case.precision = 1
     ga, gb = GpuSim(case, st), GpuSim(case, st)
     gb.pst_staging(False)
     a, b = st.copy(), st.copy()
     try:
-        for k in range(6):
-            print(f"staged-vs-walk: step {k}", flush=True)
+        for k in range(steps):
+            print(f"{tag}: step {k}", flush=True)
             ga.step(1)
             gb.step(1)
             ga.save(a)
             gb.save(b)
             bad = compare_states(a, b)
             assert not bad, f"step {k}: {bad[:6]}"
+    finally:
+        ga.close()
+        gb.close()
+
+
+@pytest.mark.gpu
+def test_mixed_pst_staged_equals_per_lane_walk(cuda):
+    # 64x32x32 P2 cells, 8 particles per cell: a workgroup's targets lie on
+    # one or two cell lines (candidates staged in LDS) or, where cells hold
+    # fewer particles, on more lines (every lane walks its candidates in
+    # global memory); both must give the per-lane sums bit for bit
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    case, st = cases.shock_bubble_3d(nx=64, ny=32, nz=32, scheme=sch)
+    _staged_vs_walk(case, st, 6, "staged-vs-walk")
+
+
+@pytest.mark.gpu
+def test_mixed_pst_staged_equals_walk_with_splits_and_merges(cuda):
+    # splits raise the particle count of some cells (candidate ranges wider
+    # than the LDS tables are walked in global memory), merges and outlet
+    # removal leave removed entries in the bins
+    from forced_events import shock_bubble_forced
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    case, st = shock_bubble_forced(nx=32, ny=16, nz=16, scheme=sch, big_frac=0.2)
+    _staged_vs_walk(case, st, 3, "staged-vs-walk-forced")
+
+
+@pytest.mark.gpu
+def test_mixed_pst_staged_equals_walk_27_per_cell(cuda):
+    # 27 particles per cell: a workgroup's targets span several cells of a
+    # line and the candidate ranges are wide
+    sch = dict(order=2, rk_stages=3, ppc_min=8, ppc_max=40)
+    case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=8, ppc_axis=3, scheme=sch)
+    _staged_vs_walk(case, st, 2, "staged-vs-walk-27ppc")
+
+
+@pytest.mark.gpu
+def test_mixed_rhs_matrix_cores_match_lane_per_cell(cuda):
+    # the right-hand side as 16x16 matrix products on the matrix cores
+    # against the one-lane-per-cell kernel: the same point terms summed in
+    # another order; the cell densities stay within the mixed form's stated
+    # tolerance of each other (the higher modes at the shock and the bubble
+    # interface follow the limiter's switching and are compared through the
+    # densities, as between the mixed and FP64 forms); the shock-bubble
+    # channel has wall, inlet and outlet faces and two phases
+    from pgdg_p2_amd.step import GpuSim
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    case, st = cases.shock_bubble_3d(nx=32, ny=16, nz=16, scheme=sch)
+    case.precision = 1
+    ga, gb = GpuSim(case, st), GpuSim(case, st)
+    ga.rhs_mfma(True)
+    gb.rhs_mfma(False)
+    a, b = st.copy(), st.copy()
+    try:
+        for k in range(6):
+            print(f"rhs-mfma-vs-lanes: step {k}", flush=True)
+            ga.step(1)
+            gb.step(1)
+            ga.save(a)
+            gb.save(b)
+            assert a.n == b.n, (k, a.n, b.n)
+            ra, rb = _mix_rho(a), _mix_rho(b)
+            l1 = np.abs(ra - rb).sum() / np.abs(rb).sum()
+            linf = np.abs(ra - rb).max() / np.abs(rb).max()
+            assert l1 <= 1e-6 and linf <= 1e-3, (k, l1, linf)
+        assert l1 > 0  # the two kernels did differ
     finally:
         ga.close()
         gb.close()

@@ -314,12 +314,19 @@ def error_horizon(size=(64, 32, 32), steps=200, every=50, scheme=None, perturb=1
             "L1_le_1e-6_at_every_row": all(r["density_L1_rel"] <= 1e-6 for r in rows)}
 
 
+# MFMA flops of one network pass on one face (padded operands: 16 inputs,
+# three hidden layers of 32 units, 16 outputs)
+MLP_FLOPS_PER_FACE = 2.0 * (16 * 32 + 32 * 32 + 32 * 32 + 32 * 16)
+
+
 def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
     """Batched MGFM on n stiffened-gas interface face pairs resident on the
     device: faces/s and Newton iterations per face for the reference
     tolerance and for 1e-12 relative on p*, from the reference's initial
     guess and from the learned guess (MLP trained on n_train solver-labelled
-    faces, seed 0x5EED0004; training time reported)."""
+    faces, seed 0x5EED0004; training time reported): network and Newton
+    solve in one kernel ("learned") and as two kernels with the guesses
+    passing through memory ("learned_two_pass")."""
     import numpy as np
     import pgdg_p2_amd as P
     from pgdg_p2_amd import workloads
@@ -341,14 +348,19 @@ def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
     tm = P.EventTimer()
     reps = 5
     p_ref = None
+    g0 = P.DeviceArray((n,), np.float64)
     for name, mode, it in (("reference_tol", P.TOL_REFERENCE, 50), ("rel_1e-12", P.TOL_RELATIVE, 100)):
-        for guess in ("plain", "learned"):
+        for guess in ("plain", "learned", "learned_two_pass"):
             def run():
                 if guess == "plain":
                     P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12,
                                      max_iter=it)
-                else:
+                elif guess == "learned":
                     mlp.mgfm_dev(inp, out, its, n, eos, tol_mode=mode, rtol=1e-12, max_iter=it)
+                else:
+                    mlp.guess_dev(inp, n, eos, g0)
+                    P.mgfm_batch_dev(inp, out, its, n, gl, pil, gr, pir, tol_mode=mode, rtol=1e-12,
+                                     max_iter=it, d_p0=g0)
             run()
             tm.start()
             for _ in range(reps):
@@ -365,17 +377,16 @@ def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
                                                                     np.abs(p_ref)))
             res[f"{name}_{guess}"] = r
     # the network alone: one inference pass over the n faces
-    g0 = P.DeviceArray((n,), np.float64)
     tm.start()
     for _ in range(reps):
         mlp.guess_dev(inp, n, eos, g0)
     tm.stop()
     ms = tm.ms() / reps
     res["mlp_inference"] = {"ms": ms, "faces_per_s": n / (ms * 1e-3),
-                            "bf16_mfma_TFLOPs": 17920.0 * n / (ms * 1e-3) / 1e12}
+                            "bf16_mfma_TFLOPs": MLP_FLOPS_PER_FACE * n / (ms * 1e-3) / 1e12}
     for key in ("reference_tol", "rel_1e-12"):
-        res[f"{key}_speedup_learned"] = res[f"{key}_learned"]["faces_per_s"] / \
-            res[f"{key}_plain"]["faces_per_s"]
+        for v in ("learned", "learned_two_pass"):
+            res[f"{key}_speedup_{v}"] = res[f"{key}_{v}"]["faces_per_s"] / res[f"{key}_plain"]["faces_per_s"]
     tm.close()
     for d in (inp, out, its, g0):
         d.free()

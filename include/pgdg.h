@@ -120,19 +120,19 @@ int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream);
  *  counterpart: the reference starts from max(pL, pR), S3:mpm.cpp:891)    *
  * ======================================================================== */
 
-/* A 4-layer MLP 10 -> 64 -> 64 -> 64 -> 2 (tanh) mapping the standardised
+/* A 4-layer MLP 10 -> 32 -> 32 -> 32 -> 2 (ReLU) mapping the standardised
  * face features (ln pL, ln pR, ln rhoL, ln rhoR, uL, uR, gammaL,
  * ln(1 + PinfL), gammaR, ln(1 + PinfR)) to the standardised targets
  * (ln p* - ln max(pL, pR), u*).  Matrix products run on MFMA with BF16
  * operands and FP32 accumulation; the master parameters and the optimiser
  * (Adam) are FP32.  Parameter vector layout (FP32, PGDG_MLP_NPARAM):
- * W0[16][64] (rows 10..15 zero), b0[64], W1[64][64], b1[64], W2[64][64],
- * b2[64], W3[64][16] (columns 2..15 zero), b3[16] (entries 2..15 zero).  */
+ * W0[16][32] (rows 10..15 zero), b0[32], W1[32][32], b1[32], W2[32][32],
+ * b2[32], W3[32][16] (columns 2..15 zero), b3[16] (entries 2..15 zero).  */
 #define PGDG_MLP_NIN     10
 #define PGDG_MLP_NPAD    16
-#define PGDG_MLP_WIDTH   64
+#define PGDG_MLP_WIDTH   32
 #define PGDG_MLP_NOUT    2
-#define PGDG_MLP_NPARAM  10448
+#define PGDG_MLP_NPARAM  3184
 
 typedef struct pgdg_mlp pgdg_mlp;
 

@@ -2,7 +2,7 @@
 network (pgdg_p2_amd/csrc/mlp.hip) for checking the GPU forward pass and
 gradients.  Not used by the product.
 
-The network is 10 -> 64 -> 64 -> 64 -> 2 with tanh, parameters in the flat
+The network is 10 -> 32 -> 32 -> 32 -> 2 with ReLU, parameters in the flat
 layout of include/pgdg.h (PGDG_MLP_NPARAM).  With emulate=True the BF16
 roundings of the GPU kernels are applied at the same points (inputs,
 weights, activations, output and hidden gradients entering a matrix
@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import numpy as np
 
-NIN, NPAD, H, NOUT, OPAD = 10, 16, 64, 2, 16
+NIN, NPAD, H, NOUT, OPAD = 10, 16, 32, 2, 16
 OW0 = 0
 OB0 = OW0 + NPAD * H
 OW1 = OB0 + H
@@ -68,9 +68,9 @@ def forward(P, X, emulate=True):
     W0, b0, W1, b1, W2, b2, W3, b3 = unpack(P)
     W0q, W1q, W2q, W3q = q(W0), q(W1), q(W2), q(W3)
     Xq = q(X)
-    H0 = q(np.tanh(Xq @ W0q + b0))
-    H1 = q(np.tanh(H0 @ W1q + b1))
-    H2 = q(np.tanh(H1 @ W2q + b2))
+    H0 = q(np.maximum(Xq @ W0q + b0, 0.0))
+    H1 = q(np.maximum(H0 @ W1q + b1, 0.0))
+    H2 = q(np.maximum(H1 @ W2q + b2, 0.0))
     Y = H2 @ W3q + b3
     return Y[:, :NOUT], (Xq, H0, H1, H2, W0q, W1q, W2q, W3q)
 
@@ -96,13 +96,13 @@ def loss_grad(P, X, T, emulate=True):
     g = np.zeros(NPARAM)
     g[OW3:OB3] = (H2.T @ dYq).ravel()
     g[OB3:NPARAM] = dY.sum(axis=0)
-    dZ2 = q((dYq @ W3q.T) * (1.0 - H2 * H2))
+    dZ2 = q((dYq @ W3q.T) * (H2 > 0))
     g[OW2:OB2] = (H1.T @ dZ2).ravel()
     g[OB2:OW3] = dZ2.sum(axis=0)
-    dZ1 = q((dZ2 @ W2q.T) * (1.0 - H1 * H1))
+    dZ1 = q((dZ2 @ W2q.T) * (H1 > 0))
     g[OW1:OB1] = (H0.T @ dZ1).ravel()
     g[OB1:OW2] = dZ1.sum(axis=0)
-    dZ0 = q((dZ1 @ W1q.T) * (1.0 - H0 * H0))
+    dZ0 = q((dZ1 @ W1q.T) * (H0 > 0))
     g[OW0:OB0] = (Xq.T @ dZ0).ravel()
     g[OB0:OW1] = dZ0.sum(axis=0)
     g[~live_mask()] = 0.0

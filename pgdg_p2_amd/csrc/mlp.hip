@@ -1,6 +1,6 @@
 // mlp.hip - learned initial guess for the MGFM Newton solve.
 //
-// A 10 -> 64 -> 64 -> 64 -> 2 tanh MLP.  Every matrix product is a set of
+// A 10 -> 32 -> 32 -> 32 -> 2 ReLU MLP.  Every matrix product is a set of
 // 16x16 output tiles computed with v_mfma_f32_16x16x16_bf16 (BF16 operands
 // staged in LDS, FP32 accumulation); a workgroup of four wavefronts handles
 // 64 samples.  Training keeps FP32 master parameters; each workgroup writes
@@ -24,6 +24,9 @@ typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = PGDG_MLP_NIN, kInPad = PGDG_MLP_NPAD, kH = PGDG_MLP_WIDTH;
+static_assert(kH % 32 == 0, "the hidden width is a multiple of 32 (K = 32 MFMA steps)");
+constexpr int kNT = kH / 16;  // 16-unit tiles of a hidden layer
+constexpr int kKS = kH / 32;  // K = 32 steps over a hidden layer's inputs
 constexpr int kOut = PGDG_MLP_NOUT, kOutPad = 16;
 constexpr int oW0 = 0;
 constexpr int oB0 = oW0 + kInPad * kH;
@@ -46,16 +49,20 @@ __host__ __device__ inline bool param_live(int p) {
   return true;
 }
 
+// Natural logarithm and exponential from the hardware log2 and exp2 (1 ulp
+// in FP32): the features enter the network as BF16 and the output sets only
+// a starting point, so these are ample, and they cost two instructions.
+__device__ inline float ln_fast(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+__device__ inline float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
 // FP32 -> BF16, round to nearest even (the conversion instruction of gfx950)
 __device__ inline unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal
-// (relative error ~1e-6, far below the BF16 rounding the result takes).
-__device__ inline float tanh_fast(float x) {
-  const float e = __builtin_amdgcn_exp2f(fminf(x, 40.f) * 2.8853900817779268f);  // 2x / ln 2
-  return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
-}
+// Activation of the hidden layers (one maximum per unit: the network runs
+// once per face ahead of a few FP64 Newton iterations, so its activations
+// must cost far less than one iteration).
+__device__ inline float relu(float x) { return fmaxf(x, 0.f); }
 
 // One 16x16 tile of C = A * B over K (a multiple of 16) from BF16 LDS
 // operands with element strides: A(m, k) = A[m * ars + k * acs],
@@ -86,8 +93,8 @@ struct FwdLds {
   float b0[kH], b1[kH], b2[kH], b3[kOutPad];
 };
 // Training adds the output gradient (FP32 and BF16) and the BF16 gradient
-// of the current hidden layer; tanh' is taken from the stored BF16
-// activations, 1 - h^2.  Total below 64 KiB.
+// of the current hidden layer; the activation's derivative is taken from
+// the stored BF16 activations (1 where h > 0, else 0).  Total below 64 KiB.
 struct TrainLds {
   FwdLds f;
   float DYf[kTile * kOutPad];
@@ -113,17 +120,17 @@ __device__ inline void load_params(const float* __restrict__ P, FwdLds& s) {
   for (int q = threadIdx.x; q < kOutPad; q += kThreads) s.b3[q] = P[oB3 + q];
 }
 
-// Hidden layer: Hout = tanh(Hin * W + b), 64 x 64, K = kin.
+// Hidden layer: Hout = relu(Hin * W + b), 64 samples x kH units, K = kin.
 __device__ inline void hidden(const unsigned short* Hin, int kin, const unsigned short* W,
                               const float* b, unsigned short* Hout) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  for (int t = w; t < 16; t += 4) {
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+  for (int t = w; t < 4 * kNT; t += 4) {
+    const int m0 = (t / kNT) * 16, n0 = (t % kNT) * 16;
     f32x4_t acc = mma_tile(Hin, kin, 1, W, kH, 1, m0, n0, kin);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
-      Hout[row * kH + col] = f2bf(tanh_fast(acc[i] + b[col]));
+      Hout[row * kH + col] = f2bf(relu(acc[i] + b[col]));
     }
   }
 }
@@ -154,8 +161,8 @@ __device__ inline void face_features(const double* __restrict__ in, int64_t n, i
   const double ul = in[i], ur = in[n + i], pl = in[2 * n + i], pr = in[3 * n + i];
   const double rl = in[4 * n + i], rr = in[5 * n + i];
   // the features enter the network as BF16: FP32 logarithms are ample
-  const float raw[kIn] = {logf(fmaxf((float)pl, 1e-30f)), logf(fmaxf((float)pr, 1e-30f)),
-                          logf(fmaxf((float)rl, 1e-30f)), logf(fmaxf((float)rr, 1e-30f)),
+  const float raw[kIn] = {ln_fast(fmaxf((float)pl, 1e-30f)), ln_fast(fmaxf((float)pr, 1e-30f)),
+                          ln_fast(fmaxf((float)rl, 1e-30f)), ln_fast(fmaxf((float)rr, 1e-30f)),
                           (float)ul, (float)ur, (float)ml.gamma,
                           (float)log1p(fmax(ml.pinf, 0.0)), (float)mr.gamma,
                           (float)log1p(fmax(mr.pinf, 0.0))};
@@ -212,123 +219,225 @@ __device__ inline bf16x8_t load_wt32(const float* __restrict__ W, int ldw, int m
   return cat8(load_wt(W, ldw, mt, 2 * j), load_wt(W, ldw, mt, 2 * j + 1));
 }
 
-// fx: the standardised EOS features 6..9 (the same for every face)
-__global__ void __launch_bounds__(kThreads) k_mlp_guess(const float* __restrict__ P, pgdg_mlp_norm nm,
-                                                        float4 fx, const double* __restrict__ in,
-                                                        int64_t n, double* __restrict__ pstar0) {
-  const int l = threadIdx.x & 63, g = l >> 4, f = l & 15;
-  bf16x4_t A0[4];
-  bf16x8_t A1[4][2], A2[4][2], A3[2];
-  float bb0[4][4], bb1[4][4], bb2[4][4];
+// The network's operands in the registers of a wavefront (MFMA A-operand
+// layouts of W^T and the biases of the units a lane's result entries hold),
+// loaded once and used for every 16-face tile the wavefront runs.
+struct MlpRegs {
+  bf16x4_t A0[kNT];
+  bf16x8_t A1[kNT][kKS], A2[kNT][kKS], A3[kKS];
+  float bb0[kNT][4], bb1[kNT][4], bb2[kNT][4];
+  float b3;
+  __device__ void load(const float* __restrict__ P) {
+    const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
-  for (int mt = 0; mt < 4; mt++) {
-    A0[mt] = load_wt(P + oW0, kH, mt, 0);
+    for (int mt = 0; mt < kNT; mt++) {
+      A0[mt] = load_wt(P + oW0, kH, mt, 0);
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      A1[mt][j] = load_wt32(P + oW1, kH, mt, j);
-      A2[mt][j] = load_wt32(P + oW2, kH, mt, j);
-    }
+      for (int j = 0; j < kKS; j++) {
+        A1[mt][j] = load_wt32(P + oW1, kH, mt, j);
+        A2[mt][j] = load_wt32(P + oW2, kH, mt, j);
+      }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int unit = mt * 16 + g * 4 + q;
-      bb0[mt][q] = P[oB0 + unit];
-      bb1[mt][q] = P[oB1 + unit];
-      bb2[mt][q] = P[oB2 + unit];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 2; j++) A3[j] = load_wt32(P + oW3, kOutPad, 0, j);
-  const float b3 = P[oB3];
-  const int64_t ntile = (n + 15) / 16;
-  const int64_t wstride = (int64_t)gridDim.x * (kThreads / 64);
-  // inputs of a lane: group 0 (pl, pr, rhol, rhor), group 1 (ul, ur); the
-  // next tile's are loaded while the current one is computed
-  auto load_in = [&](int64_t t, double* v) {
-    const int64_t i = t * 16 + f;
-    v[0] = v[1] = v[2] = v[3] = 1.0;
-    if (t >= ntile || i >= n) return;
-    if (g == 0) {
-      v[0] = in[2 * n + i];
-      v[1] = in[3 * n + i];
-      v[2] = in[4 * n + i];
-      v[3] = in[5 * n + i];
-    } else if (g == 1) {
-      v[0] = in[i];
-      v[1] = in[n + i];
-    }
-  };
-  int64_t tile = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-  double cur[4], nxt[4];
-  load_in(tile, cur);
-  for (; tile < ntile; tile += wstride) {
-    load_in(tile + wstride, nxt);
-    const int64_t i = tile * 16 + f;
-    const bool live = i < n;
-    // features 4g .. 4g+3 of face f
-    float x[4] = {0.f, 0.f, 0.f, 0.f};
-    if (live) {
-      if (g == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-          x[q] = (logf(fmaxf((float)cur[q], 1e-30f)) - nm.feat_mu[q]) / nm.feat_sd[q];
-      } else if (g == 1) {
-        x[0] = ((float)cur[0] - nm.feat_mu[4]) / nm.feat_sd[4];
-        x[1] = ((float)cur[1] - nm.feat_mu[5]) / nm.feat_sd[5];
-        x[2] = fx.x;
-        x[3] = fx.y;
-      } else if (g == 2) {
-        x[0] = fx.z;
-        x[1] = fx.w;
+      for (int q = 0; q < 4; q++) {
+        const int unit = mt * 16 + g * 4 + q;
+        bb0[mt][q] = P[oB0 + unit];
+        bb1[mt][q] = P[oB1 + unit];
+        bb2[mt][q] = P[oB2 + unit];
       }
     }
-    const bf16x4_t bx = pack4(x[0], x[1], x[2], x[3]);
-    bf16x4_t h[4];
-    {
-      f32x4_t acc[4];
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++) {
+    for (int j = 0; j < kKS; j++) A3[j] = load_wt32(P + oW3, kOutPad, 0, j);
+    b3 = P[oB3];
+  }
+  // Output unit 0 (standardised ln(p* / max(pL, pR))) of face l % 16 of a
+  // tile, valid in the lanes of group 0; x: features 4 g .. 4 g + 3 of that
+  // face (lane group g).
+  __device__ float run(const float* x) const {
+    const bf16x4_t bx = pack4(x[0], x[1], x[2], x[3]);
+    bf16x4_t h[kNT];
+    {
+      f32x4_t acc[kNT];
+#pragma unroll
+      for (int mt = 0; mt < kNT; mt++) {
         acc[mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
         acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0[mt], bx, acc[mt], 0, 0, 0);
       }
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++)
-        h[mt] = pack4(tanh_fast(acc[mt][0] + bb0[mt][0]), tanh_fast(acc[mt][1] + bb0[mt][1]),
-                      tanh_fast(acc[mt][2] + bb0[mt][2]), tanh_fast(acc[mt][3] + bb0[mt][3]));
+      for (int mt = 0; mt < kNT; mt++)
+        h[mt] = pack4(relu(acc[mt][0] + bb0[mt][0]), relu(acc[mt][1] + bb0[mt][1]),
+                      relu(acc[mt][2] + bb0[mt][2]), relu(acc[mt][3] + bb0[mt][3]));
     }
 #pragma unroll
     for (int layer = 0; layer < 2; layer++) {
-      const bf16x8_t b01 = cat8(h[0], h[1]), b23 = cat8(h[2], h[3]);
-      f32x4_t acc[4];
+      bf16x8_t bk[kKS];
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++) {
-        acc[mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer ? A2[mt][0] : A1[mt][0], b01,
-                                                          acc[mt], 0, 0, 0);
-      }
+      for (int j = 0; j < kKS; j++) bk[j] = cat8(h[2 * j], h[2 * j + 1]);
+      f32x4_t acc[kNT];
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++)
-        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer ? A2[mt][1] : A1[mt][1], b23,
-                                                          acc[mt], 0, 0, 0);
+      for (int mt = 0; mt < kNT; mt++) acc[mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++) {
+      for (int j = 0; j < kKS; j++)
+#pragma unroll
+        for (int mt = 0; mt < kNT; mt++)
+          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer ? A2[mt][j] : A1[mt][j], bk[j], acc[mt],
+                                                            0, 0, 0);
+#pragma unroll
+      for (int mt = 0; mt < kNT; mt++) {
         const float* bb = layer ? bb2[mt] : bb1[mt];
-        h[mt] = pack4(tanh_fast(acc[mt][0] + bb[0]), tanh_fast(acc[mt][1] + bb[1]),
-                      tanh_fast(acc[mt][2] + bb[2]), tanh_fast(acc[mt][3] + bb[3]));
+        h[mt] = pack4(relu(acc[mt][0] + bb[0]), relu(acc[mt][1] + bb[1]),
+                      relu(acc[mt][2] + bb[2]), relu(acc[mt][3] + bb[3]));
       }
     }
     f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[0], cat8(h[0], h[1]), acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[1], cat8(h[2], h[3]), acc, 0, 0, 0);
-    // lanes of group 0 hold output unit 0 (ln p* ratio) of face f in acc[0]
-    if (g == 0 && live) {
-      const double pm = cur[0] >= cur[1] ? cur[0] : cur[1];
-      const float lr = (acc[0] + b3) * nm.out_sd[0] + nm.out_mu[0];
-      double gs = pm * (double)expf(lr);
-      if (!(gs > 0) || !(gs < 1e300)) gs = pm;
-      pstar0[i] = gs;
-    }
+#pragma unroll
+    for (int j = 0; j < kKS; j++)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[j], cat8(h[2 * j], h[2 * j + 1]), acc, 0, 0, 0);
+    return acc[0] + b3;
+  }
+};
+
+// Inputs of face i for the lanes of a tile: group 0 (pL, pR, rhoL, rhoR),
+// group 1 (uL, uR); 1.0 past the end.
+__device__ inline void tile_inputs(const double* __restrict__ in, int64_t n, int64_t i, bool live, double* v) {
+  const int g = (threadIdx.x & 63) >> 4;
+  v[0] = v[1] = v[2] = v[3] = 1.0;
+  if (!live) return;
+  if (g == 0) {
+    v[0] = in[2 * n + i];
+    v[1] = in[3 * n + i];
+    v[2] = in[4 * n + i];
+    v[3] = in[5 * n + i];
+  } else if (g == 1) {
+    v[0] = in[i];
+    v[1] = in[n + i];
+  }
+}
+
+// Standardised features 4 g .. 4 g + 3 of a face from its lane's inputs
+// (fx: the standardised EOS features 6..9, the same for every face).
+__device__ inline void tile_features(const double* v, bool live, const pgdg_mlp_norm& nm, float4 fx, float* x) {
+  const int g = (threadIdx.x & 63) >> 4;
+  x[0] = x[1] = x[2] = x[3] = 0.f;
+  if (!live) return;
+  if (g == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      x[q] = (ln_fast(fmaxf((float)v[q], 1e-30f)) - nm.feat_mu[q]) * __builtin_amdgcn_rcpf(nm.feat_sd[q]);
+  } else if (g == 1) {
+    x[0] = ((float)v[0] - nm.feat_mu[4]) * __builtin_amdgcn_rcpf(nm.feat_sd[4]);
+    x[1] = ((float)v[1] - nm.feat_mu[5]) * __builtin_amdgcn_rcpf(nm.feat_sd[5]);
+    x[2] = fx.x;
+    x[3] = fx.y;
+  } else if (g == 2) {
+    x[0] = fx.z;
+    x[1] = fx.w;
+  }
+}
+
+// Initial p* from the network output y (lanes of group 0): max(pL, pR)
+// exp(y sd + mu), or max(pL, pR) where that is not a positive finite value.
+__device__ inline double guess_of(float y, const double* v, const pgdg_mlp_norm& nm) {
+  const double pm = v[0] >= v[1] ? v[0] : v[1];
+  const float lr = y * nm.out_sd[0] + nm.out_mu[0];
+  double gs = pm * (double)exp_fast(lr);
+  if (!(gs > 0) || !(gs < 1e300)) gs = pm;
+  return gs;
+}
+
+__global__ void __launch_bounds__(kThreads) k_mlp_guess(const float* __restrict__ P, pgdg_mlp_norm nm,
+                                                        float4 fx, const double* __restrict__ in,
+                                                        int64_t n, double* __restrict__ pstar0) {
+  const int l = threadIdx.x & 63, g = l >> 4, f = l & 15;
+  MlpRegs R;
+  R.load(P);
+  const int64_t ntile = (n + 15) / 16;
+  const int64_t wstride = (int64_t)gridDim.x * (kThreads / 64);
+  int64_t tile = (int64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  // the next tile's inputs are loaded while the current one is computed
+  double cur[4], nxt[4];
+  tile_inputs(in, n, tile * 16 + f, tile < ntile && tile * 16 + f < n, cur);
+  for (; tile < ntile; tile += wstride) {
+    const int64_t tn = tile + wstride;
+    tile_inputs(in, n, tn * 16 + f, tn < ntile && tn * 16 + f < n, nxt);
+    const int64_t i = tile * 16 + f;
+    const bool live = i < n;
+    float x[4];
+    tile_features(cur, live, nm, fx, x);
+    const float y = R.run(x);
+    if (g == 0 && live) pstar0[i] = guess_of(y, cur, nm);
 #pragma unroll
     for (int q = 0; q < 4; q++) cur[q] = nxt[q];
+  }
+}
+
+// MGFM batch with the learned initial guess in one kernel: a wavefront owns
+// kLearnedChunk consecutive faces; it first runs the network over the
+// chunk's 16-face tiles (operands in registers) and keeps the guesses in
+// LDS, then runs mgfm_batch_kernel's work-sharing Newton loop over the chunk
+// (a converged lane takes the next face), each face starting from its guess.
+// No guess array passes through memory.
+constexpr int kLearnedChunk = 1024;
+
+__global__ void __launch_bounds__(kThreads) k_mgfm_learned(
+    const float* __restrict__ P, pgdg_mlp_norm nm, float4 fx, const double* __restrict__ in,
+    double* __restrict__ out, int32_t* __restrict__ iters, int64_t n, Eos ml, Eos mr, int variant,
+    int tol_mode, double rtol, int max_iter) {
+  __shared__ double s_g[kThreads / 64][kLearnedChunk];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, f = lane & 15;
+  const int64_t wave = (int64_t)blockIdx.x * (kThreads / 64) + w;
+  const int64_t c0 = wave * kLearnedChunk;
+  const int64_t c1 = c0 + kLearnedChunk < n ? c0 + kLearnedChunk : n;
+  if (c0 < n) {
+    MlpRegs R;
+    R.load(P);
+    const int nt = (int)((c1 - c0 + 15) / 16);
+    double cur[4], nxt[4];
+    tile_inputs(in, n, c0 + f, c0 + f < c1, cur);
+    for (int t = 0; t < nt; t++) {
+      const int64_t in_next = c0 + (int64_t)(t + 1) * 16 + f;
+      tile_inputs(in, n, in_next, t + 1 < nt && in_next < c1, nxt);
+      const int64_t i = c0 + (int64_t)t * 16 + f;
+      const bool live = i < c1;
+      float x[4];
+      tile_features(cur, live, nm, fx, x);
+      const float y = R.run(x);
+      if (g == 0 && live) s_g[w][t * 16 + f] = guess_of(y, cur, nm);
+#pragma unroll
+      for (int q = 0; q < 4; q++) cur[q] = nxt[q];
+    }
+  }
+  __syncthreads();  // the guesses are read by other lanes of the wavefront
+  if (c0 >= n) return;
+  int64_t next = c0;  // wave-uniform
+  int64_t i = -1;
+  bool done = false;
+  MgfmState S;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (;;) {
+    const bool need = !done && i < 0;
+    const unsigned long long needm = __ballot(need);
+    if (needm) {
+      if (need) {
+        const int64_t j = next + __popcll(needm & below);
+        if (j < c1) {
+          i = j;
+          mgfm_begin(S, in[j], in[n + j], in[2 * n + j], in[3 * n + j], in[4 * n + j], in[5 * n + j],
+                     variant, s_g[w][j - c0]);
+        } else {
+          done = true;
+        }
+      }
+      next += __popcll(needm);
+    }
+    if (__ballot(!done) == 0ull) break;
+    if (i >= 0 && !mgfm_iterate(S, ml, mr, variant, tol_mode, rtol, max_iter)) {
+      const MgfmOut o = mgfm_end(S, ml, variant, tol_mode);
+      out[i] = o.rho_l;
+      out[n + i] = o.pstar;
+      out[2 * n + i] = o.u_l;
+      if (iters) iters[i] = o.iters;
+      i = -1;
+    }
   }
 }
 
@@ -362,14 +471,14 @@ __device__ inline void put_tile(float* g, int ncols, int m0, int n0, f32x4_t acc
 
 // Hidden-layer backward: from DZ (BF16 dL/dZ of the layer) compute
 // dW = Hin^T DZ and db = column sums of DZ; if `below`, also dH = DZ W^T and
-// DZ <- dH * (1 - Hin^2) for the layer underneath (Hin is its output).
+// DZ <- dH * (Hin > 0) for the layer underneath (Hin is its output).
 __device__ inline void backward_layer(TrainLds& s, const unsigned short* Hin, int kin,
                                       const unsigned short* W, float* gW, float* gb, bool below) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   // dW[k][n] = sum_s Hin[s][k] DZ[s][n]
   const int ntile_k = kin / 16;
-  for (int t = w; t < ntile_k * 4; t += 4) {
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+  for (int t = w; t < ntile_k * kNT; t += 4) {
+    const int m0 = (t / kNT) * 16, n0 = (t % kNT) * 16;
     put_tile(gW, kH, m0, n0, mma_tile(Hin, 1, kin, s.DZ, kH, 1, m0, n0, kTile));
   }
   if (threadIdx.x < kH) {
@@ -379,23 +488,23 @@ __device__ inline void backward_layer(TrainLds& s, const unsigned short* Hin, in
   }
   if (!below) return;
   // dH[s][k] = sum_n DZ[s][n] W[k][n]
-  f32x4_t dh[4];
+  f32x4_t dh[kNT];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < kNT; q++) {
     const int t = w + 4 * q;
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    const int m0 = (t / kNT) * 16, n0 = (t % kNT) * 16;
     dh[q] = mma_tile(s.DZ, kH, 1, W, 1, kH, m0, n0, kH);
   }
   __syncthreads();  // every read of DZ is done
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < kNT; q++) {
     const int t = w + 4 * q;
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    const int m0 = (t / kNT) * 16, n0 = (t % kNT) * 16;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
       const float h = bf2f(Hin[row * kH + col]);
-      s.DZ[row * kH + col] = f2bf(dh[q][i] * (1.f - h * h));
+      s.DZ[row * kH + col] = f2bf(h > 0.f ? dh[q][i] : 0.f);
     }
   }
   __syncthreads();
@@ -435,8 +544,10 @@ __global__ void __launch_bounds__(kThreads) k_mlp_fwd_bwd(const float* __restric
   }
   float* g = part + (size_t)blockIdx.x * kNParam;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  // output layer: dW3[k][o] = sum_s H2[s][k] DY[s][o]; db3; dH2 = DY W3^T
-  put_tile(g + oW3, kOutPad, w * 16, 0, mma_tile(s.f.H2, 1, kH, s.DY, kOutPad, 1, w * 16, 0, kTile));
+  // output layer: dW3[k][o] = sum_s H2[s][k] DY[s][o]; db3; dH2 = DY W3^T,
+  // times the activation's derivative (H2 > 0)
+  for (int t = w; t < kNT; t += 4)
+    put_tile(g + oW3, kOutPad, t * 16, 0, mma_tile(s.f.H2, 1, kH, s.DY, kOutPad, 1, t * 16, 0, kTile));
   if (threadIdx.x < kOutPad) {
     float acc = 0.f;
     for (int r = 0; r < kTile; r++) acc += s.DYf[r * kOutPad + threadIdx.x];
@@ -448,22 +559,22 @@ __global__ void __launch_bounds__(kThreads) k_mlp_fwd_bwd(const float* __restric
     lpart[blockIdx.x] = a;
   }
   {
-    f32x4_t dh[4];
+    f32x4_t dh[kNT];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < kNT; q++) {
       const int t = w + 4 * q;
-      const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+      const int m0 = (t / kNT) * 16, n0 = (t % kNT) * 16;
       dh[q] = mma_tile(s.DY, kOutPad, 1, s.f.W3, 1, kOutPad, m0, n0, kOutPad);
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < kNT; q++) {
       const int t = w + 4 * q;
-      const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+      const int m0 = (t / kNT) * 16, n0 = (t % kNT) * 16;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int row = m0 + (l >> 4) * 4 + i, col = n0 + (l & 15);
         const float h = bf2f(s.f.H2[row * kH + col]);
-        s.DZ[row * kH + col] = f2bf(dh[q][i] * (1.f - h * h));
+        s.DZ[row * kH + col] = f2bf(h > 0.f ? dh[q][i] : 0.f);
       }
     }
   }
@@ -566,8 +677,6 @@ struct pgdg_mlp {
   float* loss = nullptr;
   int part_blocks = 0;
   pgdg_mlp_norm norm{};
-  double* guess = nullptr;
-  int64_t guess_cap = 0;
   bool ready = false;
 };
 
@@ -618,6 +727,16 @@ void init_params(std::vector<float>& P, uint64_t seed) {
   fill(oW3, kH, kOutPad, kH, kOut, kH, kOut);
 }
 
+// The standardised EOS features 6..9 of a face pair (the same for every
+// face of a batch).
+float4 eos_features(const pgdg_mlp_norm& nm, const pgdg_eos_pair* eos) {
+  const double c[4] = {eos->gamma_l, std::log1p(std::max(eos->pinf_l, 0.0)), eos->gamma_r,
+                       std::log1p(std::max(eos->pinf_r, 0.0))};
+  float fx4[4];
+  for (int k = 0; k < 4; k++) fx4[k] = ((float)c[k] - nm.feat_mu[6 + k]) / nm.feat_sd[6 + k];
+  return make_float4(fx4[0], fx4[1], fx4[2], fx4[3]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -655,7 +774,7 @@ int pgdg_mlp_create(pgdg_mlp** out) {
 
 int pgdg_mlp_destroy(pgdg_mlp* m) {
   if (!m) return PGDG_ERR_ARG;
-  void* ptrs[] = {m->params, m->m, m->v, m->grad, m->part, m->lpart, m->loss, m->guess};
+  void* ptrs[] = {m->params, m->m, m->v, m->grad, m->part, m->lpart, m->loss};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -825,11 +944,7 @@ int pgdg_mlp_guess(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
   if (!m->ready) return PGDG_ERR_STATE;
   if (n == 0) return PGDG_OK;
   const pgdg_mlp_norm& nm = m->norm;
-  const double c[4] = {eos->gamma_l, std::log1p(std::max(eos->pinf_l, 0.0)), eos->gamma_r,
-                       std::log1p(std::max(eos->pinf_r, 0.0))};
-  float fx4[4];
-  for (int k = 0; k < 4; k++) fx4[k] = ((float)c[k] - nm.feat_mu[6 + k]) / nm.feat_sd[6 + k];
-  const float4 fx = make_float4(fx4[0], fx4[1], fx4[2], fx4[3]);
+  const float4 fx = eos_features(nm, eos);
   const int64_t ntile = (n + 15) / 16;
   const unsigned grid = (unsigned)std::min<int64_t>((ntile + 3) / 4, 256 * 8);
   hipLaunchKernelGGL(k_mlp_guess, dim3(grid), dim3(kThreads), 0, as_stream(stream), m->params, nm, fx,
@@ -840,17 +955,23 @@ int pgdg_mlp_guess(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
 
 int pgdg_mgfm_batch_learned(pgdg_mlp* m, const double* in, double* out, int32_t* iters, int64_t n,
                             const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts, void* stream) {
+  using namespace pgdg;
   if (!m || !eos || !opts || n < 0) return PGDG_ERR_ARG;
+  if (!m->ready) return PGDG_ERR_STATE;
   if (n == 0) return PGDG_OK;
-  if (n > m->guess_cap) {
-    if (m->guess) PGDG_HIP_TRY(hipFree(m->guess));
-    m->guess = nullptr;
-    m->guess_cap = 0;
-    PGDG_HIP_TRY(hipMalloc((void**)&m->guess, sizeof(double) * (size_t)n));
-    m->guess_cap = n;
-  }
-  PGDG_TRY(pgdg_mlp_guess(m, in, n, eos, m->guess, stream));
-  return pgdg_mgfm_batch(in, out, iters, n, eos, opts, m->guess, stream);
+  if (!in || !out) return PGDG_ERR_ARG;
+  if (opts->variant != MGFM_S3 && opts->variant != MGFM_T2) return PGDG_ERR_ARG;
+  if (opts->tol_mode != TOL_REF && opts->tol_mode != TOL_REL) return PGDG_ERR_ARG;
+  if (opts->max_iter < 1) return PGDG_ERR_ARG;
+  const pgdg_mlp_norm& nm = m->norm;
+  const float4 fx = eos_features(nm, eos);
+  const int64_t waves = (n + kLearnedChunk - 1) / kLearnedChunk;
+  const unsigned grid = (unsigned)((waves + kThreads / 64 - 1) / (kThreads / 64));
+  hipLaunchKernelGGL(k_mgfm_learned, dim3(grid), dim3(kThreads), 0, as_stream(stream), m->params, nm, fx,
+                     in, out, iters, n, Eos{eos->gamma_l, eos->pinf_l}, Eos{eos->gamma_r, eos->pinf_r},
+                     opts->variant, opts->tol_mode, opts->rtol, opts->max_iter);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
 }
 
 }  // extern "C"

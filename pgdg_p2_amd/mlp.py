@@ -1,6 +1,6 @@
 """Learned initial guess for the MGFM Newton solve (C ABI pgdg_mlp_*).
 
-A 10 -> 64 -> 64 -> 64 -> 2 tanh MLP trained on the GPU (BF16 MFMA forward
+A 10 -> 32 -> 32 -> 32 -> 2 ReLU MLP trained on the GPU (BF16 MFMA forward
 and backward kernels, FP32 Adam) on faces labelled by the exact MGFM solve;
 its p* prediction seeds the Newton iteration, which still runs to the
 requested tolerance.  The reference has no counterpart: its Newton solve
@@ -14,7 +14,7 @@ import numpy as np
 
 from . import DeviceArray, EosPair, MgfmOpts, check, device_sync, lib, MGFM_S3, TOL_RELATIVE
 
-NIN, NPAD, NOUT, NPARAM = 10, 16, 2, 10448
+NIN, NPAD, NOUT, NPARAM = 10, 16, 2, 3184
 
 
 class MlpNorm(C.Structure):

@@ -319,14 +319,16 @@ def error_horizon(size=(64, 32, 32), steps=200, every=50, scheme=None, perturb=1
 MLP_FLOPS_PER_FACE = 2.0 * (16 * 32 + 32 * 32 + 32 * 32 + 32 * 16)
 
 
-def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
+def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12, mlp_save=None, mlp_load=None):
     """Batched MGFM on n stiffened-gas interface face pairs resident on the
     device: faces/s and Newton iterations per face for the reference
     tolerance and for 1e-12 relative on p*, from the reference's initial
     guess and from the learned guess (MLP trained on n_train solver-labelled
     faces, seed 0x5EED0004; training time reported): network and Newton
     solve in one kernel ("learned") and as two kernels with the guesses
-    passing through memory ("learned_two_pass")."""
+    passing through memory ("learned_two_pass").  mlp_save: write the
+    trained network (parameters and normalisation, .npz); mlp_load: use such
+    a file instead of training (for counter runs that leave training out)."""
     import numpy as np
     import pgdg_p2_amd as P
     from pgdg_p2_amd import workloads
@@ -334,14 +336,27 @@ def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12):
     eos = workloads.MGFM_EOS
     gl, pil, gr, pir = eos
     res = {"faces": n, "train_faces": n_train, "epochs": epochs}
-    d_tr = P.DeviceArray.from_numpy(workloads.mgfm_faces(n_train, seed=0x5EED0004))
     mlp = Mlp()
-    P.device_sync()
-    t0 = time.perf_counter()
-    hist = mlp.train(d_tr, n_train, eos, epochs=epochs, batch=4096, lr=3e-3)
-    res["train_s"] = time.perf_counter() - t0
-    res["train_loss_first_last"] = [float(hist[:32].mean()), float(hist[-32:].mean())]
-    d_tr.free()
+    if mlp_load:
+        from pgdg_p2_amd.mlp import MlpNorm
+        z = np.load(mlp_load)
+        nm = MlpNorm()
+        for f in ("feat_mu", "feat_sd", "out_mu", "out_sd"):
+            getattr(nm, f)[:] = z[f].tolist()
+        mlp.set_params(z["params"], nm)
+        res["trained_network"] = str(mlp_load)
+    else:
+        d_tr = P.DeviceArray.from_numpy(workloads.mgfm_faces(n_train, seed=0x5EED0004))
+        P.device_sync()
+        t0 = time.perf_counter()
+        hist = mlp.train(d_tr, n_train, eos, epochs=epochs, batch=4096, lr=3e-3)
+        res["train_s"] = time.perf_counter() - t0
+        res["train_loss_first_last"] = [float(hist[:32].mean()), float(hist[-32:].mean())]
+        d_tr.free()
+        if mlp_save:
+            Pm, nm = mlp.get_params()
+            np.savez(mlp_save, params=Pm, feat_mu=np.array(nm.feat_mu), feat_sd=np.array(nm.feat_sd),
+                     out_mu=np.array(nm.out_mu), out_sd=np.array(nm.out_sd))
     inp = P.DeviceArray.from_numpy(workloads.mgfm_faces(n))
     out = P.DeviceArray((3, n), np.float64)
     its = P.DeviceArray((n,), np.int32)
@@ -456,6 +471,10 @@ def main():
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--only-mgfm", action="store_true")
+    ap.add_argument("--mlp-save", default=None, help="MGFM leg: write the trained network (.npz)")
+    ap.add_argument("--mlp-load", default=None, help="MGFM leg: use a saved network instead of training")
+    ap.add_argument("--rhs-mfma", action="store_true",
+                    help="mixed form: the right-hand side on the matrix cores (k_rhs_mfma)")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel over this many MiB (byte-counter "
@@ -485,7 +504,7 @@ def main():
         print(json.dumps({"mls": mls_bench(args.mls_n)}))
         return
     if args.only_mgfm:
-        print(json.dumps({"mgfm": mgfm_bench()}))
+        print(json.dumps({"mgfm": mgfm_bench(mlp_save=args.mlp_save, mlp_load=args.mlp_load)}))
         return
     if args.only_horizon:
         print(json.dumps({"error_horizon": [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
@@ -509,6 +528,8 @@ def main():
         case.precision = precision
         n0 = st.n
         gpu = GpuSim(case, st, cap=int(n0 * 2.2) + 4096)
+        if precision and args.rhs_mfma:
+            gpu.rhs_mfma(True)
         st.p = {}  # keep only the cell arrays, for the density comparison
         name = "mixed" if precision else "fp64"
         print(f"[{name}] setup {time.perf_counter() - t_setup:.1f}s: {nx}x{ny}x{nz} cells, "

@@ -137,8 +137,11 @@ def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nph
     (rho=1, p=1) and the bottom-right block (rho=1, p=0.1); phase 2
     gamma=1.4 the top-right block (rho=0.125, p=0.1).  Region tests follow
     T2:mpm_ini.cpp:159,200 (x > 1, y < 1.5).
-    Three phases: phase 1 left (gamma 1.5), phase 2 bottom-right (gamma 1.4),
-    phase 3 top-right (gamma 1.67).
+    Three phases, as T3 sets them up (EOS constants T3:mpm_ini.cpp:79-95,
+    regions and states T3:mpm_ini.cpp:157-236): phase 1 the bottom-right
+    block x > 1, y < 1.5 (gamma 1.4, Pinf 0, rho 1, p 0.1), phase 2 the
+    top-right block (gamma 1.5, Pinf 0, rho 0.125, p 0.1), phase 3 the left
+    block (gamma 1.6, Pinf 0.1, rho 1, p 1); ini_rho (1, 0.125, 0.125).
     """
     box_l, box_r = (0.0, 0.0, 0.0), (7.0, 3.0, 1.0)
     p_g = 3.0 / ny
@@ -146,13 +149,15 @@ def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nph
                       scheme)
     if nphase == 2:
         g = gammas or (1.5, 1.4)
+        pinf = (0.0, 0.0)
         rho0 = (1.0, 0.125)
     else:
-        g = gammas or (1.5, 1.4, 1.67)
-        rho0 = (1.0, 1.0, 0.125)
+        g = gammas or (1.4, 1.5, 1.6)
+        pinf = (0.0, 0.0, 0.1)
+        rho0 = (1.0, 0.125, 0.125)
     for s in range(nphase):
         case.gamma[s] = g[s]
-        case.pinf[s] = 0.0
+        case.pinf[s] = pinf[s]
         case.ini_rho[s] = rho0[s]
     pos = _lattice(case, jitter, seed)
     x, y = pos[0], pos[1]
@@ -168,8 +173,9 @@ def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nph
     if nphase == 2:
         phase[top] = 2
     else:
-        phase[bottom] = 2
-        phase[top] = 3
+        phase[bottom] = 1
+        phase[top] = 2
+        phase[~right] = 3
     vel = np.zeros((2, n))
     # wall particles on the particle lattice, vol = p_sz (T2:mpm_ini.cpp:385-409)
     pg = [round((box_r[a] - box_l[a]) / case.p_sz) for a in range(2)]

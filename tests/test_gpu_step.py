@@ -317,10 +317,11 @@ def test_stages_reading_bins_refuse_before_binning(cuda):
 
 @pytest.mark.gpu
 def test_step_parity_three_phase_p2_rk3_112x48_with_split_merge(cuda):
-    # the three-material triple point (phases 1/2/3: left gamma 1.5,
-    # bottom-right 1.4, top-right 1.67) with P2, SSP-RK3, CFL 0.2, 3x3
-    # jittered particles per cell and per-cell bounds [1, 27]: merges start
-    # after ~25 steps, splits after ~50; every step bit-exact
+    # the three-material triple point as T3 sets it up (phase 1 bottom-right
+    # gamma 1.4, phase 2 top-right gamma 1.5, phase 3 left gamma 1.6 with
+    # Pinf 0.1) with P2, SSP-RK3, CFL 0.2, 3x3 jittered particles per cell and
+    # per-cell bounds [1, 27]: merges start after ~45 steps, splits after ~65;
+    # every step bit-exact
     case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, nphase=3,
                                      scheme=dict(order=2, rk_stages=3, cfl=0.2, ppc_min=1,
                                                  ppc_max=27))
@@ -329,7 +330,7 @@ def test_step_parity_three_phase_p2_rk3_112x48_with_split_merge(cuda):
     gpu = GpuSim(case, st)
     a, b = st.copy(), st.copy()
     splits = merges = 0
-    for k in range(60):
+    for k in range(80):
         if k % 10 == 0:
             print(f"tp3-p2rk3: step {k}", flush=True)
         ref.step(1)
@@ -341,7 +342,7 @@ def test_step_parity_three_phase_p2_rk3_112x48_with_split_merge(cuda):
         splits += a.s.n_split
         merges += a.s.n_merge
     gpu.close()
-    assert splits > 0 and merges > 50, (splits, merges)
+    assert splits > 0 and merges > 100, (splits, merges)
 
 
 @pytest.mark.gpu
@@ -349,7 +350,7 @@ def test_step_parity_three_phase_p2_448x192_lattice_positivity(cuda):
     # the three-material junction at 448x192 P2 cells, one particle per cell:
     # phases appearing in cells next to the junction get means extrapolated
     # from quadratic particle copies; the positivity fallback of MLS_P2G
-    # keeps them physical (without it the state turns non-finite by step 15)
+    # keeps them physical, and the state stays finite and bit-exact
     case, st = cases.triple_point_2d(nx=448, ny=192, ppc_axis=1, jitter=0.0, nphase=3,
                                      scheme=dict(order=2, rk_stages=3, cfl=0.2, ppc_min=1,
                                                  ppc_max=27))

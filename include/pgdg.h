@@ -234,9 +234,24 @@ enum { PGDG_SIDE_YL = 0, PGDG_SIDE_XR, PGDG_SIDE_YR, PGDG_SIDE_XL, PGDG_SIDE_ZL,
  *                 the DG flux, T2 MGFM, per-phase v in the cell acceleration,
  *                 no inlet/outlet and no mark_in;
  *   PGDG_SEM_S3 : 3-D generalized code (S3): inlet/outlet/symmetry faces,
- *                 S3 MGFM, inlet particle copies and outlet removal.      */
+ *                 S3 MGFM, inlet particle copies and outlet removal;
+ *   PGDG_SEM_UX : 2-D underwater explosion (UX), two phases, FP64 form
+ *                 only: HLL flux at interior face points
+ *                 (UX:particle_flow.cpp:280-294), HLL against a ghost copy
+ *                 of the inner state at domain-boundary face points, with
+ *                 the y velocity reflected on PGDG_BC_WALL sides and kept
+ *                 on PGDG_BC_OUTLET sides (UX:particle_flow.cpp:314-366;
+ *                 the reference tags them 2 and -1, walls on y sides
+ *                 only); the T2 MGFM; Calculate_limiter with a zero wall
+ *                 velocity in the x columns of G_v, boundary neighbours
+ *                 equal to the cell's own means with the momentum normal
+ *                 to a domain edge set to 0, and the map
+ *                 2 ds / (ds + 1) in every cell (UX:mpm.cpp:352-587);
+ *                 MLS_P2G keeps an absent phase's density and momentum
+ *                 (UX:mpm.cpp:299-334).                                   */
 #define PGDG_SEM_T2 2
 #define PGDG_SEM_S3 3
+#define PGDG_SEM_UX 4
 
 #define PGDG_MAX_PHASE 3
 
@@ -249,7 +264,7 @@ typedef struct pgdg_case {
                             Gauss rules (no reference counterpart)        */
   int    rk_stages;      /* 2: the reference's two-stage Heun update;
                             3: SSP-RK3 (no reference counterpart)          */
-  int    semantics;      /* PGDG_SEM_T2 or PGDG_SEM_S3                     */
+  int    semantics;      /* PGDG_SEM_T2, PGDG_SEM_S3 or PGDG_SEM_UX        */
   int    n[3];           /* cells per axis (n[2] = 1 when dim == 2)         */
   int    bc[6];          /* tag of each domain side, PGDG_SIDE_* order      */
   double box_l[3], box_r[3];
@@ -333,7 +348,9 @@ typedef struct pgdg_ctx pgdg_ctx;
  * particles and n_wall wall particles (replaces the state ownership of
  * MPM::MPM / MPM::Initialize_case, S3:mpm.h:160-194).  Supported: order 1
  * or 2, rk_stages 2 or 3, dim 2 (PGDG_SEM_T2) or 3 (PGDG_SEM_S3), 2..3
- * phases.                                                                 */
+ * phases; PGDG_SEM_UX with dim 2, 2 phases, PGDG_PREC_FP64, PGDG_BC_OUTLET
+ * on the x sides and PGDG_BC_WALL or PGDG_BC_OUTLET on the y sides
+ * (anything else is PGDG_ERR_ARG).                                        */
 int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_ctx** out);
 int pgdg_destroy(pgdg_ctx* ctx);
 
@@ -460,7 +477,8 @@ int pgdg_checkpoint_read(pgdg_ctx* ctx, const char* path);
  *   #!TDV112, variables x, y[, z], phase, vx, vy, vol, rho, P, |tv|, |v|,
  *   n_neighbor in block order.
  * pgdg_plt_write_cells      the cell file (S3:mpm_ini.cpp:946-978;
- *   T2:mpm_ini.cpp:807-838): ASCII, one line per cell in i, j[, k] order.
+ *   T2:mpm_ini.cpp:807-838; UX:mpm_ini.cpp:766-795, with its extra "ca"
+ *   column): ASCII, one line per cell in i, j[, k] order.
  * pgdg_point_diagnostics    MPM::Output_point's sums (S3:mpm_ini.cpp:774-827:
  *   phase-2 front positions and mass; T2:mpm_ini.cpp:599-654: particle and
  *   grid mass and energy, particles per cell in use), in the reference's

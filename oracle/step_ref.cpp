@@ -295,6 +295,7 @@ struct Sim : Base {
   }
 
   bool s3() const { return cs.semantics == PGDG_SEM_S3; }
+  bool ux() const { return cs.semantics == PGDG_SEM_UX; }
   int cidx(int i, int j, int k) const { return (i * N[1] + j) * N[2] + k; }
   double root(double x) const { return root_dim(D, x); }
 
@@ -1198,8 +1199,12 @@ struct Sim : Base {
         if (cp.ph_state[ss] == 0) {
           std::memset(cp.coef[ss], 0, sizeof(cp.coef[ss]));
           std::memset(cp.coef0[ss], 0, sizeof(cp.coef0[ss]));
-          cp.rho[ss] = cs.ini_rho[ss];
-          for (int a = 0; a < 3; a++) cp.rhov[ss][a] = 0;
+          // T2/S3 reset the absent phase's density and momentum
+          // (T2:mpm.cpp:346-352); UX keeps them (UX:mpm.cpp:330-334)
+          if (!ux()) {
+            cp.rho[ss] = cs.ini_rho[ss];
+            for (int a = 0; a < 3; a++) cp.rhov[ss][a] = 0;
+          }
         }
       }
     }
@@ -1237,12 +1242,13 @@ struct Sim : Base {
         for (int b = 0; b < 3; b++) cp.G[a][b] = 0.;
       int bd_fg = 0;
       const double* vc = cp.v;
-      // G_v columns, S3:mpm.cpp:471-533 (T2:mpm.cpp:922-952)
+      // G_v columns, S3:mpm.cpp:471-533 (T2:mpm.cpp:922-952); UX takes a
+      // zero wall velocity on the x sides too (UX:mpm.cpp:388-399)
       if (i == N[0] - 1) {
-        for (int a = 0; a < D; a++) cp.G[a][0] = (vc[a] - vc[a]) / (0.5 * ps[0]);
+        for (int a = 0; a < D; a++) cp.G[a][0] = ux() ? (0.0 - vc[a]) / (0.5 * ps[0]) : (vc[a] - vc[a]) / (0.5 * ps[0]);
         bd_fg = 1;
       } else if (i == 0) {
-        for (int a = 0; a < D; a++) cp.G[a][0] = (vc[a] - vc[a]) / (0.5 * ps[0]);
+        for (int a = 0; a < D; a++) cp.G[a][0] = ux() ? (vc[a] - 0.0) / (0.5 * ps[0]) : (vc[a] - vc[a]) / (0.5 * ps[0]);
         bd_fg = 1;
       } else {
         const double* vR = cells[cidx(i + 1, j, k)].v;
@@ -1321,6 +1327,14 @@ struct Sim : Base {
                 for (int u = 0; u < U; u++) pj[u] = pi[s][u];
               if (j == N[1] - 1 || j == 0) pj[2] = -cp.rhov[s][1];
               if (D == 3 && (k == N[2] - 1 || k == 0)) pj[3] = -cp.rhov[s][2];
+            } else if (ux()) {
+              // the cell's own means with the momentum normal to a domain
+              // edge set to 0, whatever the side's tag (UX:mpm.cpp:478-487)
+              pj[0] = cp.rho[s];
+              for (int a = 0; a < D; a++) pj[1 + a] = cp.rhov[s][a];
+              pj[U - 1] = cp.E[s];
+              if (i == N[0] - 1 || i == 0) pj[1] = 0;
+              if (j == N[1] - 1 || j == 0) pj[2] = 0;
             } else {
               if (b == 1) {
                 pj[0] = cp.rho[s];
@@ -1353,7 +1367,8 @@ struct Sim : Base {
         else
           for (int a = 0; a < 3; a++) dr[a] = fc.cen[a] - cp.x[a];
         for (int s = 0; s < F; s++) {
-          int lim_fg = bd_fg;
+          // UX's Limiter is 2 ds / (ds + 1) everywhere (UX:glbfunc.h:291-306, UX:mpm.cpp:522-523)
+          int lim_fg = ux() ? 1 : bd_fg;
           if (cp.alph[s] != 1) lim_fg = 1;
           double phi[M - 1];
           basis_tail<D, ORD>(dr, q12, phi);
@@ -1613,7 +1628,9 @@ struct Sim : Base {
           double ci_ = std::sqrt(g * (pinf + P_i) / ti.rho);
           double vni = 0;
           for (int a = 0; a < D; a++) vni += vi[a] * n[a];
-          bool wall_like = s3() ? (b == 1) : (b != 0);
+          // T2: every boundary face is a wall; S3: tag 1; UX: none (every
+          // boundary face takes the HLL ghost flux)
+          bool wall_like = s3() ? (b == 1) : (!ux() && b != 0);
           if (b == 0) {
             const Cell& nb = cells[fc.nbr];
             for (int s1 = 0; s1 < F; s1++) {
@@ -1649,7 +1666,11 @@ struct Sim : Base {
               double cj = std::sqrt(g * (pinf + P_j) / rho_j);
               double alph_j = nb.alph[s1];
               double Fh[U][D];
-              hllc(n, ti.rho, ti.rv, ti.E, P_i, ci_, vi, rho_j, rvj, E_j, P_j, cj, vj, Fh);
+              // UX: HLL at interior faces (UX:particle_flow.cpp:280-294)
+              if (ux())
+                hll(n, ti.rho, ti.rv, ti.E, P_i, ci_, vi, rho_j, rvj, E_j, P_j, cj, vj, Fh);
+              else
+                hllc(n, ti.rho, ti.rv, ti.E, P_i, ci_, vi, rho_j, rvj, E_j, P_j, cj, vj, Fh);
               for (int bi = 0; bi < M; bi++)
                 for (int u = 0; u < U; u++)
                   for (int bb = 0; bb < D; bb++) DFij[bi][u][bb] += Fh[u][bb] * Nb[bi] * gw * alph_j;
@@ -1657,7 +1678,9 @@ struct Sim : Base {
           } else if (wall_like) {
             for (int bi = 0; bi < M; bi++)
               for (int a = 0; a < D; a++) DFij[bi][1 + a][a] += P_i * Nb[bi] * gw;
-          } else {  // S3 inlet (-1), outlet (-2), symmetry (-3): HLL with a ghost state
+          } else {  // S3 inlet (-1), outlet (-2), symmetry (-3) and every UX
+                    // boundary face: HLL with a ghost state
+                    // (UX:particle_flow.cpp:314-366)
             double rho_j = ti.rho;
             double vj[3] = {0, 0, 0};
             for (int a = 0; a < D; a++) vj[a] = vi[a];
@@ -1665,14 +1688,16 @@ struct Sim : Base {
             for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
             double P_j = P_i;
             double E_j = ti.E;
-            if (b == -1) {
+            if (b == -1 && s3()) {
               rho_j = cs.inlet_rho;
               for (int a = 0; a < D; a++) vj[a] = cs.inlet_u[a];
               for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
               P_j = cs.inlet_p;
               E_j = eos_rhoe(P_j, ss) + 0.5 * sq_d(vj) * rho_j;
             }
-            if (cp.idx[1] == N[1] - 1 || cp.idx[1] == 0) {
+            // S3 reflects v_y in the first and last y rows; UX on wall-tagged
+            // sides (its tag 2)
+            if (ux() ? b == PGDG_BC_WALL : (cp.idx[1] == N[1] - 1 || cp.idx[1] == 0)) {
               vj[1] = -vi[1];
               for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
             }

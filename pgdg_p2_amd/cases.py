@@ -16,7 +16,7 @@ import math
 import numpy as np
 
 from .state import (BC_INLET, BC_OUTLET, BC_SYMMETRY, BC_WALL, Case, HostState, SEM_S3,
-                    SEM_T2)
+                    SEM_T2, SEM_UX)
 
 CELL_RATIO, CUT_OFF = 3.2, 2.0
 
@@ -187,6 +187,58 @@ def triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, seed=0x5EED0000, nph
     st.w["norm"][:] = wn
     st.w["vol"][:] = wv
     _init_cells(st, case)
+    return case, st
+
+
+def underwater_explosion_2d(nx=400, ppc_axis=1, jitter=0.0, seed=0x5EED0006, cap_factor=3.0,
+                            scheme=None):
+    """The reference's 2-D underwater explosion (UX:mpm_ini.cpp:33-300) in
+    its own units, with UX semantics: domain [0,4]x[0,3] with
+    p_g = 4 / nx (the reference: nx = 400, giving 400 x 300 cells and one
+    particle per cell); water (phase 1: gamma 4.4, Pinf 6e8, rho 1000,
+    p 1.01325e5) below y = 1.5, air (phase 2: gamma 1.4, Pinf 0,
+    rho 1.225, p 1.01325e5) above, and a gas bubble of phase 2 (rho 1250,
+    p 1e9; its mass, as the reference sets it, p_sz^2 * 1.225) of radius
+    0.12 around (2, 1.2) (UX:mpm_ini.cpp:152-236).  Side tags: the bottom
+    reflects the y velocity (the reference's tag 2, PGDG_BC_WALL), the other
+    three sides are transmissive (tag -1, PGDG_BC_OUTLET)
+    (UX:mpm_ini.cpp:300-380).  Wall particles on the particle lattice's
+    boundary nodes with vol = p_sz (UX:mpm_ini.cpp:386-443); cells start
+    with ini_rho and the sound speeds sqrt(gamma (Pinf + p) / ini_rho) at
+    p = 1.01325e5 (water) and 1e9 (gas) (UX:mpm_ini.cpp:84-90, 262-282).
+    nx must be a multiple of 4."""
+    if nx % 4:
+        raise ValueError("nx must be a multiple of 4 (3 nx / 4 rows)")
+    box_l, box_r = (0.0, 0.0, 0.0), (4.0, 3.0, 1.0)
+    p_g = 4.0 / nx
+    ny = int(3.0 / p_g)
+    bc = [BC_WALL, BC_OUTLET, BC_OUTLET, BC_OUTLET, BC_OUTLET, BC_OUTLET]
+    case = _base_case(2, 2, (nx, ny, 1), box_l, box_r, SEM_UX, bc, p_g, ppc_axis, scheme)
+    case.gamma[0], case.pinf[0], case.ini_rho[0] = 4.4, 6e8, 1000.0
+    case.gamma[1], case.pinf[1], case.ini_rho[1] = 1.4, 0.0, 1.225
+    pos = _lattice(case, jitter, seed)
+    n = pos.shape[1]
+    x, y = pos[0], pos[1]
+    water = y < 1.5
+    bubble = water & ((x - 2) ** 2 + (y - 1.2) ** 2 < 0.12 * 0.12)
+    phase = np.where(water & ~bubble, 1, 2).astype(np.int32)
+    rho = np.where(water, 1000.0, 1.225)
+    p = np.full(n, 1.01325e5)
+    rho[bubble], p[bubble] = 1250.0, 1e9
+    vel = np.zeros((2, n))
+    pg = [round((box_r[a] - box_l[a]) / case.p_sz) for a in range(2)]
+    spacing = [(box_r[a] - box_l[a]) / pg[a] for a in range(2)]
+    wx, wn, wv = _walls_nodes(case, spacing, case.p_sz)
+    st = HostState(case, int(n * cap_factor) + 64, wx.shape[1])
+    _fill_particles(st, case, pos, phase, rho, p, vel)
+    vol = st.p["vol"][:n]
+    st.p["mass"][:n][bubble] = vol[bubble] * case.ini_rho[1]
+    st.w["x"][:] = wx
+    st.w["norm"][:] = wn
+    st.w["vol"][:] = wv
+    _init_cells(st, case)
+    for s_, p0 in ((0, 1.01325e5), (1, 1e9)):
+        st.c["c"][s_] = math.sqrt(case.gamma[s_] * (case.pinf[s_] + p0) / case.ini_rho[s_])
     return case, st
 
 

@@ -855,8 +855,19 @@ int pgdg_create(const pgdg_case* c, int64_t particle_cap, int64_t n_wall, pgdg_c
   if (c->rk_stages != 2 && c->rk_stages != 3) return PGDG_ERR_ARG;
   if (c->cfl < 0 || c->ppc_min < 0 || c->ppc_max < 0) return PGDG_ERR_ARG;
   if (c->ppc_max > 0 && c->ppc_min > c->ppc_max) return PGDG_ERR_ARG;
-  if (c->semantics != PGDG_SEM_T2 && c->semantics != PGDG_SEM_S3) return PGDG_ERR_ARG;
+  if (c->semantics != PGDG_SEM_T2 && c->semantics != PGDG_SEM_S3 && c->semantics != PGDG_SEM_UX)
+    return PGDG_ERR_ARG;
   if (c->precision != PGDG_PREC_FP64 && c->precision != PGDG_PREC_MIXED) return PGDG_ERR_ARG;
+  if (c->semantics == PGDG_SEM_UX) {
+    // the underwater-explosion routines: 2-D, two phases, FP64 form; x sides
+    // transmissive, y sides transmissive or reflecting (the reference's
+    // tags -1 and 2)
+    if (c->dim != 2 || c->nphase != 2 || c->precision != PGDG_PREC_FP64) return PGDG_ERR_ARG;
+    for (int f = 0; f < 4; f++) {
+      const bool yside = f == PGDG_SIDE_YL || f == PGDG_SIDE_YR;
+      if (c->bc[f] != PGDG_BC_OUTLET && !(yside && c->bc[f] == PGDG_BC_WALL)) return PGDG_ERR_ARG;
+    }
+  }
   // every domain side needs a boundary treatment (interior is no tag for a
   // side of the domain); sides 4 and 5 (z) only in 3-D
   for (int f = 0; f < 2 * c->dim; f++)

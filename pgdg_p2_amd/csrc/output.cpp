@@ -124,8 +124,10 @@ int pgdg_plt_write_cells(const char* path, const pgdg_case* c, const pgdg_cells*
   if (!out) return PGDG_ERR_ARG;
   // one line: the reference's string literal continues across a source line
   // (spliced), leaving eight spaces before "E0"
+  const bool ux = c->semantics == PGDG_SEM_UX;
   out << "VARIABLES = \"x\",\"y\",\"z\", \"v1\",\"v2\", \"v\",\"rho0\",\"rho1\", "
-         "       \"E0\",\"E1\",\"Lim0\",\"Lim1\",\"st\", \"st1\", \"P\"\n";
+         "       \"E0\",\"E1\",\"Lim0\",\"Lim1\",\"st\", \"st1\", "
+      << (ux ? "\"ca\", \"P\"\n" : "\"P\"\n");
   out << "ZONE T=" << "\"GRID 1\"," << "SOLUTIONTIME=" << run_time << "\n";
   for (int i = 0; i < n0; i++)
     for (int j = 0; j < n1; j++)
@@ -140,6 +142,12 @@ int pgdg_plt_write_cells(const char* path, const pgdg_case* c, const pgdg_cells*
           // S3: the x-momentum row's x and y slopes, the density limiters
           out << cl->coef[((size_t)1 * M + 1) * nc + ci] << ' ' << cl->coef[((size_t)1 * M + 2) * nc + ci] << ' '
               << cl->lim[0 * nc + ci] << ' ' << cl->lim[(size_t)U * nc + ci] << ' ';
+        } else if (ux) {
+          // UX: phase 2's y-momentum x and y slopes, the energy limiters
+          // (UX:mpm_ini.cpp:781-789)
+          out << cl->coef[((size_t)(1 * U + 2) * M + 1) * nc + ci] << ' '
+              << cl->coef[((size_t)(1 * U + 2) * M + 2) * nc + ci] << ' ' << cl->lim[(size_t)3 * nc + ci] << ' '
+              << cl->lim[((size_t)U + 3) * nc + ci] << ' ';
         } else {
           // T2: each phase's speed, the energy limiters
           double s[2];
@@ -151,7 +159,9 @@ int pgdg_plt_write_cells(const char* path, const pgdg_case* c, const pgdg_cells*
           out << s[0] << ' ' << s[1] << ' ' << cl->lim[(size_t)3 * nc + ci] << ' '
               << cl->lim[((size_t)U + 3) * nc + ci] << ' ';
         }
-        out << cl->alph[ci] << ' ' << cl->alph[nc + ci] << ' ' << cl->P[ci] << ' ' << std::endl;
+        out << cl->alph[ci] << ' ' << cl->alph[nc + ci] << ' ';
+        if (ux) out << 0.0 << ' ';  // the cell's ca, never assigned (UX:particle.h:58)
+        out << cl->P[ci] << ' ' << std::endl;
       }
   return out.good() ? PGDG_OK : PGDG_ERR_ARG;
 }

@@ -159,9 +159,13 @@ __device__ inline void p2g_finish(const Geo& g, const CellDev& C, int c, const d
           C.coef[CIDX(ss, u, m)] = 0;
           C.coef0[CIDX(ss, u, m)] = 0;
         }
-      C.rho[ss * nc + c] = g.ini_rho[ss];
+      // T2/S3 reset the absent phase's density and momentum
+      // (T2:mpm.cpp:346-352); UX keeps them (UX:mpm.cpp:330-334)
+      if (g.sem != PGDG_SEM_UX) {
+        C.rho[ss * nc + c] = g.ini_rho[ss];
 #pragma unroll
-      for (int a = 0; a < 3; a++) C.rhov[(ss * 3 + a) * nc + c] = 0;
+        for (int a = 0; a < 3; a++) C.rhov[(ss * 3 + a) * nc + c] = 0;
+      }
     }
   }
 #pragma unroll
@@ -444,10 +448,17 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
     for (int b = 0; b < 3; b++) G[a][b] = 0.;
   int bd_fg = 0;
   double vc[3] = {C.v[c], C.v[nc + c], C.v[2 * nc + c]};
+  const bool ux = g.sem == PGDG_SEM_UX;
   {
-    if (i == g.N[0] - 1 || i == 0) {
+    // UX takes a zero wall velocity on the x sides (UX:mpm.cpp:388-399);
+    // T2/S3 the cell's own (a zero difference)
+    if (i == g.N[0] - 1) {
 #pragma unroll
-      for (int a = 0; a < D; a++) G[a][0] = (vc[a] - vc[a]) / (0.5 * g.ps[0]);
+      for (int a = 0; a < D; a++) G[a][0] = ux ? (0.0 - vc[a]) / (0.5 * g.ps[0]) : (vc[a] - vc[a]) / (0.5 * g.ps[0]);
+      bd_fg = 1;
+    } else if (i == 0) {
+#pragma unroll
+      for (int a = 0; a < D; a++) G[a][0] = ux ? (vc[a] - 0.0) / (0.5 * g.ps[0]) : (vc[a] - vc[a]) / (0.5 * g.ps[0]);
       bd_fg = 1;
     } else {
       int cr = c + g.N[1] * g.N[2], cl = c - g.N[1] * g.N[2];
@@ -550,6 +561,15 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
             for (int u = 0; u < U; u++) pj[u] = pi[s][u];
           if (j == g.N[1] - 1 || j == 0) pj[2] = -crv[s][1];
           if (D == 3 && (k == g.N[2] - 1 || k == 0)) pj[3] = -crv[s][2];
+        } else if (ux) {
+          // the cell's own means, the momentum normal to a domain edge set
+          // to 0, whatever the side's tag (UX:mpm.cpp:478-487)
+          pj[0] = crho[s];
+#pragma unroll
+          for (int a = 0; a < D; a++) pj[1 + a] = crv[s][a];
+          pj[U - 1] = pi[s][U - 1];
+          if (i == g.N[0] - 1 || i == 0) pj[1] = 0;
+          if (j == g.N[1] - 1 || j == 0) pj[2] = 0;
         } else {
           if (fg.b == 1) {
             pj[0] = crho[s];
@@ -590,7 +610,8 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
     T lim[U];
 #pragma unroll
     for (int u = 0; u < U; u++) lim[u] = 1.;
-    int lim_fg = bd_fg;
+    // UX's Limiter is 2 ds / (ds + 1) in every cell (UX:glbfunc.h:291-306)
+    int lim_fg = ux ? 1 : bd_fg;
     if (calph[s] != 1) lim_fg = 1;
     for (int f = 0; f < NF; f++) {
       FaceGeo<D> fg;
@@ -693,6 +714,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
   double xc[3];
   cell_center<D>(g, i, j, k, xc);
   const bool s3 = g.sem == PGDG_SEM_S3;
+  const bool ux = g.sem == PGDG_SEM_UX;
   const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
   const double vol64 = D == 3 ? g.ps[0] * g.ps[1] * g.ps[2] : g.ps[0] * g.ps[1];
   const T voli = (T)vol64;
@@ -857,7 +879,8 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
           continue;
         }
       }
-      const bool wall_like = s3 ? (fg.b == 1) : (fg.b != 0);
+      // T2: every boundary face is a wall; S3: tag 1; UX: none (HLL ghost)
+      const bool wall_like = s3 ? (fg.b == 1) : (!ux && fg.b != 0);
       int s_first = -1;
       if (!kFaceBuf && fg.b == 0) {
         for (int s1 = 0; s1 < PGDG_MAX_PHASE && s_first < 0; s1++)
@@ -999,9 +1022,25 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
               for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
               T SL = amin1(vni - ci, vnj - cj);
               T SR = amax1(vni + ci, vnj + cj);
+              T Fh[U];
+              if (ux) {  // HLL (UX:particle_flow.cpp:280-294)
+                if (SL > 0) {
+#pragma unroll
+                  for (int u = 0; u < U; u++) Fh[u] = DFi[u];
+                } else if (SL <= 0 && SR >= 0) {
+#pragma unroll
+                  for (int u = 0; u < U; u++)
+                    Fh[u] = qdiv(SR * DFi[u] - SL * DFj[u] + SL * SR * (Ujs[u] - Ui[u]) * sgn, SR - SL);
+                } else if (SR < 0) {
+#pragma unroll
+                  for (int u = 0; u < U; u++) Fh[u] = DFj[u];
+                } else {
+#pragma unroll
+                  for (int u = 0; u < U; u++) Fh[u] = 0;
+                }
+              } else {
               T S_s = ((P_j - P_i) + rho_i * vni * (SL - vni) - rho_j * vnj * (SR - vnj)) /
                            (rho_i * (SL - vni) - rho_j * (SR - vnj));
-              T Fh[U];
               if (SL > 0) {
 #pragma unroll
                 for (int u = 0; u < U; u++) Fh[u] = DFi[u];
@@ -1030,6 +1069,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
 #pragma unroll
                 for (int u = 0; u < U; u++) Fh[u] = 0;
               }
+              }
 #pragma unroll
               for (int bi = 0; bi < M; bi++)
 #pragma unroll
@@ -1049,7 +1089,8 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
                   else DFn[bi][1 + a] += t;
                 }
             }
-          } else {  // S3 inlet / outlet / symmetry: HLL against a ghost state
+          } else {  // S3 inlet / outlet / symmetry and every UX boundary face: HLL
+                    // against a ghost state (UX:particle_flow.cpp:314-366)
             T rho_j = rho_i;
             T vj[3] = {0, 0, 0}, rvj[3] = {0, 0, 0};
 #pragma unroll
@@ -1057,7 +1098,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
 #pragma unroll
             for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];
             T P_j = P_i, E_j = E_i;
-            if (fg.b == -1) {
+            if (fg.b == -1 && s3) {
               rho_j = g.inlet_rho;
 #pragma unroll
               for (int a = 0; a < D; a++) vj[a] = g.inlet_u[a];
@@ -1069,7 +1110,9 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
               for (int a = 0; a < D; a++) sqv += vj[a] * vj[a];
               E_j = mt.rhoe(P_j) + T(0.5) * sqv * rho_j;
             }
-            if (j == g.N[1] - 1 || j == 0) {
+            // S3 reflects v_y in the first and last y rows; UX on wall-tagged
+            // sides (its tag 2)
+            if (ux ? fg.b == PGDG_BC_WALL : (j == g.N[1] - 1 || j == 0)) {
               vj[1] = -vi[1];
 #pragma unroll
               for (int a = 0; a < D; a++) rvj[a] = rho_j * vj[a];

@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 PGDG_MAX_PHASE = 3
-SEM_T2, SEM_S3 = 2, 3
+SEM_T2, SEM_S3, SEM_UX = 2, 3, 4
 BC_INTERIOR, BC_WALL, BC_INLET, BC_OUTLET, BC_SYMMETRY = 0, 1, -1, -2, -3
 
 

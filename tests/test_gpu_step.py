@@ -356,3 +356,28 @@ def test_step_parity_three_phase_p2_448x192_lattice_positivity(cuda):
                                                  ppc_max=27))
     a, events = run_pair(case, st, 20, "tp3-448x192-p2")
     assert np.isfinite(a.p["x"][:2, :a.n]).all() and np.isfinite(a.c["coef"]).all()
+
+
+@pytest.mark.gpu
+def test_step_parity_underwater_explosion_2d(cuda):
+    # the reference's UX routines (HLL faces, HLL ghost boundaries, its
+    # limiter and P2G) on its case at 80 x 60 cells, P1 and two RK stages as
+    # the reference ships them; splits and merges start within 30 steps
+    case, st = cases.underwater_explosion_2d(nx=80)
+    a, events = run_pair(case, st, 30, "ux-80x60")
+    assert events > 0
+
+
+@pytest.mark.gpu
+def test_step_parity_p2_rk3_underwater_explosion_2d(cuda):
+    case, st = cases.underwater_explosion_2d(nx=80, scheme=dict(order=2, rk_stages=3, ppc_min=1,
+                                                                  ppc_max=9))
+    run_pair(case, st, 20, "ux-80x60-p2")
+
+
+@pytest.mark.gpu
+def test_step_parity_underwater_explosion_reference_grid(cuda):
+    # the reference's own resolution: 400 x 300 cells, one particle per cell
+    case, st = cases.underwater_explosion_2d(nx=400)
+    a, _ = run_pair(case, st, 8, "ux-400x300")
+    assert np.isfinite(a.c["coef"]).all()

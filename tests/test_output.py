@@ -20,8 +20,10 @@ import pytest
 from pgdg_p2_amd import cases, output
 
 
-def _state(dim, seed=1):
-    if dim == 2:
+def _state(dim, seed=1, ux=False):
+    if ux:
+        case, st = cases.underwater_explosion_2d(nx=4)
+    elif dim == 2:
         case, st = cases.triple_point_2d(nx=4, ny=2, ppc_axis=1, jitter=0.0)
     else:
         case, st = cases.shock_bubble_3d(nx=2, ny=1, nz=1, ppc_axis=2, jitter=0.25)
@@ -90,9 +92,10 @@ def _g(x):
     return "%g" % x
 
 
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [2, 3, "ux"])
 def test_cell_file_text(tmp_path, dim):
-    case, st = _state(dim)
+    ux = dim == "ux"
+    case, st = _state(2 if ux else dim, ux=ux)
     rng = np.random.default_rng(7)
     c = st.c
     for f in ("v", "rho", "alph", "lim", "coef", "rhov"):
@@ -101,8 +104,9 @@ def test_cell_file_text(tmp_path, dim):
     path = os.path.join(tmp_path, "ele.plt")
     output.write_cells_plt(path, case, st, 0.25)
     lines = open(path).read().split("\n")
+    # UX adds the cells' ca column, never assigned (UX:mpm_ini.cpp:771-789)
     assert lines[0] == ('VARIABLES = "x","y","z", "v1","v2", "v","rho0","rho1",        '
-                        '"E0","E1","Lim0","Lim1","st", "st1", "P"')
+                        '"E0","E1","Lim0","Lim1","st", "st1", ' + ('"ca", "P"' if ux else '"P"'))
     assert lines[1] == 'ZONE T="GRID 1",SOLUTIONTIME=0.25'
     D = case.dim
     n0, n1 = case.n[0], case.n[1]
@@ -119,15 +123,17 @@ def test_cell_file_text(tmp_path, dim):
                 rho = c["rho"].reshape(case.nphase, -1)[:, ci]
                 vals = [x[0], x[1], x[2], v[0], v[1], vn, rho[0], rho[1]]
                 lim = c["lim"].reshape(case.nphase, D + 2, -1)
+                coef = c["coef"].reshape(case.nphase, D + 2, -1, c["coef"].shape[-1])
                 if D == 3:
-                    coef = c["coef"].reshape(case.nphase, D + 2, -1, c["coef"].shape[-1])
                     vals += [coef[0, 1, 1, ci], coef[0, 1, 2, ci], lim[0, 0, ci], lim[1, 0, ci]]
+                elif ux:  # phase 2's y-momentum slopes (UX:mpm_ini.cpp:784)
+                    vals += [coef[1, 2, 1, ci], coef[1, 2, 2, ci], lim[0, 3, ci], lim[1, 3, ci]]
                 else:
                     rv = c["rhov"].reshape(case.nphase, 3, -1)
                     sp = [math.sqrt(sum((rv[f, a, ci] / rho[f]) ** 2 for a in range(D))) for f in range(2)]
                     vals += [sp[0], sp[1], lim[0, 3, ci], lim[1, 3, ci]]
                 al = c["alph"].reshape(case.nphase, -1)[:, ci]
-                vals += [al[0], al[1], c["P"].reshape(-1)[ci]]
+                vals += [al[0], al[1]] + ([0.0] if ux else []) + [c["P"].reshape(-1)[ci]]
                 assert lines[k] == " ".join(_g(t) for t in vals) + " ", (k, lines[k])
                 k += 1
     assert lines[k] == "" and k == len(lines) - 1

@@ -140,17 +140,12 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
     thr = fmaf(-vb[a], vb[a], thr);
   }
   if (x0 >= x1) return;
-  // the next record is loaded one iteration ahead (the last iteration reads
-  // one record past x1: both record tables carry a spare record at the end)
-  float4 c1n = rec1(x0);
-  for (int x = x0; x < x1; x++) {
-    const float4 c1 = c1n;
-    c1n = rec1(x + 1);
+  auto pair = [&](int x, const float4 c1) {
     const float cb[3] = {c1.x, c1.y, c1.z};
     float t = c1.w;
 #pragma unroll
     for (int a = 0; a < D; a++) t = fmaf(m2vb[a], cb[a], t);
-    if (!(t <= thr)) continue;
+    if (!(t <= thr)) return;
     const float4 c2 = rec2(x);
     const float cu[3] = {c2.x, c2.y, c2.z};
     float dr[3] = {0, 0, 0};
@@ -192,7 +187,19 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
 #pragma unroll
       for (int a = 0; a < D; a++) pst[a] = fmaf((float)W.norm[a * W.n + w], fw, pst[a]);
     }
+  };
+  // two candidates per iteration, each record loaded one candidate ahead
+  // (the reads stay within x1 + 1: both record tables carry a spare record
+  // at the end)
+  float4 ca = rec1(x0);
+  int x = x0;
+  for (; x + 1 < x1; x += 2) {
+    const float4 cb_ = rec1(x + 1);
+    pair(x, ca);
+    ca = rec1(x + 2);
+    pair(x + 1, cb_);
   }
+  if (x < x1) pair(x, ca);
 }
 
 // One lane per bin entry (target); workgroups take runs of consecutive

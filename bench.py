@@ -343,6 +343,7 @@ def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12, mlp_save=None, mlp_lo
         nm = MlpNorm()
         for f in ("feat_mu", "feat_sd", "out_mu", "out_sd"):
             getattr(nm, f)[:] = z[f].tolist()
+        nm.keep = float(z["keep"]) if "keep" in z else 0.0
         mlp.set_params(z["params"], nm)
         res["trained_network"] = str(mlp_load)
     else:
@@ -356,7 +357,7 @@ def mgfm_bench(n=10_000_000, n_train=2_000_000, epochs=12, mlp_save=None, mlp_lo
         if mlp_save:
             Pm, nm = mlp.get_params()
             np.savez(mlp_save, params=Pm, feat_mu=np.array(nm.feat_mu), feat_sd=np.array(nm.feat_sd),
-                     out_mu=np.array(nm.out_mu), out_sd=np.array(nm.out_sd))
+                     out_mu=np.array(nm.out_mu), out_sd=np.array(nm.out_sd), keep=np.float32(nm.keep))
     inp = P.DeviceArray.from_numpy(workloads.mgfm_faces(n))
     out = P.DeviceArray((3, n), np.float64)
     its = P.DeviceArray((n,), np.int32)

@@ -120,15 +120,17 @@ int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream);
  *  counterpart: the reference starts from max(pL, pR), S3:mpm.cpp:891)    *
  * ======================================================================== */
 
-/* A 4-layer MLP 10 -> 32 -> 32 -> 32 -> 2 (ReLU) mapping the standardised
+/* A 4-layer MLP 12 -> 32 -> 32 -> 32 -> 2 (ReLU) mapping the standardised
  * face features (ln pL, ln pR, ln rhoL, ln rhoR, uL, uR, gammaL,
- * ln(1 + PinfL), gammaR, ln(1 + PinfR)) to the standardised targets
+ * ln(1 + PinfL), gammaR, ln(1 + PinfR), ln(pL / pR), uL - uR; the last two
+ * formed in FP64 so that small jumps survive the BF16 inputs) to the
+ * standardised targets
  * (ln p* - ln max(pL, pR), u*).  Matrix products run on MFMA with BF16
  * operands and FP32 accumulation; the master parameters and the optimiser
  * (Adam) are FP32.  Parameter vector layout (FP32, PGDG_MLP_NPARAM):
- * W0[16][32] (rows 10..15 zero), b0[32], W1[32][32], b1[32], W2[32][32],
+ * W0[16][32] (rows 12..15 zero), b0[32], W1[32][32], b1[32], W2[32][32],
  * b2[32], W3[32][16] (columns 2..15 zero), b3[16] (entries 2..15 zero).  */
-#define PGDG_MLP_NIN     10
+#define PGDG_MLP_NIN     12
 #define PGDG_MLP_NPAD    16
 #define PGDG_MLP_WIDTH   32
 #define PGDG_MLP_NOUT    2
@@ -139,6 +141,11 @@ typedef struct pgdg_mlp pgdg_mlp;
 typedef struct pgdg_mlp_norm {
   float feat_mu[PGDG_MLP_NIN], feat_sd[PGDG_MLP_NIN];
   float out_mu[PGDG_MLP_NOUT], out_sd[PGDG_MLP_NOUT];
+  /* A predicted |ln(p*0 / max(pL, pR))| below keep leaves the initial p* at
+   * max(pL, pR): the network cannot place p* closer than its own residual
+   * there.  Training sets keep to twice the residual RMS of that output on
+   * the training faces; 0 always takes the network's p*0.               */
+  float keep;
 } pgdg_mlp_norm;
 
 typedef struct pgdg_mlp_train_opts {
@@ -166,6 +173,20 @@ int pgdg_mlp_guess(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
  * the requested tolerance).                                                */
 int pgdg_mgfm_batch_learned(pgdg_mlp* m, const double* in, double* out, int32_t* iters, int64_t n,
                             const pgdg_eos_pair* eos, const pgdg_mgfm_opts* opts, void* stream);
+/* Faces whose media differ from face to face: pair[i] (device, int32, may be
+ * NULL when npairs == 1) selects eos[pair[i]] for face i, as the reference's
+ * MGFM(ul, ur, pl, pr, rhol, rhor, s, s1) (S3:mpm.cpp:882; UX:mpm.cpp:739)
+ * takes the two media with every face.  pgdg_mlp_train_pairs: as
+ * pgdg_mlp_train over such faces (npairs <= PGDG_MAX_PHASE^2; the EOS
+ * features of each face enter the network).  pgdg_mgfm_batch_pairs: the
+ * batched solve over such faces, from the network's p*0 when m is given
+ * (trained), else from the variant's own.                                 */
+int pgdg_mlp_train_pairs(pgdg_mlp* m, const double* in, const int32_t* pair, int64_t n,
+                         const pgdg_eos_pair* eos, int npairs, const pgdg_mlp_train_opts* opts,
+                         float* loss_hist, int hist_cap, int* steps, void* stream);
+int pgdg_mgfm_batch_pairs(pgdg_mlp* m, const double* in, const int32_t* pair, double* out,
+                          int32_t* iters, int64_t n, const pgdg_eos_pair* eos, int npairs,
+                          const pgdg_mgfm_opts* opts, void* stream);
 /* Parameters and normalisation (host buffers).                             */
 int pgdg_mlp_get_params(pgdg_mlp* m, float* params, pgdg_mlp_norm* norm);
 int pgdg_mlp_set_params(pgdg_mlp* m, const float* params, const pgdg_mlp_norm* norm);
@@ -430,6 +451,35 @@ int pgdg_pst_staging_enable(pgdg_ctx* ctx, int on);
  * hide it against the vector kernel's more (6.0 against 4.5 ms per stage
  * at 256x128x128 on MI355X).                                             */
 int pgdg_rhs_mfma_enable(pgdg_ctx* ctx, int on);
+/* Mixed form: how the step solves MGFM at its interface face points (the
+ * reference calls MPM::MGFM once per face point inside the face integral,
+ * S3:particle_flow.cpp:192-204).
+ *   PGDG_IFACE_INLINE  (default) each point's Newton solve inside the face
+ *                      kernel, from the reference's initial p*;
+ *   PGDG_IFACE_BATCH   the stage's points collected as records and solved
+ *                      as one lane-refilling batch, from the reference's
+ *                      initial p* (the same solutions bit for bit);
+ *   PGDG_IFACE_LEARNED as BATCH, from the trained network m's p*0 (no
+ *                      reference counterpart); the Newton iteration runs to
+ *                      the reference's tolerance, so the solutions differ
+ *                      only within it.  The context reads m's parameters
+ *                      when a stage runs: set the mode again after
+ *                      retraining m (its normalisation is copied here).
+ * PGDG_ERR_ARG for the FP64 form (which keeps the reference's solve).      */
+#define PGDG_IFACE_INLINE  0
+#define PGDG_IFACE_BATCH   1
+#define PGDG_IFACE_LEARNED 2
+int pgdg_iface_mgfm_mode(pgdg_ctx* ctx, int mode, pgdg_mlp* m);
+/* The EOS pair of each record pair code a * PGDG_MAX_PHASE + b (left medium
+ * phase a, right medium phase b): eos[PGDG_MAX_PHASE^2], host.             */
+int pgdg_iface_pairs(pgdg_ctx* ctx, pgdg_eos_pair* eos);
+/* The interface-point records of the last stage solved in a batch mode:
+ * *n records; the first min(*n, cap) are copied (any memory) as compact SoA
+ * in[6][k] (ul, ur, pl, pr, rhol, rhor), out[3][k] (rho*_L, p*, u*_L),
+ * pair[k], iters[k] (each may be NULL) - the training set for
+ * pgdg_mlp_train_pairs and the iteration counts of the step's solves.     */
+int pgdg_iface_records(pgdg_ctx* ctx, double* in, double* out, int32_t* pair, int32_t* iters, int64_t cap,
+                       int64_t* n);
 int pgdg_profile_read(pgdg_ctx* ctx, double* ms, int64_t* launches, int nslot);
 
 /* ======================================================================== *

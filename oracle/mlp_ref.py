@@ -2,7 +2,7 @@
 network (pgdg_p2_amd/csrc/mlp.hip) for checking the GPU forward pass and
 gradients.  Not used by the product.
 
-The network is 10 -> 32 -> 32 -> 32 -> 2 with ReLU, parameters in the flat
+The network is 12 -> 32 -> 32 -> 32 -> 2 with ReLU, parameters in the flat
 layout of include/pgdg.h (PGDG_MLP_NPARAM).  With emulate=True the BF16
 roundings of the GPU kernels are applied at the same points (inputs,
 weights, activations, output and hidden gradients entering a matrix
@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import numpy as np
 
-NIN, NPAD, H, NOUT, OPAD = 10, 16, 32, 2, 16
+NIN, NPAD, H, NOUT, OPAD = 12, 16, 32, 2, 16
 OW0 = 0
 OB0 = OW0 + NPAD * H
 OW1 = OB0 + H

@@ -27,6 +27,11 @@ def declare(L: C.CDLL) -> None:
     L.pgdg_autotune_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_pst_staging_enable.argtypes = [_CTX, C.c_int]
     L.pgdg_rhs_mfma_enable.argtypes = [_CTX, C.c_int]
+    L.pgdg_iface_mgfm_mode.argtypes = [_CTX, C.c_int, C.c_void_p]
+    from . import EosPair
+    L.pgdg_iface_pairs.argtypes = [_CTX, C.POINTER(EosPair)]
+    L.pgdg_iface_records.argtypes = [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.POINTER(C.c_int64)]
     L.pgdg_neighbor_counts.argtypes = [_CTX, C.POINTER(C.c_int32)]
     L.pgdg_neighbor_counts_noalloc.argtypes = [_CTX, C.POINTER(C.c_int32)]
     L.pgdg_used_cells.argtypes = [_CTX, C.POINTER(C.c_int32)]

@@ -102,9 +102,20 @@ __device__ inline size_t face_slot(const Geo& g, int side, int ss, int q, int u,
 // the weights and alpha_j.  Used by the mixed form only (T = float, with the
 // hardware reciprocal and square root); the FP64 form keeps the reference's
 // operations inside its cell kernel.
-template <int D, typename T, bool kSamePhase = false>
+// The MGFM solve of a ghost state, in place (the reference's call and
+// initial guess).
+struct MgfmInline {
+  __device__ MgfmOut operator()(double ul, double ur, double pl, double pr, double rhol, double rhor, Eos ml,
+                                Eos mr, int variant) const {
+    return mgfm_solve(ul, ur, pl, pr, rhol, rhor, ml, mr, variant, TOL_REF, 0.0, 50, -1.0);
+  }
+};
+
+// mg: the MGFM solve of the ghost state (MgfmInline, or a batched solve's
+// record).
+template <int D, typename T, bool kSamePhase = false, typename MG = MgfmInline>
 __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss, int s1, int ax, T sgn,
-                                  T* Fh) {
+                                  T* Fh, MG mg = MG{}) {
   constexpr int U = D + 2;
   const bool s3 = g.sem == PGDG_SEM_S3;
   const Eos me{g.gamma[ss], g.pinf[ss]};
@@ -147,8 +158,7 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
 #pragma unroll
   for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
   if (!kSamePhase && ss != s1) {
-    MgfmOut o = mgfm_solve(vni, vnj, P_i, P_j, rho_i, rho_j, me, Eos{g.gamma[s1], g.pinf[s1]},
-                           s3 ? MGFM_S3 : MGFM_T2, TOL_REF, 0.0, 50, -1.0);
+    MgfmOut o = mg(vni, vnj, P_i, P_j, rho_i, rho_j, me, Eos{g.gamma[s1], g.pinf[s1]}, s3 ? MGFM_S3 : MGFM_T2);
     rho_j = o.rho_l;
     P_j = o.pstar;
     vnj = o.u_l;

@@ -126,6 +126,11 @@ struct Ctx {
   MixedBins MB{};
   void* mb_slab = nullptr;
   void* ff_slab = nullptr;
+  void* mg_slab = nullptr;
+  size_t mg_bytes = 0;
+  // mixed form: how the step's interface points solve MGFM (in the face
+  // kernel, or as a batch from the reference's or the network's p*)
+  IfaceBatch ib;
   int32_t* bi_slab = nullptr;
   Topo T;
   DevScalars* sc = nullptr;
@@ -210,6 +215,8 @@ struct Ctx {
   int set_timestep();
   template <int D_>
   int run_simulation(int64_t n);
+  int fit_iface_records();
+  int alloc_iface_records(int64_t cap_rec);
   template <int D_>
   int stage(int fn, int arg);
   template <int D_>
@@ -331,6 +338,10 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
     C.bflux = D == 3 ? C.fflux + nflux : nullptr;
     C.flist = (int32_t*)(C.fflux + nflux + nbflux);
     C.fcount = C.flist + (size_t)D * nc;
+    // interface-point MGFM records for the batched solves: inputs [6] and
+    // solutions [3] (FP64), pair code and iterations per record, the first
+    // record of each listed face [D][nc], the record count and its peak
+    PGDG_TRY(alloc_iface_records(std::max<int64_t>(65536, nc)));
   }
   // stage buffers: xb[3], hb, x0[3], vol0
   PGDG_HIP_TRY(hipMalloc((void**)&s_slab, sizeof(double) * 8 * cap));
@@ -385,7 +396,7 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PGDG_HIP_TRY(hipStreamSynchronize(st));
   bytes = sizeof(double) * (2 * (size_t)PA.rows64 * cap + nd + 8 * cap + 7 * nwall +
                             9 * (size_t)ne + 6 * (size_t)nc) +
-          p2g_bytes + T.bytes_total + mb_bytes + ff_bytes;
+          p2g_bytes + T.bytes_total + mb_bytes + ff_bytes + mg_bytes;
   return PGDG_OK;
 }
 
@@ -395,7 +406,7 @@ void Ctx::release() {
   PA.release();
   T.release();
   void* ptrs[] = {w_slab, c_slab, ci_slab, used, s_slab, S.in_ele, sc, b_slab, bi_slab, PB.light, mb_slab,
-                  ff_slab};
+                  ff_slab, mg_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (own_stream && st) (void)hipStreamDestroy(st);
@@ -667,9 +678,9 @@ int Ctx::stage_impl(int fn, int arg) {
       const int ra = fuse ? (arg & 3) : -1;
       if (tuning_now)
         PGDG_TRY(pick_block([&](int b) {
-          return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, -1, st, b);
+          return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, -1, st, b, &ib);
         }, &tune.rhs));
-      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, ra, st, tune.rhs);
+      return launch_cells<D_>(CELL_RHS, g, C, P, T.bin_off, T.bin_ent, used, sc, ra, st, tune.rhs, &ib);
     }
     case FN_UPDATE_GRID: {
       const int ua = (arg & 3) | (rhs_fused ? kCellSkipCoef : 0);
@@ -781,6 +792,42 @@ int Ctx::run_simulation(int64_t n) {
   if (prof.on) PGDG_TRY(prof.begin(PS_LISTS, st, r));
   PGDG_TRY(T.end_of_step_lists<D_>(g, PA.view(), S.xb, S.hb, W, np));
   if (prof.on) PGDG_TRY(prof.end(r, st));
+  if (ib.mode != IFACE_INLINE) PGDG_TRY(fit_iface_records());
+  return PGDG_OK;
+}
+
+// The interface-point records of a batched stage live in a buffer sized
+// for the largest count seen: a face whose records did not fit solved its
+// points in the face kernel (same solutions with the reference's initial
+// p*), and the buffer grows before the next step.
+int Ctx::fit_iface_records() {
+  int32_t peak = 0;
+  PGDG_HIP_TRY(hipMemcpyAsync(&peak, C.mg_peak, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PGDG_HIP_TRY(hipStreamSynchronize(st));
+  if (peak <= C.mg_cap) return PGDG_OK;
+  return alloc_iface_records((int64_t)peak + peak / 4);
+}
+
+int Ctx::alloc_iface_records(int64_t cap_rec) {
+  if (mg_slab) {
+    PGDG_HIP_TRY(hipStreamSynchronize(st));
+    drop_graphs();  // the stage graphs hold the buffer's addresses
+    (void)hipFree(mg_slab);
+    mg_slab = nullptr;
+  }
+  C.mg_cap = cap_rec;
+  const size_t bytes_ = sizeof(double) * 9 * (size_t)cap_rec +
+                        sizeof(int32_t) * (2 * (size_t)cap_rec + (size_t)D * nc + 64);
+  PGDG_HIP_TRY(hipMalloc(&mg_slab, bytes_));
+  PGDG_HIP_TRY(hipMemsetAsync(mg_slab, 0, bytes_, st));
+  C.mg_in = (double*)mg_slab;
+  C.mg_out = C.mg_in + 6 * (size_t)cap_rec;
+  C.mg_pair = (int32_t*)(C.mg_out + 3 * (size_t)cap_rec);
+  C.mg_it = C.mg_pair + cap_rec;
+  C.mg_base = C.mg_it + cap_rec;
+  C.mg_count = C.mg_base + (size_t)D * nc;
+  C.mg_peak = C.mg_count + 1;
+  mg_bytes = bytes_;
   return PGDG_OK;
 }
 
@@ -1261,6 +1308,61 @@ int pgdg_rhs_mfma_enable(pgdg_ctx* x, int on) {
   if (!x) return PGDG_ERR_ARG;
   x->c.g.mfma_rhs = on != 0;
   x->c.drop_graphs();
+  return PGDG_OK;
+}
+
+int pgdg_iface_mgfm_mode(pgdg_ctx* x, int mode, pgdg_mlp* m) {
+  using namespace pgdg;
+  if (!x) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  if (mode != PGDG_IFACE_INLINE && mode != PGDG_IFACE_BATCH && mode != PGDG_IFACE_LEARNED) return PGDG_ERR_ARG;
+  if (mode != PGDG_IFACE_INLINE && c.cs.precision != PGDG_PREC_MIXED) return PGDG_ERR_ARG;
+  IfaceBatch ib;
+  ib.mode = mode;
+  if (mode == PGDG_IFACE_LEARNED) {
+    if (!m) return PGDG_ERR_ARG;
+    PGDG_TRY(mlp_view(m, &ib.P, &ib.nm));
+  }
+  pgdg_eos_pair eos[kMaxPairs];
+  for (int a = 0; a < PGDG_MAX_PHASE; a++)
+    for (int b = 0; b < PGDG_MAX_PHASE; b++)
+      eos[a * PGDG_MAX_PHASE + b] = pgdg_eos_pair{c.g.gamma[a], c.g.pinf[a], c.g.gamma[b], c.g.pinf[b]};
+  ib.tab = make_pair_tab(ib.nm, eos, kMaxPairs);
+  c.ib = ib;
+  c.drop_graphs();  // the stage graphs hold the face launches' arguments
+  return PGDG_OK;
+}
+
+int pgdg_iface_pairs(pgdg_ctx* x, pgdg_eos_pair* eos) {
+  if (!x || !eos) return PGDG_ERR_ARG;
+  const pgdg::Geo& g = x->c.g;
+  for (int a = 0; a < PGDG_MAX_PHASE; a++)
+    for (int b = 0; b < PGDG_MAX_PHASE; b++)
+      eos[a * PGDG_MAX_PHASE + b] = pgdg_eos_pair{g.gamma[a], g.pinf[a], g.gamma[b], g.pinf[b]};
+  return PGDG_OK;
+}
+
+int pgdg_iface_records(pgdg_ctx* x, double* in, double* out, int32_t* pair, int32_t* iters, int64_t cap,
+                       int64_t* n) {
+  if (!x || !n || cap < 0) return PGDG_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.C.mg_count) return PGDG_ERR_STATE;
+  int32_t cnt = 0;
+  PGDG_HIP_TRY(hipMemcpyAsync(&cnt, c.C.mg_count, sizeof(int32_t), hipMemcpyDeviceToHost, c.st));
+  PGDG_HIP_TRY(hipStreamSynchronize(c.st));
+  const int64_t ld = c.C.mg_cap;
+  const int64_t have = std::min<int64_t>(cnt, ld);
+  *n = have;
+  const int64_t k = std::min(have, cap);
+  if (k == 0) return PGDG_OK;
+  // compact SoA copies with leading dimension k
+  for (int q = 0; q < 6 && in; q++)
+    PGDG_HIP_TRY(hipMemcpyAsync(in + q * k, c.C.mg_in + q * ld, sizeof(double) * k, hipMemcpyDefault, c.st));
+  for (int q = 0; q < 3 && out; q++)
+    PGDG_HIP_TRY(hipMemcpyAsync(out + q * k, c.C.mg_out + q * ld, sizeof(double) * k, hipMemcpyDefault, c.st));
+  if (pair) PGDG_HIP_TRY(hipMemcpyAsync(pair, c.C.mg_pair, sizeof(int32_t) * k, hipMemcpyDefault, c.st));
+  if (iters) PGDG_HIP_TRY(hipMemcpyAsync(iters, c.C.mg_it, sizeof(int32_t) * k, hipMemcpyDefault, c.st));
+  PGDG_HIP_TRY(hipStreamSynchronize(c.st));
   return PGDG_OK;
 }
 

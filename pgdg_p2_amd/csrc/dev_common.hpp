@@ -124,6 +124,16 @@ struct CellDev {
   int32_t *flist, *fcount;
   // mixed form, 3-D: domain-boundary face point fluxes [F][6][NQF][U][plane]
   float* bflux;
+  // mixed form, interface points solved as a batch: the MGFM inputs of every
+  // (listed face, side, phase pair with different phases, face point)
+  // ([6][mg_cap]: ul, ur, pl, pr, rhol, rhor), their pair codes
+  // (ss * PGDG_MAX_PHASE + s1), solutions ([3][mg_cap]) and Newton
+  // iterations; the record count, the largest count any stage reached, and
+  // each listed face's first record (-1: the face kernel solves that face's
+  // points itself)
+  double *mg_in, *mg_out;
+  int32_t *mg_pair, *mg_it, *mg_count, *mg_peak, *mg_base;
+  int64_t mg_cap;
 };
 
 struct WallDev {

@@ -1,6 +1,6 @@
 // mlp.hip - learned initial guess for the MGFM Newton solve.
 //
-// A 10 -> 32 -> 32 -> 32 -> 2 ReLU MLP.  Every matrix product is a set of
+// A 12 -> 32 -> 32 -> 32 -> 2 ReLU MLP.  Every matrix product is a set of
 // 16x16 output tiles computed with v_mfma_f32_16x16x16_bf16 (BF16 operands
 // staged in LDS, FP32 accumulation); a workgroup of four wavefronts handles
 // 64 samples.  Training keeps FP32 master parameters; each workgroup writes
@@ -16,6 +16,7 @@
 
 #include "dev_common.hpp"
 #include "hip_util.hpp"
+#include "mgfm_pairs.hpp"
 #include "riemann.hpp"
 
 namespace pgdg {
@@ -155,6 +156,12 @@ __device__ inline void forward(FwdLds& s, Emit emit) {
   }
 }
 
+// ln(pL / pR) from the FP64 pressures: the two sides' small differences
+// survive the features' BF16 rounding (ln pL and ln pR alone lose them)
+__device__ inline float ln_ratio(double pl, double pr) {
+  return ln_fast((float)(fmax(pl, 1e-30) / fmax(pr, 1e-30)));
+}
+
 // standardised face features (FP32, padded to 16)
 __device__ inline void face_features(const double* __restrict__ in, int64_t n, int64_t i, Eos ml,
                                      Eos mr, const pgdg_mlp_norm& nm, float* f) {
@@ -165,7 +172,7 @@ __device__ inline void face_features(const double* __restrict__ in, int64_t n, i
                           ln_fast(fmaxf((float)rl, 1e-30f)), ln_fast(fmaxf((float)rr, 1e-30f)),
                           (float)ul, (float)ur, (float)ml.gamma,
                           (float)log1p(fmax(ml.pinf, 0.0)), (float)mr.gamma,
-                          (float)log1p(fmax(mr.pinf, 0.0))};
+                          (float)log1p(fmax(mr.pinf, 0.0)), ln_ratio(pl, pr), (float)(ul - ur)};
 #pragma unroll
   for (int k = 0; k < kIn; k++) f[k] = (raw[k] - nm.feat_mu[k]) / nm.feat_sd[k];
 #pragma unroll
@@ -297,7 +304,7 @@ struct MlpRegs {
 };
 
 // Inputs of face i for the lanes of a tile: group 0 (pL, pR, rhoL, rhoR),
-// group 1 (uL, uR); 1.0 past the end.
+// group 1 (uL, uR), group 2 (pL, pR, uL, uR); 1.0 past the end.
 __device__ inline void tile_inputs(const double* __restrict__ in, int64_t n, int64_t i, bool live, double* v) {
   const int g = (threadIdx.x & 63) >> 4;
   v[0] = v[1] = v[2] = v[3] = 1.0;
@@ -310,11 +317,16 @@ __device__ inline void tile_inputs(const double* __restrict__ in, int64_t n, int
   } else if (g == 1) {
     v[0] = in[i];
     v[1] = in[n + i];
+  } else if (g == 2) {
+    v[0] = in[2 * n + i];
+    v[1] = in[3 * n + i];
+    v[2] = in[i];
+    v[3] = in[n + i];
   }
 }
 
 // Standardised features 4 g .. 4 g + 3 of a face from its lane's inputs
-// (fx: the standardised EOS features 6..9, the same for every face).
+// (fx: the standardised EOS features 6..9 of the face's pair of media).
 __device__ inline void tile_features(const double* v, bool live, const pgdg_mlp_norm& nm, float4 fx, float* x) {
   const int g = (threadIdx.x & 63) >> 4;
   x[0] = x[1] = x[2] = x[3] = 0.f;
@@ -331,14 +343,19 @@ __device__ inline void tile_features(const double* v, bool live, const pgdg_mlp_
   } else if (g == 2) {
     x[0] = fx.z;
     x[1] = fx.w;
+    x[2] = (ln_ratio(v[0], v[1]) - nm.feat_mu[10]) * __builtin_amdgcn_rcpf(nm.feat_sd[10]);
+    x[3] = ((float)(v[2] - v[3]) - nm.feat_mu[11]) * __builtin_amdgcn_rcpf(nm.feat_sd[11]);
   }
 }
 
 // Initial p* from the network output y (lanes of group 0): max(pL, pR)
-// exp(y sd + mu), or max(pL, pR) where that is not a positive finite value.
+// exp(y sd + mu), or max(pL, pR) where the predicted correction is within
+// the network's residual (nm.keep) or the product is not a positive finite
+// value.
 __device__ inline double guess_of(float y, const double* v, const pgdg_mlp_norm& nm) {
   const double pm = v[0] >= v[1] ? v[0] : v[1];
   const float lr = y * nm.out_sd[0] + nm.out_mu[0];
+  if (fabsf(lr) < nm.keep) return pm;
   double gs = pm * (double)exp_fast(lr);
   if (!(gs > 0) || !(gs < 1e300)) gs = pm;
   return gs;
@@ -439,6 +456,108 @@ __global__ void __launch_bounds__(kThreads) k_mgfm_learned(
       i = -1;
     }
   }
+}
+
+// MGFM over faces with per-face EOS pairs (pair[i] into the table), the
+// learned initial guess optional (P == nullptr: the variant's own guess).
+// Faces i < n of SoA arrays with leading dimension ld (in[6][ld],
+// out[3][ld]); n is min(*count, ld) when count is given (read on the
+// device, so a captured launch follows the count of each replay).  Same
+// wavefront chunking, guess kept in LDS and lane-refilling Newton loop as
+// k_mgfm_learned; every face runs exactly the iterations of mgfm_solve from
+// its starting point.
+template <bool kNet>
+__global__ void __launch_bounds__(kThreads) k_mgfm_pairs(
+    const float* __restrict__ P, pgdg_mlp_norm nm, PairTab tab, const double* __restrict__ in,
+    const int32_t* __restrict__ pair, double* __restrict__ out, int32_t* __restrict__ iters, int64_t n,
+    int64_t ld, const int32_t* __restrict__ count, int variant, int tol_mode, double rtol, int max_iter) {
+  __shared__ double s_g[kNet ? kThreads / 64 : 1][kNet ? kLearnedChunk : 1];
+  __shared__ PairTab s_tab;
+  if (threadIdx.x == 0) s_tab = tab;
+  if (count) {
+    const int64_t cn = *count;
+    n = cn < ld ? cn : ld;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, f = lane & 15;
+  const int64_t wave = (int64_t)blockIdx.x * (kThreads / 64) + w;
+  const int64_t c0 = wave * kLearnedChunk;
+  const int64_t c1 = c0 + kLearnedChunk < n ? c0 + kLearnedChunk : n;
+  __syncthreads();
+  if constexpr (kNet) {
+    if (c0 < n) {
+      MlpRegs R;
+      R.load(P);
+      const int nt = (int)((c1 - c0 + 15) / 16);
+      for (int t = 0; t < nt; t++) {
+        const int64_t i = c0 + (int64_t)t * 16 + f;
+        const bool live = i < c1;
+        double v[4];
+        tile_inputs(in, ld, i, live, v);
+        const int c = live && pair ? pair[i] : 0;
+        const float4 fx = make_float4(s_tab.fx[c][0], s_tab.fx[c][1], s_tab.fx[c][2], s_tab.fx[c][3]);
+        float x[4];
+        tile_features(v, live, nm, fx, x);
+        const float y = R.run(x);
+        if (g == 0 && live) s_g[w][t * 16 + f] = guess_of(y, v, nm);
+      }
+    }
+    __syncthreads();  // the guesses are read by other lanes of the wavefront
+  }
+  if (c0 >= n) return;
+  int64_t next = c0;  // wave-uniform
+  int64_t i = -1;
+  bool done = false;
+  MgfmState S;
+  Eos ml{1.4, 0.0}, mr{1.4, 0.0};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (;;) {
+    const bool need = !done && i < 0;
+    const unsigned long long needm = __ballot(need);
+    if (needm) {
+      if (need) {
+        const int64_t j = next + __popcll(needm & below);
+        if (j < c1) {
+          i = j;
+          const int c = pair ? pair[j] : 0;
+          ml = s_tab.l[c];
+          mr = s_tab.r[c];
+          mgfm_begin(S, in[j], in[ld + j], in[2 * ld + j], in[3 * ld + j], in[4 * ld + j], in[5 * ld + j],
+                     variant, kNet ? s_g[w][j - c0] : -1.0);
+        } else {
+          done = true;
+        }
+      }
+      next += __popcll(needm);
+    }
+    if (__ballot(!done) == 0ull) break;
+    if (i >= 0 && !mgfm_iterate(S, ml, mr, variant, tol_mode, rtol, max_iter)) {
+      const MgfmOut o = mgfm_end(S, ml, variant, tol_mode);
+      out[i] = o.rho_l;
+      out[ld + i] = o.pstar;
+      out[2 * ld + i] = o.u_l;
+      if (iters) iters[i] = o.iters;
+      i = -1;
+    }
+  }
+}
+
+// Launch of k_mgfm_pairs over capacity ld (count on the device) or n faces.
+int launch_mgfm_pairs(const float* P, const pgdg_mlp_norm& nm, const PairTab& tab, const double* in,
+                      const int32_t* pair, double* out, int32_t* iters, int64_t n, int64_t ld,
+                      const int32_t* count, int variant, int tol_mode, double rtol, int max_iter,
+                      hipStream_t st) {
+  const int64_t span = count ? ld : n;
+  if (span <= 0) return PGDG_OK;
+  const int64_t waves = (span + kLearnedChunk - 1) / kLearnedChunk;
+  const unsigned grid = (unsigned)((waves + kThreads / 64 - 1) / (kThreads / 64));
+  if (P)
+    hipLaunchKernelGGL(k_mgfm_pairs<true>, dim3(grid), dim3(kThreads), 0, st, P, nm, tab, in, pair, out,
+                       iters, n, ld, count, variant, tol_mode, rtol, max_iter);
+  else
+    hipLaunchKernelGGL(k_mgfm_pairs<false>, dim3(grid), dim3(kThreads), 0, st, P, nm, tab, in, pair, out,
+                       iters, n, ld, count, variant, tol_mode, rtol, max_iter);
+  PGDG_HIP_TRY(hipGetLastError());
+  return PGDG_OK;
 }
 
 // forward on standardised inputs (y[n][2] standardised)
@@ -616,12 +735,13 @@ __global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ m, float* 
 
 // training set: standardised features and targets from labelled faces
 __global__ void k_mlp_build_set(const double* __restrict__ in, const double* __restrict__ out,
-                                int64_t n, Eos ml, Eos mr, pgdg_mlp_norm nm, float* __restrict__ x,
-                                float* __restrict__ t) {
+                                int64_t n, const int32_t* __restrict__ pair, PairTab tab, pgdg_mlp_norm nm,
+                                float* __restrict__ x, float* __restrict__ t) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int c = pair ? pair[i] : 0;
   float f[kInPad];
-  face_features(in, n, i, ml, mr, nm, f);
+  face_features(in, n, i, tab.l[c], tab.r[c], nm, f);
 #pragma unroll
   for (int k = 0; k < kInPad; k++) x[i * kInPad + k] = f[k];
   const double pl = in[2 * n + i], pr = in[3 * n + i];
@@ -632,9 +752,10 @@ __global__ void k_mlp_build_set(const double* __restrict__ in, const double* __r
 }
 
 // raw feature / target moments: per-block FP64 sums of (v, v^2) for the
-// 10 features and 2 targets
+// 12 features and 2 targets
 __global__ void k_mlp_moments(const double* __restrict__ in, const double* __restrict__ out,
-                              int64_t n, Eos ml, Eos mr, double* __restrict__ part) {
+                              int64_t n, const int32_t* __restrict__ pair, PairTab tab,
+                              double* __restrict__ part) {
   constexpr int NV = kIn + kOut;
   __shared__ double red[kThreads];
   double acc[2 * NV];
@@ -644,10 +765,12 @@ __global__ void k_mlp_moments(const double* __restrict__ in, const double* __res
     const double ul = in[i], ur = in[n + i], pl = in[2 * n + i], pr = in[3 * n + i];
     const double rl = in[4 * n + i], rr = in[5 * n + i];
     const double pm = pl >= pr ? pl : pr;
+    const int c = pair ? pair[i] : 0;
+    const Eos ml = tab.l[c], mr = tab.r[c];
     const double v[NV] = {log(fmax(pl, 1e-30)), log(fmax(pr, 1e-30)), log(fmax(rl, 1e-30)),
                           log(fmax(rr, 1e-30)), ul, ur, ml.gamma, log1p(fmax(ml.pinf, 0.0)),
-                          mr.gamma, log1p(fmax(mr.pinf, 0.0)), log(out[n + i]) - log(pm),
-                          out[2 * n + i]};
+                          mr.gamma, log1p(fmax(mr.pinf, 0.0)), log(fmax(pl, 1e-30) / fmax(pr, 1e-30)),
+                          ul - ur, log(out[n + i]) - log(pm), out[2 * n + i]};
     for (int q = 0; q < NV; q++) {
       acc[q] += v[q];
       acc[NV + q] += v[q] * v[q];
@@ -662,6 +785,25 @@ __global__ void k_mlp_moments(const double* __restrict__ in, const double* __res
       part[(size_t)blockIdx.x * 2 * NV + q] = a;
     }
     __syncthreads();
+  }
+}
+
+// per-block FP64 sums of (y0 - t0)^2: the residual of the standardised
+// ln(p* / max(pL, pR)) output over a training set
+__global__ void k_mlp_resid(const float* __restrict__ y, const float* __restrict__ t, int64_t n,
+                            double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  double acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = (double)y[i * kOut] - (double)t[i * kOut];
+    acc += d * d;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0;
+    for (int q = 0; q < kThreads; q++) a += red[q];
+    part[blockIdx.x] = a;
   }
 }
 
@@ -739,6 +881,32 @@ float4 eos_features(const pgdg_mlp_norm& nm, const pgdg_eos_pair* eos) {
 
 }  // namespace
 
+namespace pgdg {
+
+PairTab make_pair_tab(const pgdg_mlp_norm& nm, const pgdg_eos_pair* eos, int np) {
+  PairTab t{};
+  for (int c = 0; c < np; c++) {
+    t.l[c] = pgdg::Eos{eos[c].gamma_l, eos[c].pinf_l};
+    t.r[c] = pgdg::Eos{eos[c].gamma_r, eos[c].pinf_r};
+    const float4 fx = eos_features(nm, &eos[c]);
+    t.fx[c][0] = fx.x;
+    t.fx[c][1] = fx.y;
+    t.fx[c][2] = fx.z;
+    t.fx[c][3] = fx.w;
+  }
+  return t;
+}
+
+int mlp_view(pgdg_mlp* m, const float** params, pgdg_mlp_norm* nm) {
+  if (!m) return PGDG_ERR_ARG;
+  if (!m->ready) return PGDG_ERR_STATE;
+  *params = m->params;
+  *nm = m->norm;
+  return PGDG_OK;
+}
+
+}  // namespace pgdg
+
 extern "C" {
 
 int pgdg_mlp_create(pgdg_mlp** out) {
@@ -803,6 +971,7 @@ int pgdg_mlp_set_params(pgdg_mlp* m, const float* params, const pgdg_mlp_norm* n
       if (!(norm->feat_sd[k] > 0)) return PGDG_ERR_ARG;
     for (int k = 0; k < kOut; k++)
       if (!(norm->out_sd[k] > 0)) return PGDG_ERR_ARG;
+    if (!(norm->keep >= 0)) return PGDG_ERR_ARG;
     m->norm = *norm;
   }
   m->ready = true;
@@ -830,13 +999,20 @@ int pgdg_mlp_loss_grad(pgdg_mlp* m, const float* x, const float* t, int64_t n, f
 int pgdg_mlp_train(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair* eos,
                    const pgdg_mlp_train_opts* o, float* loss_hist, int hist_cap, int* steps,
                    void* stream) {
+  return pgdg_mlp_train_pairs(m, in, nullptr, n, eos, 1, o, loss_hist, hist_cap, steps, stream);
+}
+
+int pgdg_mlp_train_pairs(pgdg_mlp* m, const double* in, const int32_t* pair, int64_t n,
+                         const pgdg_eos_pair* eos, int npairs, const pgdg_mlp_train_opts* o,
+                         float* loss_hist, int hist_cap, int* steps, void* stream) {
   using namespace pgdg;
   if (!m || !in || !eos || !o || n < kTile) return PGDG_ERR_ARG;
+  if (npairs < 1 || npairs > kMaxPairs || (npairs > 1 && !pair)) return PGDG_ERR_ARG;
   if (o->epochs < 1 || o->batch < kTile || o->batch % kTile || !(o->lr > 0)) return PGDG_ERR_ARG;
   if (o->variant != MGFM_S3 && o->variant != MGFM_T2) return PGDG_ERR_ARG;
   if (hist_cap < 0 || (hist_cap > 0 && !loss_hist)) return PGDG_ERR_ARG;
   hipStream_t st = as_stream(stream);
-  const Eos ml{eos->gamma_l, eos->pinf_l}, mr{eos->gamma_r, eos->pinf_r};
+  PairTab tab = make_pair_tab(m->norm, eos, npairs);  // (the features are standardised below)
   const int64_t nt = n / o->batch * o->batch;  // whole batches
   if (nt == 0) return PGDG_ERR_ARG;
   // 1. label with the exact solve
@@ -863,16 +1039,14 @@ int pgdg_mlp_train(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
   MLP_TRY(hipMalloc((void**)&x, sizeof(float) * kInPad * (size_t)n));
   MLP_TRY(hipMalloc((void**)&t, sizeof(float) * kOut * (size_t)n));
   MLP_TRY(hipMalloc((void**)&mom, sizeof(double) * 2 * NV * mblocks));
-  {
-    pgdg_mgfm_opts lo{o->variant, PGDG_TOL_RELATIVE, 1e-12, 100};
-    rc = pgdg_mgfm_batch(in, lab, nullptr, n, eos, &lo, nullptr, stream);
-    if (rc != PGDG_OK) {
-      cleanup();
-      return rc;
-    }
+  rc = launch_mgfm_pairs(nullptr, m->norm, tab, in, pair, lab, nullptr, n, n, nullptr, o->variant, TOL_REL,
+                         1e-12, 100, st);
+  if (rc != PGDG_OK) {
+    cleanup();
+    return rc;
   }
   // 2. normalisation from the labelled set (block sums added in block order)
-  hipLaunchKernelGGL(k_mlp_moments, dim3(mblocks), dim3(kThreads), 0, st, in, lab, n, ml, mr, mom);
+  hipLaunchKernelGGL(k_mlp_moments, dim3(mblocks), dim3(kThreads), 0, st, in, lab, n, pair, tab, mom);
   std::vector<double> hm((size_t)2 * NV * mblocks);
   MLP_TRY(hipMemcpyAsync(hm.data(), mom, sizeof(double) * hm.size(), hipMemcpyDeviceToHost, st));
   MLP_TRY(hipStreamSynchronize(st));
@@ -897,8 +1071,9 @@ int pgdg_mlp_train(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
     }
   }
   m->norm = nm;
+  tab = make_pair_tab(nm, eos, npairs);
   hipLaunchKernelGGL(k_mlp_build_set, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, lab,
-                     n, ml, mr, nm, x, t);
+                     n, pair, tab, nm, x, t);
   // 3. parameters
   std::vector<float> P;
   init_params(P, o->seed);
@@ -929,6 +1104,26 @@ int pgdg_mlp_train(pgdg_mlp* m, const double* in, int64_t n, const pgdg_eos_pair
   MLP_TRY(hipGetLastError());
   if (nrec > 0)
     MLP_TRY(hipMemcpyAsync(loss_hist, hist, sizeof(float) * nrec, hipMemcpyDeviceToHost, st));
+  // 5. the trained network's residual on its training faces sets where its
+  //    p* guess is used (beyond twice the residual RMS of ln p* / max(pL, pR))
+  {
+    float* y = nullptr;
+    MLP_TRY(hipMalloc((void**)&y, sizeof(float) * kOut * (size_t)nt));
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((nt + kTile - 1) / kTile)), dim3(kThreads), 0, st,
+                       m->params, x, nt, y);
+    hipLaunchKernelGGL(k_mlp_resid, dim3(mblocks), dim3(kThreads), 0, st, y, t, nt, mom);
+    std::vector<double> hr(mblocks);
+    const hipError_t e1 = hipMemcpyAsync(hr.data(), mom, sizeof(double) * mblocks, hipMemcpyDeviceToHost, st);
+    const hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipFree(y);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      cleanup();
+      return PGDG_ERR_HIP;
+    }
+    double ss = 0;
+    for (int b = 0; b < mblocks; b++) ss += hr[b];
+    m->norm.keep = (float)(2.0 * std::sqrt(ss / (double)nt) * nm.out_sd[0]);
+  }
   MLP_TRY(hipStreamSynchronize(st));
   if (steps) *steps = (int)total;
   cleanup();
@@ -972,6 +1167,23 @@ int pgdg_mgfm_batch_learned(pgdg_mlp* m, const double* in, double* out, int32_t*
                      opts->variant, opts->tol_mode, opts->rtol, opts->max_iter);
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
+}
+
+int pgdg_mgfm_batch_pairs(pgdg_mlp* m, const double* in, const int32_t* pair, double* out,
+                          int32_t* iters, int64_t n, const pgdg_eos_pair* eos, int npairs,
+                          const pgdg_mgfm_opts* opts, void* stream) {
+  using namespace pgdg;
+  if (!eos || !opts || n < 0 || npairs < 1 || npairs > kMaxPairs) return PGDG_ERR_ARG;
+  if (m && !m->ready) return PGDG_ERR_STATE;
+  if (n == 0) return PGDG_OK;
+  if (!in || !out || (npairs > 1 && !pair)) return PGDG_ERR_ARG;
+  if (opts->variant != MGFM_S3 && opts->variant != MGFM_T2) return PGDG_ERR_ARG;
+  if (opts->tol_mode != TOL_REF && opts->tol_mode != TOL_REL) return PGDG_ERR_ARG;
+  if (opts->max_iter < 1) return PGDG_ERR_ARG;
+  const pgdg_mlp_norm nm = m ? m->norm : pgdg_mlp_norm{};
+  return launch_mgfm_pairs(m ? m->params : nullptr, nm, make_pair_tab(nm, eos, npairs), in, pair, out, iters, n, n,
+                           nullptr, opts->variant, opts->tol_mode, opts->rtol, opts->max_iter,
+                           as_stream(stream));
 }
 
 }  // extern "C"

@@ -1612,7 +1612,7 @@ __global__ void __launch_bounds__(256) k_update_grid(Geo g, CellDev C, const int
 // ---------------------------------------------------------------------------
 template <int D, int ORD>
 static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32_t* used,
-                          const DevScalars* sc, int arg, hipStream_t st, int block) {
+                          const DevScalars* sc, int arg, hipStream_t st, int block, const IfaceBatch* ib) {
   const int nc = g.nc;
   switch (which) {
     case CELL_LIMITER:
@@ -1634,7 +1634,7 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
       break;
     case CELL_RHS:
       if (g.prec == PGDG_PREC_MIXED) {
-        PGDG_TRY(launch_faces<D>(g, C, used, st));
+        PGDG_TRY(launch_faces<D>(g, C, used, st, ib));
         if (D == 3 && ORD == 2 && g.mfma_rhs) {
           hipLaunchKernelGGL(k_rhs_mfma, dim3(grid_xcd(((long long)nc + 63) / 64, 4)), dim3(256), 0, st, g, C,
                              used, sc, arg);
@@ -1675,12 +1675,12 @@ static int launch_cells_o(int which, const Geo& g, const CellDev& C, const int32
 template <int D>
 int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
-                 hipStream_t st, int block) {
+                 hipStream_t st, int block, const IfaceBatch* ib) {
   (void)P;
   (void)bin_off;
   (void)bin_ent;
-  if (g.ord == 2) return launch_cells_o<D, 2>(which, g, C, used, sc, arg, st, block);
-  return launch_cells_o<D, 1>(which, g, C, used, sc, arg, st, block);
+  if (g.ord == 2) return launch_cells_o<D, 2>(which, g, C, used, sc, arg, st, block, ib);
+  return launch_cells_o<D, 1>(which, g, C, used, sc, arg, st, block, ib);
 }
 
 template <int D>
@@ -1715,8 +1715,8 @@ template int launch_p2g<3>(const Geo&, const CellDev&, const PtlDev&, const int*
                            const int32_t*, const P2gBuf&, hipStream_t);
 
 template int launch_cells<2>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
-                             const int32_t*, const DevScalars*, int, hipStream_t, int);
+                             const int32_t*, const DevScalars*, int, hipStream_t, int, const IfaceBatch*);
 template int launch_cells<3>(int, const Geo&, const CellDev&, const PtlDev&, const int*, const int*,
-                             const int32_t*, const DevScalars*, int, hipStream_t, int);
+                             const int32_t*, const DevScalars*, int, hipStream_t, int, const IfaceBatch*);
 
 }  // namespace pgdg

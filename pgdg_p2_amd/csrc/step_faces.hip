@@ -21,8 +21,11 @@
 
 namespace pgdg {
 
-__global__ void k_face_clear(int32_t* count) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0;
+__global__ void k_face_clear(int32_t* count, int32_t* mg_count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *count = 0;
+    if (mg_count) *mg_count = 0;
+  }
 }
 
 // Phase masks of a cell: present (ph_state 1) and nonzero volume fraction.
@@ -124,12 +127,19 @@ k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
   }
 }
 
+// How k_face_general meets the MGFM solves of its interface points:
+// FG_INLINE solves each in place; FG_COLLECT reserves the face's records
+// and writes the solves' inputs there (no fluxes); FG_APPLY takes the
+// solutions from the records (after the batch kernel has run).
+enum { FG_INLINE = 0, FG_COLLECT = 1, FG_APPLY = 2 };
+
 // The listed faces, each side as its cell sees it: for every present phase
 // ss of the side's cell, the sum over the other cell's phases s1 (alpha != 0)
 // of alpha_s1 * flux(ss, s1), with the MGFM ghost state where s1 != ss.  The
 // two cells' coefficient sets are loaded once per phase into registers; the
 // sum over s1 accumulates in the output slots (this lane's own, in s1 order).
-template <int D, int ORD>
+// A face's MGFM records follow its traversal order (side, ss, s1, point).
+template <int D, int ORD, int MODE>
 __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const int32_t* __restrict__ used) {
   constexpr int U = D + 2, M = Basis<D, ORD>::M, B = M - 1, NQ = Basis<D, ORD>::NQ;
   constexpr int NQF = D == 3 ? NQ * NQ : NQ;
@@ -157,10 +167,42 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
       Uo[u] = cf[u][0] + lm[u] * acc;
     }
   };
+  const int64_t ld = C.mg_cap;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     const int id = C.flist[t];
     const int ax = id / nc, c = id % nc;
     const int nb = c + stride[ax];
+    int64_t rec = -1;  // this face's next MGFM record (-1: solved in place)
+    if constexpr (MODE == FG_COLLECT) {
+      int nrec = 0;
+      for (int side = 0; side < 2; side++) {
+        const int ci = side == 0 ? c : nb, cj = side == 0 ? nb : c;
+        if (!used[ci]) continue;
+        for (int ss = 0; ss < PGDG_MAX_PHASE && ss < g.F; ss++) {
+          if (C.ph_state[ss * nc + ci] != 1) continue;
+          for (int s1 = 0; s1 < PGDG_MAX_PHASE && s1 < g.F; s1++)
+            if (s1 != ss && C.alph[s1 * nc + cj] != 0) nrec += NQF;
+        }
+      }
+      if (nrec > 0) {
+        const int64_t base = atomicAdd(C.mg_count, nrec);
+        atomicMax(C.mg_peak, (int32_t)(base + nrec));
+        if (base + nrec <= ld) {
+          rec = base;
+        } else {
+          // past the record capacity: this face solves its points in place;
+          // its reserved records below the capacity hold a trivial solve
+          for (int64_t r = base; r < ld; r++) {
+            for (int q = 0; q < 6; q++) C.mg_in[q * ld + r] = q < 2 ? 0.0 : 1.0;
+            C.mg_pair[r] = 0;
+          }
+        }
+      }
+      C.mg_base[t] = (int32_t)rec;
+      if (rec < 0) continue;
+    } else if constexpr (MODE == FG_APPLY) {
+      rec = C.mg_base[t];
+    }
     for (int side = 0; side < 2; side++) {
       const int ci = side == 0 ? c : nb, cj = side == 0 ? nb : c;
       if (!used[ci]) continue;
@@ -189,7 +231,38 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
             basis_tail<D, ORD>(dj, g.q12, phj);
             eval(cfi, lmi, phi, Ui);
             eval(cfj, lmj, phj, Uj);
-            iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh);
+            if constexpr (MODE == FG_COLLECT) {
+              if (s1 != ss) {
+                const int64_t r = rec++;
+                const int code = ss * PGDG_MAX_PHASE + s1;
+                iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh,
+                                     [&](double ul, double ur, double pl, double pr, double rl, double rr, Eos,
+                                         Eos, int) {
+                                       C.mg_in[r] = ul;
+                                       C.mg_in[ld + r] = ur;
+                                       C.mg_in[2 * ld + r] = pl;
+                                       C.mg_in[3 * ld + r] = pr;
+                                       C.mg_in[4 * ld + r] = rl;
+                                       C.mg_in[5 * ld + r] = rr;
+                                       C.mg_pair[r] = code;
+                                       return MgfmOut{rl, pl, ul, 0};
+                                     });
+              }
+              continue;  // no fluxes in this pass
+            } else if constexpr (MODE == FG_APPLY) {
+              if (rec >= 0 && s1 != ss) {
+                const int64_t r = rec++;
+                iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh,
+                                     [&](double, double, double, double, double, double, Eos, Eos, int) {
+                                       return MgfmOut{C.mg_out[r], C.mg_out[ld + r], C.mg_out[2 * ld + r],
+                                                      C.mg_it[r]};
+                                     });
+              } else {
+                iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh);
+              }
+            } else {
+              iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh);
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
               float* o = &C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, c)];
@@ -198,7 +271,7 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
           }
           first = false;
         }
-        if (first)  // no phase of the other cell has a volume fraction
+        if (MODE != FG_COLLECT && first)  // no phase of the other cell has a volume fraction
 #pragma unroll 1
           for (int q = 0; q < NQF; q++)
 #pragma unroll
@@ -281,13 +354,33 @@ __global__ void __launch_bounds__(256) k_face_boundary3(Geo g, CellDev C, const 
   }
 }
 
+// The listed faces with their interface points' MGFM solved in place, or
+// collected, solved as one batch (the reference's initial p* or the
+// network's) and applied.
+template <int D, int ORD>
+static int launch_general(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st,
+                          const IfaceBatch* ib) {
+  if (!ib || ib->mode == IFACE_INLINE) {
+    hipLaunchKernelGGL((k_face_general<D, ORD, FG_INLINE>), dim3(1024), dim3(128), 0, st, g, C, used);
+    return PGDG_OK;
+  }
+  if (!C.mg_in || !C.mg_count || C.mg_cap <= 0) return PGDG_ERR_STATE;
+  if (ib->mode == IFACE_LEARNED && !ib->P) return PGDG_ERR_STATE;
+  hipLaunchKernelGGL((k_face_general<D, ORD, FG_COLLECT>), dim3(1024), dim3(128), 0, st, g, C, used);
+  PGDG_TRY(launch_mgfm_pairs(ib->mode == IFACE_LEARNED ? ib->P : nullptr, ib->nm, ib->tab, C.mg_in, C.mg_pair,
+                             C.mg_out, C.mg_it, 0, C.mg_cap, C.mg_count, g.sem == PGDG_SEM_S3 ? MGFM_S3 : MGFM_T2,
+                             TOL_REF, 0.0, 50, st));
+  hipLaunchKernelGGL((k_face_general<D, ORD, FG_APPLY>), dim3(1024), dim3(128), 0, st, g, C, used);
+  return PGDG_OK;
+}
+
 template <int D>
-int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st) {
+int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st, const IfaceBatch* ib) {
   if (!C.fflux || !C.flist || !C.fcount) return PGDG_ERR_STATE;
-  hipLaunchKernelGGL(k_face_clear, dim3(1), dim3(64), 0, st, C.fcount);
+  hipLaunchKernelGGL(k_face_clear, dim3(1), dim3(64), 0, st, C.fcount, C.mg_count);
   if (g.ord == 2) {
     hipLaunchKernelGGL((k_face_simple<D, 2>), dim3(grid_xcd(g.nc, 256)), dim3(256), 0, st, g, C, used);
-    hipLaunchKernelGGL((k_face_general<D, 2>), dim3(1024), dim3(128), 0, st, g, C, used);
+    PGDG_TRY((launch_general<D, 2>(g, C, used, st, ib)));
     if (D == 3 && g.mfma_rhs) {
       if (!C.bflux) return PGDG_ERR_STATE;
       const long long n = 6LL * std::max(g.N[1] * g.N[2], std::max(g.N[0] * g.N[2], g.N[0] * g.N[1]));
@@ -295,12 +388,12 @@ int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_
     }
   } else {
     hipLaunchKernelGGL((k_face_simple<D, 1>), dim3(grid_xcd(g.nc, 256)), dim3(256), 0, st, g, C, used);
-    hipLaunchKernelGGL((k_face_general<D, 1>), dim3(1024), dim3(128), 0, st, g, C, used);
+    PGDG_TRY((launch_general<D, 1>(g, C, used, st, ib)));
   }
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }
-template int launch_faces<2>(const Geo&, const CellDev&, const int32_t*, hipStream_t);
-template int launch_faces<3>(const Geo&, const CellDev&, const int32_t*, hipStream_t);
+template int launch_faces<2>(const Geo&, const CellDev&, const int32_t*, hipStream_t, const IfaceBatch*);
+template int launch_faces<3>(const Geo&, const CellDev&, const int32_t*, hipStream_t, const IfaceBatch*);
 
 }  // namespace pgdg

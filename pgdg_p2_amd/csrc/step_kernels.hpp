@@ -2,6 +2,7 @@
 // units (cells, particles, topology) and the context.
 #pragma once
 #include "dev_common.hpp"
+#include "mgfm_pairs.hpp"
 
 namespace pgdg {
 
@@ -18,14 +19,17 @@ struct Tuning {
   int limiter = 128;
 };
 
-// Mixed form: interior-face point fluxes into C.fflux (before k_rhs<float>).
+// Mixed form: interior-face point fluxes into C.fflux (before k_rhs<float>);
+// ib (may be nullptr: IFACE_INLINE) selects how the interface points' MGFM
+// is solved.
 template <int D>
-int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st);
+int launch_faces(const Geo& g, const CellDev& C, const int32_t* used, hipStream_t st,
+                 const IfaceBatch* ib = nullptr);
 
 template <int D>
 int launch_cells(int which, const Geo& g, const CellDev& C, const PtlDev& P, const int* bin_off,
                  const int* bin_ent, const int32_t* used, const DevScalars* sc, int arg,
-                 hipStream_t st, int block = 256);
+                 hipStream_t st, int block = 256, const IfaceBatch* ib = nullptr);
 
 // Particle data MLS_P2G's kernel weights need (position, volume, phase;
 // phase -1 for walls), gathered into bin-entry order.  The state and slopes

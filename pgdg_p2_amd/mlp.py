@@ -1,6 +1,6 @@
 """Learned initial guess for the MGFM Newton solve (C ABI pgdg_mlp_*).
 
-A 10 -> 32 -> 32 -> 32 -> 2 ReLU MLP trained on the GPU (BF16 MFMA forward
+A 12 -> 32 -> 32 -> 32 -> 2 ReLU MLP trained on the GPU (BF16 MFMA forward
 and backward kernels, FP32 Adam) on faces labelled by the exact MGFM solve;
 its p* prediction seeds the Newton iteration, which still runs to the
 requested tolerance.  The reference has no counterpart: its Newton solve
@@ -14,12 +14,12 @@ import numpy as np
 
 from . import DeviceArray, EosPair, MgfmOpts, check, device_sync, lib, MGFM_S3, TOL_RELATIVE
 
-NIN, NPAD, NOUT, NPARAM = 10, 16, 2, 3184
+NIN, NPAD, NOUT, NPARAM = 12, 16, 2, 3184
 
 
 class MlpNorm(C.Structure):
     _fields_ = [("feat_mu", C.c_float * NIN), ("feat_sd", C.c_float * NIN),
-                ("out_mu", C.c_float * NOUT), ("out_sd", C.c_float * NOUT)]
+                ("out_mu", C.c_float * NOUT), ("out_sd", C.c_float * NOUT), ("keep", C.c_float)]
 
 
 class MlpTrainOpts(C.Structure):
@@ -33,6 +33,10 @@ def _declare(L):
     L.pgdg_mlp_destroy.argtypes = [vp]
     L.pgdg_mlp_train.argtypes = [vp, vp, i64, C.POINTER(EosPair), C.POINTER(MlpTrainOpts), vp,
                                  C.c_int, C.POINTER(C.c_int), vp]
+    L.pgdg_mlp_train_pairs.argtypes = [vp, vp, vp, i64, C.POINTER(EosPair), C.c_int,
+                                       C.POINTER(MlpTrainOpts), vp, C.c_int, C.POINTER(C.c_int), vp]
+    L.pgdg_mgfm_batch_pairs.argtypes = [vp, vp, vp, vp, vp, i64, C.POINTER(EosPair), C.c_int,
+                                        C.POINTER(MgfmOpts), vp]
     L.pgdg_mlp_guess.argtypes = [vp, vp, i64, C.POINTER(EosPair), vp, vp]
     L.pgdg_mgfm_batch_learned.argtypes = [vp, vp, vp, vp, i64, C.POINTER(EosPair),
                                           C.POINTER(MgfmOpts), vp]
@@ -70,6 +74,19 @@ class Mlp:
         check(self.L.pgdg_mlp_train(self.h, d_in.ptr, n, C.byref(EosPair(*eos)), C.byref(opts),
                                     C.c_void_p(hist.ctypes.data), hist_cap, C.byref(steps), None),
               "pgdg_mlp_train")
+        return hist[:min(steps.value, hist_cap)].copy()
+
+    def train_pairs(self, d_in: DeviceArray, d_pair: DeviceArray | None, n: int, eos_list, epochs=8,
+                    batch=4096, lr=2e-3, seed=0x5EED0004, variant=MGFM_S3, hist_cap=1 << 16):
+        """As train, over faces whose EOS pair d_pair[i] indexes eos_list."""
+        hist = np.zeros(hist_cap, dtype=np.float32)
+        steps = C.c_int(0)
+        opts = MlpTrainOpts(epochs, batch, lr, seed, variant)
+        tab = (EosPair * len(eos_list))(*[EosPair(*e) for e in eos_list])
+        check(self.L.pgdg_mlp_train_pairs(self.h, d_in.ptr, d_pair.ptr if d_pair is not None else None, n, tab,
+                                          len(eos_list), C.byref(opts), C.c_void_p(hist.ctypes.data), hist_cap,
+                                          C.byref(steps), None),
+              "pgdg_mlp_train_pairs")
         return hist[:min(steps.value, hist_cap)].copy()
 
     def guess_dev(self, d_in, n, eos, d_p0):
@@ -118,3 +135,18 @@ class Mlp:
               "pgdg_mlp_loss_grad")
         device_sync()
         return float(dl.numpy()[0]), dg.numpy()
+
+
+def mgfm_pairs_dev(mlp: Mlp | None, d_in, d_pair, d_out, d_it, n, eos_list, variant=MGFM_S3,
+                   tol_mode=TOL_RELATIVE, rtol=1e-12, max_iter=100):
+    """pgdg_mgfm_batch_pairs: faces with per-face EOS pairs, from the
+    network's p*0 when mlp is given (asynchronous)."""
+    L = lib()
+    _declare(L)
+    opts = MgfmOpts(variant, tol_mode, rtol, max_iter)
+    tab = (EosPair * len(eos_list))(*[EosPair(*e) for e in eos_list])
+    check(L.pgdg_mgfm_batch_pairs(mlp.h if mlp is not None else None, d_in.ptr,
+                                  d_pair.ptr if d_pair is not None else None, d_out.ptr,
+                                  d_it.ptr if d_it is not None else None, n, tab, len(eos_list),
+                                  C.byref(opts), None),
+          "pgdg_mgfm_batch_pairs")

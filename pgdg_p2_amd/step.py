@@ -79,6 +79,40 @@ class GpuSim:
         matrix cores or one lane per cell (default; pgdg_rhs_mfma_enable)."""
         check(self.L.pgdg_rhs_mfma_enable(self.h, 1 if on else 0), "pgdg_rhs_mfma_enable")
 
+    IFACE_INLINE, IFACE_BATCH, IFACE_LEARNED = 0, 1, 2
+
+    def iface_mode(self, mode: int, mlp=None):
+        """How the mixed step solves MGFM at interface face points
+        (pgdg_iface_mgfm_mode): in the face kernel, as one batch, or as one
+        batch from the trained network mlp's initial p*."""
+        check(self.L.pgdg_iface_mgfm_mode(self.h, mode, mlp.h if mlp is not None else None),
+              "pgdg_iface_mgfm_mode")
+
+    def iface_pairs(self):
+        """EOS pair (gamma_l, pinf_l, gamma_r, pinf_r) of each record pair code."""
+        from . import EosPair
+        from .state import PGDG_MAX_PHASE
+        tab = (EosPair * (PGDG_MAX_PHASE * PGDG_MAX_PHASE))()
+        check(self.L.pgdg_iface_pairs(self.h, tab), "pgdg_iface_pairs")
+        return [(e.gamma_l, e.pinf_l, e.gamma_r, e.pinf_r) for e in tab]
+
+    def iface_records(self, cap: int | None = None):
+        """The last batched stage's interface-point records as host arrays:
+        (inputs [6][n], solutions [3][n], pair codes [n], iterations [n])."""
+        import numpy as np
+        n = C.c_int64(0)
+        check(self.L.pgdg_iface_records(self.h, None, None, None, None, 0, C.byref(n)), "pgdg_iface_records")
+        k = n.value if cap is None else min(cap, n.value)
+        a_in = np.zeros((6, k), np.float64)
+        a_out = np.zeros((3, k), np.float64)
+        a_pair = np.zeros(k, np.int32)
+        a_it = np.zeros(k, np.int32)
+        if k:
+            check(self.L.pgdg_iface_records(self.h, C.c_void_p(a_in.ctypes.data), C.c_void_p(a_out.ctypes.data),
+                                            C.c_void_p(a_pair.ctypes.data), C.c_void_p(a_it.ctypes.data), k,
+                                            C.byref(n)), "pgdg_iface_records")
+        return a_in, a_out, a_pair, a_it
+
     def autotune(self, on: bool):
         check(self.L.pgdg_autotune_enable(self.h, 1 if on else 0), "pgdg_autotune_enable")
 

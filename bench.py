@@ -453,6 +453,134 @@ def mls_bench(n=128, reps=5):
     return res
 
 
+def train_iface_guess(gpu):
+    """Train the network on the interface-point records of the stage just
+    solved (the step's own exact MGFM solves, on the GPU) and switch the
+    step to the learned initial guess."""
+    import numpy as np
+    import pgdg_p2_amd as P
+    from pgdg_p2_amd.mlp import Mlp
+    from pgdg_p2_amd.step import GpuSim
+    rin, _, rpair, _ = gpu.iface_records()
+    extra = 0
+    if rin.shape[1] < 4096:  # the record buffer has just grown: one more step's records
+        gpu.step(1)
+        extra = 1
+        rin, _, rpair, _ = gpu.iface_records()
+    n = rin.shape[1] // 4096 * 4096
+    res = {"train_records": n, "steps_taken": extra}
+    if n < 4096:
+        res["note"] = "too few interface records to train; the batch keeps the reference's initial p*"
+        return res
+    mlp = Mlp()
+    d_in = P.DeviceArray.from_numpy(np.ascontiguousarray(rin[:, :n]))
+    d_pair = P.DeviceArray.from_numpy(np.ascontiguousarray(rpair[:n]))
+    P.device_sync()
+    t0 = time.perf_counter()
+    hist = mlp.train_pairs(d_in, d_pair, n, gpu.iface_pairs(), epochs=8, batch=4096, lr=3e-3)
+    res["train_s"] = time.perf_counter() - t0
+    res["train_loss_first_last"] = [float(hist[:8].mean()), float(hist[-8:].mean())]
+    _, nm = mlp.get_params()
+    res["keep_below_abs_ln_correction"] = float(nm.keep)
+    d_in.free()
+    d_pair.free()
+    gpu.iface_mode(GpuSim.IFACE_LEARNED, mlp)
+    res["mlp"] = mlp
+    return res
+
+
+def iface_iterations(gpu, mlp):
+    """Newton iterations per interface point on the last stage's records:
+    the step's own solves against the same inputs solved from the
+    reference's initial p* (and, to 1e-12 relative, from both starts)."""
+    import numpy as np
+    import pgdg_p2_amd as P
+    from pgdg_p2_amd.mlp import mgfm_pairs_dev
+    rin, _, rpair, rit = gpu.iface_records()
+    n = rin.shape[1]
+    res = {"points_last_stage": n}
+    if n == 0:
+        return res
+    d_x = P.DeviceArray.from_numpy(np.ascontiguousarray(rin))
+    d_c = P.DeviceArray.from_numpy(np.ascontiguousarray(rpair))
+    d_o = P.DeviceArray((3, n), np.float64)
+    d_it = P.DeviceArray((n,), np.int32)
+    eos = gpu.iface_pairs()
+    its = {"step": float(rit.mean())}
+    for who, m in (("reference_guess", None), ("learned_guess", mlp)):
+        if who == "learned_guess" and m is None:
+            continue
+        for tol, (mode, rtol, mx) in (("reference_tol", (P.TOL_REFERENCE, 0.0, 50)),
+                                      ("rel_1e-12", (P.TOL_RELATIVE, 1e-12, 100))):
+            mgfm_pairs_dev(m, d_x, d_c, d_o, d_it, n, eos, tol_mode=mode, rtol=rtol, max_iter=mx)
+            P.device_sync()
+            its[f"{who}_{tol}"] = float(d_it.numpy().astype(np.float64).mean())
+    for d in (d_x, d_c, d_o, d_it):
+        d.free()
+    res["newton_its_per_point"] = its
+    return res
+
+
+def learned_step_bench(size=(128, 64, 64), x_shock=0.3, steps=10, scheme=None):
+    """The learned-guess step where its interface points meet strong jumps:
+    the shock-bubble channel with the shock starting inside the helium
+    sphere's span, P2 / SSP-RK3 / bounds [4,16].  Three runs from the same
+    state: FP64, mixed with MGFM solved in the face kernel, and mixed with
+    the stage's interface points solved as one batch from the network's
+    initial p* (trained on the GPU on the first warm-up step's own solves).
+    Reports ms/step, the cell density deviations and the Newton iterations
+    per interface point."""
+    import numpy as np
+    from pgdg_p2_amd import cases
+    from pgdg_p2_amd.step import GpuSim
+    nx, ny, nz = size
+    res = {"workload": f"3-D shock-bubble, {nx}x{ny}x{nz} P2 cells, 8 particles/cell, SSP-RK3, bounds [4,16], "
+                       f"shock starting at x = {x_shock} (inside the bubble's span)", "steps": steps}
+    rho = {}
+    for name in ("fp64", "mixed_inline", "mixed_learned"):
+        case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, x_shock=x_shock, cap_factor=1.0, scheme=scheme)
+        case.precision = 0 if name == "fp64" else 1
+        gpu = GpuSim(case, st, cap=int(st.n * 2.2) + 4096)
+        mlp, info = None, {}
+        if name == "mixed_learned":
+            # the first batched step sizes the record buffer (its records are
+            # dropped when it grows); the second's are the training set
+            gpu.iface_mode(GpuSim.IFACE_BATCH)
+            gpu.step(1)
+            info = train_iface_guess(gpu)
+            mlp = info.pop("mlp", None)
+            gpu.step(1 - info.pop("steps_taken"))  # two warm-up steps in every run
+        else:
+            gpu.step(2)
+        gpu.sync()
+        t0 = time.perf_counter()
+        gpu.step(steps)
+        gpu.sync()
+        el = time.perf_counter() - t0
+        r = {"ms_per_step": el / steps * 1e3, "cell_steps_per_s": nx * ny * nz * steps / el,
+             "particles_end": gpu.count()}
+        if name == "mixed_learned":
+            info.update(iface_iterations(gpu, mlp))
+            r.update(info)
+        gpu.save_cells(st)
+        rho[name] = (st.c["alph"] * st.c["rho"]).sum(axis=0)
+        gpu.close()
+        if mlp is not None:
+            mlp.close()
+        res[name] = r
+        print(f"[learned-step] {name}: {r['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    for a in ("mixed_inline", "mixed_learned"):
+        d = np.abs(rho[a] - rho["fp64"])
+        res[a]["density_vs_fp64"] = {"L1_rel": float(d.sum() / np.abs(rho["fp64"]).sum()),
+                                     "Linf_rel": float(d.max() / np.abs(rho["fp64"]).max())}
+        res[a]["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res[a]["ms_per_step"]
+    d = np.abs(rho["mixed_learned"] - rho["mixed_inline"])
+    res["mixed_learned"]["density_vs_mixed_inline"] = {
+        "L1_rel": float(d.sum() / np.abs(rho["mixed_inline"]).sum()),
+        "Linf_rel": float(d.max() / np.abs(rho["mixed_inline"]).max())}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -472,10 +600,17 @@ def main():
     ap.add_argument("--mls-n", type=int, default=128)
     ap.add_argument("--only-mls", action="store_true")
     ap.add_argument("--only-mgfm", action="store_true")
+    ap.add_argument("--no-learned-step", action="store_true",
+                    help="skip the learned-guess step leg (128x64x64, shock inside the bubble's span)")
+    ap.add_argument("--only-learned-step", action="store_true")
     ap.add_argument("--mlp-save", default=None, help="MGFM leg: write the trained network (.npz)")
     ap.add_argument("--mlp-load", default=None, help="MGFM leg: use a saved network instead of training")
     ap.add_argument("--rhs-mfma", action="store_true",
                     help="mixed form: the right-hand side on the matrix cores (k_rhs_mfma)")
+    ap.add_argument("--iface", default="inline", choices=["inline", "batch", "learned"],
+                    help="mixed form: MGFM at the interface face points solved in the face kernel, "
+                         "as one batch per stage, or as one batch from the network's initial p* "
+                         "(trained on the first warm-up step's own solves)")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel over this many MiB (byte-counter "
@@ -511,6 +646,9 @@ def main():
         print(json.dumps({"error_horizon": [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
                                             for sz in ((64, 32, 32), (128, 64, 64))]}))
         return
+    if args.only_learned_step:
+        print(json.dumps({"learned_guess_step": learned_step_bench(scheme=SCHEMES[args.scheme])}))
+        return
     if args.only_3phase:
         print(json.dumps({"three_phase_2d": three_phase_bench(warmup=args.tp3_warmup)}))
         return
@@ -535,7 +673,18 @@ def main():
         name = "mixed" if precision else "fp64"
         print(f"[{name}] setup {time.perf_counter() - t_setup:.1f}s: {nx}x{ny}x{nz} cells, "
               f"{n0} particles", file=sys.stderr, flush=True)
-        gpu.step(args.warmup)
+        iface = None
+        if precision and args.iface != "inline":
+            gpu.iface_mode(GpuSim.IFACE_BATCH)
+            gpu.step(1)
+            iface = {"mode": args.iface}
+            taken = 1
+            if args.iface == "learned":
+                iface.update(train_iface_guess(gpu))
+                taken += iface.pop("steps_taken")
+            gpu.step(max(args.warmup - taken, 0))
+        else:
+            gpu.step(args.warmup)
         gpu.sync()
         n_start = gpu.count()
         t0 = time.perf_counter()
@@ -551,14 +700,20 @@ def main():
         gpu.sync()
         prof = gpu.profile_read()
         gpu.profile(False)
+        mlp = None
+        if iface is not None:
+            mlp = iface.pop("mlp", None)  # the context reads its parameters until closed
+            iface.update(iface_iterations(gpu, mlp))
         stages = {k: {"ms_total": v[0], "launches": v[1]} for k, v in prof.items() if v[1] > 0}
         n_mid = gpu.count()
         tuning = gpu.tuning()
         gpu.save_cells(st)
         rho = (st.c["alph"] * st.c["rho"]).sum(axis=0)
         gpu.close()
+        if mlp is not None:
+            mlp.close()
         return {"case": case, "el": el, "n_start": n_start, "n_end": n_end, "n_mid": n_mid,
-                "stages": stages, "rho": rho, "tuning": tuning}
+                "stages": stages, "rho": rho, "tuning": tuning, "iface": iface}
 
     plain = run_form(0)
     mixed = run_form(1)
@@ -642,11 +797,14 @@ def main():
             "same_particle_count": plain["n_end"] == R["n_end"]},
         "roofline": roof,
         "autotuned_workgroups": {"mixed": R["tuning"], "fp64": plain["tuning"]},
+        "interface_mgfm": R["iface"] or {"mode": "inline"},
         "stage_graphs": "RK stages replayed as HIP graphs in the timed steps",
     }
     if not args.no_horizon:
         line["error_horizon"] = [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
                                  for sz in ((64, 32, 32), (128, 64, 64))]
+    if not args.no_learned_step:
+        line["learned_guess_step"] = learned_step_bench(scheme=SCHEMES[args.scheme])
     if not args.no_tp:
         line["triple_point_112x48"] = triple_point_bench()
     if not args.no_mgfm:

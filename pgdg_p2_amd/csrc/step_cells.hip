@@ -448,7 +448,9 @@ __global__ void __launch_bounds__(BS) k_limiter(Geo g, CellDev C, const int32_t*
     for (int b = 0; b < 3; b++) G[a][b] = 0.;
   int bd_fg = 0;
   double vc[3] = {C.v[c], C.v[nc + c], C.v[2 * nc + c]};
-  const bool ux = g.sem == PGDG_SEM_UX;
+  // the underwater-explosion semantics run the FP64 form only (the mixed
+  // form refuses them): no UX code in the FP32 kernels
+  const bool ux = sizeof(T) == 8 && g.sem == PGDG_SEM_UX;
   {
     // UX takes a zero wall velocity on the x sides (UX:mpm.cpp:388-399);
     // T2/S3 the cell's own (a zero difference)
@@ -714,7 +716,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
   double xc[3];
   cell_center<D>(g, i, j, k, xc);
   const bool s3 = g.sem == PGDG_SEM_S3;
-  const bool ux = g.sem == PGDG_SEM_UX;
+  const bool ux = sizeof(T) == 8 && g.sem == PGDG_SEM_UX;  // FP64 form only (see k_limiter)
   const double hp[3] = {g.ps[0] / 2, g.ps[1] / 2, g.ps[2] / 2};
   const double vol64 = D == 3 ? g.ps[0] * g.ps[1] * g.ps[2] : g.ps[0] * g.ps[1];
   const T voli = (T)vol64;

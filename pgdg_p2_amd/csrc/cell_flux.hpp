@@ -105,6 +105,7 @@ __device__ inline size_t face_slot(const Geo& g, int side, int ss, int q, int u,
 // The MGFM solve of a ghost state, in place (the reference's call and
 // initial guess).
 struct MgfmInline {
+  static constexpr bool kInputsOnly = false;
   __device__ MgfmOut operator()(double ul, double ur, double pl, double pr, double rhol, double rhor, Eos ml,
                                 Eos mr, int variant) const {
     return mgfm_solve(ul, ur, pl, pr, rhol, rhor, ml, mr, variant, TOL_REF, 0.0, 50, -1.0);
@@ -112,7 +113,8 @@ struct MgfmInline {
 };
 
 // mg: the MGFM solve of the ghost state (MgfmInline, or a batched solve's
-// record).
+// record); with MG::kInputsOnly the flux stops after handing mg the solve's
+// inputs (Fh is left unset).
 template <int D, typename T, bool kSamePhase = false, typename MG = MgfmInline>
 __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss, int s1, int ax, T sgn,
                                   T* Fh, MG mg = MG{}) {
@@ -159,6 +161,7 @@ __device__ inline void iface_flux(const Geo& g, const T* Ui, const T* Uj, int ss
   for (int a = 0; a < D; a++) vnj += vj[a] * nrm[a];
   if (!kSamePhase && ss != s1) {
     MgfmOut o = mg(vni, vnj, P_i, P_j, rho_i, rho_j, me, Eos{g.gamma[s1], g.pinf[s1]}, s3 ? MGFM_S3 : MGFM_T2);
+    if constexpr (MG::kInputsOnly) return;
     rho_j = o.rho_l;
     P_j = o.pstar;
     vnj = o.u_l;

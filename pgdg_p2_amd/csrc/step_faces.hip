@@ -127,6 +127,37 @@ k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
   }
 }
 
+// The MGFM inputs of one interface point into record r.
+struct MgfmCollect {
+  static constexpr bool kInputsOnly = true;
+  double* in;
+  int32_t* pair;
+  int64_t ld, r;
+  int code;
+  __device__ MgfmOut operator()(double ul, double ur, double pl, double pr, double rl, double rr, Eos, Eos,
+                                int) const {
+    in[r] = ul;
+    in[ld + r] = ur;
+    in[2 * ld + r] = pl;
+    in[3 * ld + r] = pr;
+    in[4 * ld + r] = rl;
+    in[5 * ld + r] = rr;
+    pair[r] = code;
+    return MgfmOut{rl, pl, ul, 0};
+  }
+};
+
+// The batched solve's solution of record r.
+struct MgfmRecord {
+  static constexpr bool kInputsOnly = false;
+  const double* out;
+  const int32_t* it;
+  int64_t ld, r;
+  __device__ MgfmOut operator()(double, double, double, double, double, double, Eos, Eos, int) const {
+    return MgfmOut{out[r], out[ld + r], out[2 * ld + r], it[r]};
+  }
+};
+
 // How k_face_general meets the MGFM solves of its interface points:
 // FG_INLINE solves each in place; FG_COLLECT reserves the face's records
 // and writes the solves' inputs there (no fluxes); FG_APPLY takes the
@@ -232,31 +263,13 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
             eval(cfi, lmi, phi, Ui);
             eval(cfj, lmj, phj, Uj);
             if constexpr (MODE == FG_COLLECT) {
-              if (s1 != ss) {
-                const int64_t r = rec++;
-                const int code = ss * PGDG_MAX_PHASE + s1;
+              if (s1 != ss)
                 iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh,
-                                     [&](double ul, double ur, double pl, double pr, double rl, double rr, Eos,
-                                         Eos, int) {
-                                       C.mg_in[r] = ul;
-                                       C.mg_in[ld + r] = ur;
-                                       C.mg_in[2 * ld + r] = pl;
-                                       C.mg_in[3 * ld + r] = pr;
-                                       C.mg_in[4 * ld + r] = rl;
-                                       C.mg_in[5 * ld + r] = rr;
-                                       C.mg_pair[r] = code;
-                                       return MgfmOut{rl, pl, ul, 0};
-                                     });
-              }
+                                     MgfmCollect{C.mg_in, C.mg_pair, ld, rec++, ss * PGDG_MAX_PHASE + s1});
               continue;  // no fluxes in this pass
             } else if constexpr (MODE == FG_APPLY) {
               if (rec >= 0 && s1 != ss) {
-                const int64_t r = rec++;
-                iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh,
-                                     [&](double, double, double, double, double, double, Eos, Eos, int) {
-                                       return MgfmOut{C.mg_out[r], C.mg_out[ld + r], C.mg_out[2 * ld + r],
-                                                      C.mg_it[r]};
-                                     });
+                iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh, MgfmRecord{C.mg_out, C.mg_it, ld, rec++});
               } else {
                 iface_flux<D, float>(g, Ui, Uj, ss, s1, ax, sgn, Fh);
               }

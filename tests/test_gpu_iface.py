@@ -64,6 +64,38 @@ def test_iface_batch_equals_inline(cuda):
 
 
 @pytest.mark.gpu
+def test_iface_batch_equals_inline_three_phase_2d(cuda):
+    # the T2/T3 MGFM variant (initial p* (pL + pR) / 2), three media (six
+    # ordered pairs), P2 / SSP-RK3 with splits and merges, and a 2-D grid
+    from pgdg_p2_amd.compare import compare_states
+    from pgdg_p2_amd.step import GpuSim
+    case, st = cases.triple_point_2d(nx=112, ny=48, ppc_axis=3, jitter=0.25, nphase=3,
+                                     scheme=dict(order=2, rk_stages=3, ppc_min=1, ppc_max=27))
+    case.precision = 1
+    ga, gb = GpuSim(case, st), GpuSim(case, st)
+    gb.iface_mode(GpuSim.IFACE_BATCH)
+    a, b = st.copy(), st.copy()
+    codes = set()
+    try:
+        for k in range(20):
+            print(f"iface batch vs inline, three phases: step {k}", flush=True)
+            ga.step(1)
+            gb.step(1)
+            ga.save(a)
+            gb.save(b)
+            bad = compare_states(a, b)
+            assert not bad, f"step {k}: {bad[:6]}"
+            _, _, rpair, _ = gb.iface_records()
+            codes |= set(np.unique(rpair).tolist())
+        # records of at least two of the three interfaces, each pair code
+        # naming two different media
+        assert len(codes) >= 2 and all(c // 3 != c % 3 for c in codes), codes
+    finally:
+        ga.close()
+        gb.close()
+
+
+@pytest.mark.gpu
 def test_iface_learned_guess_in_step(cuda):
     from pgdg_p2_amd.mlp import Mlp, mgfm_pairs_dev
     from pgdg_p2_amd.step import GpuSim

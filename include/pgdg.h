@@ -121,9 +121,10 @@ int pgdg_root_dim(const double* x, double* y, int64_t n, int dim, void* stream);
  * ======================================================================== */
 
 /* A 4-layer MLP 12 -> 32 -> 32 -> 32 -> 2 (ReLU) mapping the standardised
- * face features (ln pL, ln pR, ln rhoL, ln rhoR, uL, uR, gammaL,
- * ln(1 + PinfL), gammaR, ln(1 + PinfR), ln(pL / pR), uL - uR; the last two
- * formed in FP64 so that small jumps survive the BF16 inputs) to the
+ * face features (ln pL, ln pR, ln rhoL, ln rhoR, uL, uR, (pL - pR) / (pL +
+ * pR), uL - uR, gammaL, ln(1 + PinfL), gammaR, ln(1 + PinfR); the two
+ * differences formed in FP64 so that small jumps survive the BF16 inputs)
+ * to the
  * standardised targets
  * (ln p* - ln max(pL, pR), u*).  Matrix products run on MFMA with BF16
  * operands and FP32 accumulation; the master parameters and the optimiser

@@ -156,10 +156,13 @@ __device__ inline void forward(FwdLds& s, Emit emit) {
   }
 }
 
-// ln(pL / pR) from the FP64 pressures: the two sides' small differences
-// survive the features' BF16 rounding (ln pL and ln pR alone lose them)
-__device__ inline float ln_ratio(double pl, double pr) {
-  return ln_fast((float)(fmax(pl, 1e-30) / fmax(pr, 1e-30)));
+// (pL - pR) / (pL + pR) with the difference and the sum formed in FP64:
+// the two sides' small pressure differences survive the features' BF16
+// rounding (ln pL and ln pR alone lose them); the quotient needs only the
+// hardware reciprocal
+__device__ inline float p_contrast(double pl, double pr) {
+  const double d = pl - pr, t = pl + pr;
+  return t > 0 ? (float)d * __builtin_amdgcn_rcpf((float)t) : 0.f;
 }
 
 // standardised face features (FP32, padded to 16)
@@ -170,9 +173,9 @@ __device__ inline void face_features(const double* __restrict__ in, int64_t n, i
   // the features enter the network as BF16: FP32 logarithms are ample
   const float raw[kIn] = {ln_fast(fmaxf((float)pl, 1e-30f)), ln_fast(fmaxf((float)pr, 1e-30f)),
                           ln_fast(fmaxf((float)rl, 1e-30f)), ln_fast(fmaxf((float)rr, 1e-30f)),
-                          (float)ul, (float)ur, (float)ml.gamma,
+                          (float)ul, (float)ur, p_contrast(pl, pr), (float)(ul - ur), (float)ml.gamma,
                           (float)log1p(fmax(ml.pinf, 0.0)), (float)mr.gamma,
-                          (float)log1p(fmax(mr.pinf, 0.0)), ln_ratio(pl, pr), (float)(ul - ur)};
+                          (float)log1p(fmax(mr.pinf, 0.0))};
 #pragma unroll
   for (int k = 0; k < kIn; k++) f[k] = (raw[k] - nm.feat_mu[k]) / nm.feat_sd[k];
 #pragma unroll
@@ -304,7 +307,7 @@ struct MlpRegs {
 };
 
 // Inputs of face i for the lanes of a tile: group 0 (pL, pR, rhoL, rhoR),
-// group 1 (uL, uR), group 2 (pL, pR, uL, uR); 1.0 past the end.
+// group 1 (uL, uR, pL, pR); 1.0 past the end.
 __device__ inline void tile_inputs(const double* __restrict__ in, int64_t n, int64_t i, bool live, double* v) {
   const int g = (threadIdx.x & 63) >> 4;
   v[0] = v[1] = v[2] = v[3] = 1.0;
@@ -317,16 +320,13 @@ __device__ inline void tile_inputs(const double* __restrict__ in, int64_t n, int
   } else if (g == 1) {
     v[0] = in[i];
     v[1] = in[n + i];
-  } else if (g == 2) {
-    v[0] = in[2 * n + i];
-    v[1] = in[3 * n + i];
-    v[2] = in[i];
-    v[3] = in[n + i];
+    v[2] = in[2 * n + i];
+    v[3] = in[3 * n + i];
   }
 }
 
 // Standardised features 4 g .. 4 g + 3 of a face from its lane's inputs
-// (fx: the standardised EOS features 6..9 of the face's pair of media).
+// (fx: the standardised EOS features 8..11 of the face's pair of media).
 __device__ inline void tile_features(const double* v, bool live, const pgdg_mlp_norm& nm, float4 fx, float* x) {
   const int g = (threadIdx.x & 63) >> 4;
   x[0] = x[1] = x[2] = x[3] = 0.f;
@@ -338,13 +338,13 @@ __device__ inline void tile_features(const double* v, bool live, const pgdg_mlp_
   } else if (g == 1) {
     x[0] = ((float)v[0] - nm.feat_mu[4]) * __builtin_amdgcn_rcpf(nm.feat_sd[4]);
     x[1] = ((float)v[1] - nm.feat_mu[5]) * __builtin_amdgcn_rcpf(nm.feat_sd[5]);
-    x[2] = fx.x;
-    x[3] = fx.y;
+    x[2] = (p_contrast(v[2], v[3]) - nm.feat_mu[6]) * __builtin_amdgcn_rcpf(nm.feat_sd[6]);
+    x[3] = ((float)(v[0] - v[1]) - nm.feat_mu[7]) * __builtin_amdgcn_rcpf(nm.feat_sd[7]);
   } else if (g == 2) {
-    x[0] = fx.z;
-    x[1] = fx.w;
-    x[2] = (ln_ratio(v[0], v[1]) - nm.feat_mu[10]) * __builtin_amdgcn_rcpf(nm.feat_sd[10]);
-    x[3] = ((float)(v[2] - v[3]) - nm.feat_mu[11]) * __builtin_amdgcn_rcpf(nm.feat_sd[11]);
+    x[0] = fx.x;
+    x[1] = fx.y;
+    x[2] = fx.z;
+    x[3] = fx.w;
   }
 }
 
@@ -768,9 +768,9 @@ __global__ void k_mlp_moments(const double* __restrict__ in, const double* __res
     const int c = pair ? pair[i] : 0;
     const Eos ml = tab.l[c], mr = tab.r[c];
     const double v[NV] = {log(fmax(pl, 1e-30)), log(fmax(pr, 1e-30)), log(fmax(rl, 1e-30)),
-                          log(fmax(rr, 1e-30)), ul, ur, ml.gamma, log1p(fmax(ml.pinf, 0.0)),
-                          mr.gamma, log1p(fmax(mr.pinf, 0.0)), log(fmax(pl, 1e-30) / fmax(pr, 1e-30)),
-                          ul - ur, log(out[n + i]) - log(pm), out[2 * n + i]};
+                          log(fmax(rr, 1e-30)), ul, ur, pl + pr > 0 ? (pl - pr) / (pl + pr) : 0.0, ul - ur,
+                          ml.gamma, log1p(fmax(ml.pinf, 0.0)), mr.gamma, log1p(fmax(mr.pinf, 0.0)),
+                          log(out[n + i]) - log(pm), out[2 * n + i]};
     for (int q = 0; q < NV; q++) {
       acc[q] += v[q];
       acc[NV + q] += v[q] * v[q];
@@ -875,7 +875,7 @@ float4 eos_features(const pgdg_mlp_norm& nm, const pgdg_eos_pair* eos) {
   const double c[4] = {eos->gamma_l, std::log1p(std::max(eos->pinf_l, 0.0)), eos->gamma_r,
                        std::log1p(std::max(eos->pinf_r, 0.0))};
   float fx4[4];
-  for (int k = 0; k < 4; k++) fx4[k] = ((float)c[k] - nm.feat_mu[6 + k]) / nm.feat_sd[6 + k];
+  for (int k = 0; k < 4; k++) fx4[k] = ((float)c[k] - nm.feat_mu[8 + k]) / nm.feat_sd[8 + k];
   return make_float4(fx4[0], fx4[1], fx4[2], fx4[3]);
 }
 

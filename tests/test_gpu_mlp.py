@@ -100,8 +100,8 @@ def test_guess_kernel_matches_restatement(cuda):
     m = Mlp()
     Pm = R.init_params(9).astype(np.float32)
     nm = MlpNorm()
-    mu = [15.0, 15.0, 3.0, -1.0, 0.0, 0.0, 4.4, 20.0, 1.4, 0.0, 0.0, 0.0]
-    sd = [4.0, 4.0, 1.0, 2.0, 300.0, 300.0, 1.0, 1.0, 1.0, 1.0, 5.0, 400.0]
+    mu = [15.0, 15.0, 3.0, -1.0, 0.0, 0.0, 0.0, 0.0, 4.4, 20.0, 1.4, 0.0]
+    sd = [4.0, 4.0, 1.0, 2.0, 300.0, 300.0, 0.5, 400.0, 1.0, 1.0, 1.0, 1.0]
     nm.feat_mu[:], nm.feat_sd[:] = mu, sd
     nm.out_mu[:], nm.out_sd[:] = [-1.0, 0.0], [2.0, 100.0]
     m.set_params(Pm, nm)
@@ -116,10 +116,10 @@ def test_guess_kernel_matches_restatement(cuda):
                     np.log(np.maximum(inp[4].astype(f32), f32(1e-30))),
                     np.log(np.maximum(inp[5].astype(f32), f32(1e-30))),
                     inp[0].astype(f32), inp[1].astype(f32),
+                    (inp[2] - inp[3]).astype(f32) * (1 / (inp[2] + inp[3]).astype(f32)),
+                    (inp[0] - inp[1]).astype(f32),
                     np.full(n, eos[0], f32), np.full(n, np.log1p(eos[1]), f32),
-                    np.full(n, eos[2], f32), np.full(n, np.log1p(eos[3]), f32),
-                    np.log((np.maximum(inp[2], 1e-30) / np.maximum(inp[3], 1e-30)).astype(f32)),
-                    (inp[0] - inp[1]).astype(f32)], axis=1)
+                    np.full(n, eos[2], f32), np.full(n, np.log1p(eos[3]), f32)], axis=1)
     X = np.zeros((n, R.NPAD))
     X[:, :R.NIN] = (raw - np.array(mu, f32)) / np.array(sd, f32)
     y, _ = R.forward(Pm.astype(np.float64), X, emulate=True)

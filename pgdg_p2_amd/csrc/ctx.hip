@@ -332,12 +332,13 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
                                            std::max((size_t)cs.n[0] * cs.n[2], (size_t)cs.n[0] * cs.n[1]))
                                 : 0;
     const size_t nbflux = (size_t)F * 6 * nqf * U * plane;
-    ff_bytes = sizeof(float) * (nflux + nbflux) + sizeof(int32_t) * ((size_t)D * nc + 64);
+    ff_bytes = sizeof(float) * (nflux + nbflux) + sizeof(int32_t) * ((size_t)D * nc + 64) + (size_t)D * nc;
     PGDG_HIP_TRY(hipMalloc(&ff_slab, ff_bytes));
     C.fflux = (float*)ff_slab;
     C.bflux = D == 3 ? C.fflux + nflux : nullptr;
     C.flist = (int32_t*)(C.fflux + nflux + nbflux);
     C.fcount = C.flist + (size_t)D * nc;
+    C.fkind = (int8_t*)(C.fcount + 64);
     // interface-point MGFM records for the batched solves: inputs [6] and
     // solutions [3] (FP64), pair code and iterations per record, the first
     // record of each listed face [D][nc], the record count and its peak

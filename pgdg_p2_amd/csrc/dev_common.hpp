@@ -119,9 +119,13 @@ struct CellDev {
   double *coef, *coef0, *dcoef, *lim, *alph, *rho, *E, *c, *ac, *rhov, *v, *a, *P, *G;
   int32_t *ph_state, *ph_state_b;
   // mixed form only: interior-face point fluxes, the faces left to the
-  // general face kernel and their count
+  // general face kernel and their count; fkind[ax][c] is 1 where the face
+  // (c, +ax) holds one flux per point for both sides, unweighted (the side-0
+  // slots; each side multiplies by the other cell's volume fraction), 0 where
+  // each side has its own, weighted, slots (the listed faces)
   float* fflux;
   int32_t *flist, *fcount;
+  int8_t* fkind;
   // mixed form, 3-D: domain-boundary face point fluxes [F][6][NQF][U][plane]
   float* bflux;
   // mixed form, interface points solved as a batch: the MGFM inputs of every

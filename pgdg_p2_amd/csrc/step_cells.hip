@@ -858,8 +858,12 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
         for (int u = 0; u < U; u++) DFn[m][u] = 0;
       if constexpr (kFaceBuf) {
         if (fg.b == 0) {  // interior face: the face kernels' point fluxes
-          const int side = fg.sgn > 0 ? 0 : 1;
           const int fc = fg.sgn > 0 ? c : fg.nbr;
+          // one flux for both sides (weighted here by the other cell's
+          // volume fraction) or this side's own, weighted, slots
+          const bool one = C.fkind[ax * nc + fc] != 0;
+          const int side = one ? 0 : (fg.sgn > 0 ? 0 : 1);
+          const T wo = one ? (T)C.alph[ss * nc + fg.nbr] : T(1);
 #pragma unroll 1
           for (int q = 0; q < NQF; q++) {
             double off[3], gwd;
@@ -873,7 +877,7 @@ k_rhs(Geo g, CellDev C, const int32_t* __restrict__ used, const DevScalars* __re
             const T wq = (T)gwd * fs;
 #pragma unroll
             for (int u = 0; u < U; u++) {
-              const T Fw = C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, fc)] * wq;
+              const T Fw = (C.fflux[face_slot<D, ORD>(g, side, ss, q, u, ax, fc)] * wo) * wq;
 #pragma unroll
               for (int bi = 0; bi < M; bi++) dc[u][bi] = fmaf(Fw, Nb[bi], dc[u][bi]);
             }
@@ -1395,10 +1399,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             const int ia = ax == 0 ? ci : (ax == 1 ? cj : ck);
             const int na = ia + (sgn > 0 ? 1 : -1);
             if (na >= 0 && na < g.N[ax]) {
-              const int side = sgn > 0 ? 0 : 1;
               const int fc = sgn > 0 ? c : c - st;
+              const bool one = C.fkind[ax * nc + fc] != 0;
+              const int side = one ? 0 : (sgn > 0 ? 0 : 1);
+              const float wo = one ? (float)C.alph[ss * nc + (sgn > 0 ? c + st : c - st)] : 1.f;
 #pragma unroll
-              for (int u = 0; u < U; u++) Fo[t][u] = C.fflux[face_slot<D, 2>(g, side, ss, q, u, ax, fc)];
+              for (int u = 0; u < U; u++) Fo[t][u] = C.fflux[face_slot<D, 2>(g, side, ss, q, u, ax, fc)] * wo;
             } else if (g.bc[f] != 0) {
               const int pidx = bplane_index(g, f, ci, cj, ck);
 #pragma unroll

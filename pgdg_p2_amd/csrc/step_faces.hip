@@ -39,8 +39,10 @@ __device__ inline void phase_masks(const Geo& g, const CellDev& C, int c, unsign
 }
 
 // One lane per cell c, its faces towards +x, +y(, +z).  Same-single-phase
-// faces: one HLLC per point, stored for both sides; other faces: appended
-// to the general list.  Only sides whose cell is in use are stored.
+// faces: one HLLC per point, stored once for both sides (fkind 1; each
+// cell's kernel weights it by the other cell's volume fraction); other
+// faces: appended to the general list (fkind 0).  Faces between two unused
+// cells are skipped.
 template <int D, int ORD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
@@ -82,9 +84,11 @@ k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
     unsigned pn, an;
     phase_masks(g, C, nb, pn, an);
     if (!(one && pn == an && pn == pc)) {
+      C.fkind[ax * nc + c] = 0;
       C.flist[atomicAdd(C.fcount, 1)] = ax * nc + c;
       continue;
     }
+    C.fkind[ax * nc + c] = 1;
     float cr[U][M], lr[U];
     const double* pr = C.coef + rows * ncz + nb;
 #pragma unroll
@@ -93,7 +97,6 @@ k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
 #pragma unroll
       for (int m = 0; m < M; m++) cr[u][m] = (float)pr[(size_t)(u * M + m) * ncz];
     }
-    const float wl = (float)C.alph[s * nc + c], wr = (float)C.alph[s * nc + nb];
 #pragma unroll 1
     for (int q = 0; q < NQF; q++) {
       double off[3], gwd;
@@ -119,10 +122,7 @@ k_face_simple(Geo g, CellDev C, const int32_t* __restrict__ used) {
       float Fh[U];
       iface_flux<D, float, true>(g, UL, UR, s, s, ax, 1.f, Fh);
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (uc) C.fflux[face_slot<D, ORD>(g, 0, s, q, u, ax, c)] = wr * Fh[u];
-        if (un) C.fflux[face_slot<D, ORD>(g, 1, s, q, u, ax, c)] = wl * Fh[u];
-      }
+      for (int u = 0; u < U; u++) C.fflux[face_slot<D, ORD>(g, 0, s, q, u, ax, c)] = Fh[u];
     }
   }
 }

@@ -453,7 +453,35 @@ def mls_bench(n=128, reps=5):
     return res
 
 
-def train_iface_guess(gpu):
+def _save_mlp(mlp, path):
+    import numpy as np
+    Pm, nm = mlp.get_params()
+    np.savez(path, params=Pm, feat_mu=np.array(nm.feat_mu), feat_sd=np.array(nm.feat_sd),
+             out_mu=np.array(nm.out_mu), out_sd=np.array(nm.out_sd), keep=np.float32(nm.keep))
+
+
+def _load_mlp(path):
+    import numpy as np
+    from pgdg_p2_amd.mlp import Mlp, MlpNorm
+    z = np.load(path)
+    nm = MlpNorm()
+    for f in ("feat_mu", "feat_sd", "out_mu", "out_sd"):
+        getattr(nm, f)[:] = z[f].tolist()
+    nm.keep = float(z["keep"]) if "keep" in z else 0.0
+    mlp = Mlp()
+    mlp.set_params(z["params"], nm)
+    return mlp
+
+
+def load_iface_guess(gpu, path):
+    """Switch the step to the learned initial guess of a saved network."""
+    from pgdg_p2_amd.step import GpuSim
+    mlp = _load_mlp(path)
+    gpu.iface_mode(GpuSim.IFACE_LEARNED, mlp)
+    return {"trained_network": str(path), "mlp": mlp}
+
+
+def train_iface_guess(gpu, save=None):
     """Train the network on the interface-point records of the stage just
     solved (the step's own exact MGFM solves, on the GPU) and switch the
     step to the learned initial guess."""
@@ -482,6 +510,8 @@ def train_iface_guess(gpu):
     res["train_loss_first_last"] = [float(hist[:8].mean()), float(hist[-8:].mean())]
     _, nm = mlp.get_params()
     res["keep_below_abs_ln_correction"] = float(nm.keep)
+    if save:
+        _save_mlp(mlp, save)
     d_in.free()
     d_pair.free()
     gpu.iface_mode(GpuSim.IFACE_LEARNED, mlp)
@@ -611,6 +641,11 @@ def main():
                     help="mixed form: MGFM at the interface face points solved in the face kernel, "
                          "as one batch per stage, or as one batch from the network's initial p* "
                          "(trained on the first warm-up step's own solves)")
+    ap.add_argument("--iface-mlp-save", default=None,
+                    help="learned interface guess: write the network trained on the step's records (.npz)")
+    ap.add_argument("--iface-mlp-load", default=None,
+                    help="learned interface guess: use a saved network instead of training in this run "
+                         "(counter runs keep the training's thousands of launches out of the profiler)")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain copy kernel over this many MiB (byte-counter "
@@ -679,8 +714,10 @@ def main():
             gpu.step(1)
             iface = {"mode": iface_mode}
             taken = 1
-            if iface_mode == "learned":
-                iface.update(train_iface_guess(gpu))
+            if iface_mode == "learned" and args.iface_mlp_load:
+                iface.update(load_iface_guess(gpu, args.iface_mlp_load))
+            elif iface_mode == "learned":
+                iface.update(train_iface_guess(gpu, save=args.iface_mlp_save))
                 taken += iface.pop("steps_taken")
             gpu.step(max(args.warmup - taken, 0))
         else:

@@ -376,14 +376,15 @@ int Ctx::init(const pgdg_case* c, int64_t cap, int64_t nwall) {
   PB.flag = (int32_t*)(PB.mass + PGDG_MAX_PHASE * (size_t)nc);
   size_t mb_bytes = 0;
   if (cs.precision == PGDG_PREC_MIXED) {
-    // boxes 32 B per cell; r1 (16 B, plus one spare record: the shifting
-    // loop reads one record ahead), r2 (16 B), r3 (8 B), ph, ecell (4 B)
-    // per entry
-    mb_bytes = 32 * (size_t)nc + (size_t)ne * (16 + 16 + 8 + 4 + 4) + 16 + 64;
+    // boxes and their unions along the cell lines 2 x 32 B per cell; r1
+    // (16 B, plus one spare record: the shifting loop reads one record
+    // ahead), r2 (16 B), r3 (8 B), ph, ecell (4 B) per entry
+    mb_bytes = 64 * (size_t)nc + (size_t)ne * (16 + 16 + 8 + 4 + 4) + 16 + 64;
     PGDG_HIP_TRY(hipMalloc(&mb_slab, mb_bytes));
     PGDG_HIP_TRY(hipMemset(mb_slab, 0, mb_bytes));
     char* q = (char*)mb_slab;
     MB.box = (float4*)q; q += 32 * (size_t)nc;
+    MB.ubox = (float4*)q; q += 32 * (size_t)nc;
     MB.r1 = (float4*)q; q += 16 * ((size_t)ne + 1);
     MB.r2 = (float4*)q; q += 16 * (size_t)ne;
     MB.r3 = (float2*)q; q += 8 * (size_t)ne;

@@ -100,7 +100,10 @@ struct MixedBins {
   float2* r3;       // density, sound speed
   int32_t* ph;      // phase (10 for walls, -1 removed)
   int32_t* ecell;
-  float4* box;  // [nc][2]
+  float4* box;   // [nc][2]: lower and upper corner of the entries' stage-start positions (cell frame)
+  float4* ubox;  // [nc][2]: x, y union of the boxes of the cells within the search radius along the
+                 // cell's line (k +- radius); z: the least lower and the largest upper z of those
+                 // boxes, each in its own cell's frame
   int64_t ne_cap;
 };
 

@@ -98,10 +98,34 @@ __global__ void k_cell_bbox_mixed(Geo g, MixedBins MB, const int* __restrict__ b
   MB.box[2 * c + 1] = hi;
 }
 
+// Union of the boxes of the cells k - radius .. k + radius of each cell's
+// line (3-D): a target whose position is more than the search radius away
+// from it (x, y from the union; z from the cells' least lower and largest
+// upper offsets within their own cells) reaches none of those cells.
+__global__ void k_cell_ubox_mixed(Geo g, MixedBins MB) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const int N2 = g.N[2], k = c % N2, line = c - k;
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+  float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  for (int m = max(k - g.radius, 0); m <= min(k + g.radius, N2 - 1); m++) {
+    const float4 a = MB.box[2 * (line + m)], b = MB.box[2 * (line + m) + 1];
+    lo.x = fminf(lo.x, a.x);
+    lo.y = fminf(lo.y, a.y);
+    lo.z = fminf(lo.z, a.z);
+    hi.x = fmaxf(hi.x, b.x);
+    hi.y = fmaxf(hi.y, b.y);
+    hi.z = fmaxf(hi.z, b.z);
+  }
+  MB.ubox[2 * c] = lo;
+  MB.ubox[2 * c + 1] = hi;
+}
+
 // Per-target constants of the FP32 pair terms.
 struct PstConst {
   float r2, r2_box, Cf, inv_h, dWc, inv_Wdp, tmp1_base, rhoi_f, cri;
-  float cw, ca;  // Cf * inv_Wdp and dWc * tmp1_base
+  float cw, ca;  // Cf * inv_Wdp and dWc * tmp1_base (applied to a line's sums)
+  float cwall;   // Cf / dWc: a wall term relative to ca
   int phi;
 };
 
@@ -126,7 +150,9 @@ __device__ __forceinline__ bool box_reach(const float4 blo, const float4 bhi, co
 // normals).  vb, vu: the target's stage-start and current positions in the
 // candidate cell's frame.  The terms are summed in FP32 within the cell
 // (pf, sf: at most the cell's entries) and the caller adds each cell's sums
-// to the FP64 totals.
+// to the FP64 totals; the particle-shifting sums leave out the per-target
+// factor ca (dW/r prefactor times the pressure term), which the caller
+// applies to each line's sum.
 template <int D, typename R1, typename R2, typename RP, typename G>
 __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const float* vu, const PstConst& k,
                                           R1 rec1, R2 rec2, RP recp, G glob, const MixedBins& MB,
@@ -171,7 +197,7 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
     const int phj = recp(x);
     const float vol_j = c2.w;
     if (phj != 10) {
-      const float fac = (k.ca * q3) * poly * vol_j;
+      const float fac = (q3 * poly) * vol_j;
 #pragma unroll
       for (int a = 0; a < D; a++) pst[a] = fmaf(dr[a], fac, pst[a]);
       if (phj != k.phi) {
@@ -183,7 +209,7 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
       }
     } else {
       const int w = ref_wall(B.ref[glob(x)]);
-      const float fw = (k.Cf * (q3 * q) * t2) * (k.tmp1_base * poly) * vol_j;
+      const float fw = (k.cwall * (q3 * q) * t2) * poly * vol_j;
 #pragma unroll
       for (int a = 0; a < D; a++) pst[a] = fmaf((float)W.norm[a * W.n + w], fw, pst[a]);
     }
@@ -207,11 +233,17 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
 // one or two (i, j) cell lines; for each candidate line offset the
 // candidate entries of the cells within reach of those lines (a contiguous
 // entry range per line) are staged in LDS, so that every lane reads its
-// candidates from LDS rather than through the vector memory path.  Each
-// lane still visits its own candidate cells (i, j, k ascending) and entries
-// in bin order, so the staged sum equals the per-lane walk (the path taken
-// in 2-D, for workgroups spanning more lines, candidate ranges wider than
-// the LDS tables, or with staging switched off) bit for bit.
+// candidates from LDS rather than through the vector memory path.  The
+// staged records are moved into one frame per line (z relative to the
+// middle cell of the staged range), so each lane walks a single window of
+// entries per candidate line: from the first to the last of its candidate
+// cells whose box is within reach (the cells between, out of reach, add no
+// pair).  The lanes of a wavefront step through their windows together,
+// where a walk cell by cell would step through the longest lane's cell at
+// every position.  The candidates and their order are the per-lane walk's
+// (the path taken in 2-D, for workgroups spanning more lines, candidate
+// ranges wider than the LDS tables, or with staging switched off); the
+// pair offsets differ by the FP32 rounding of the frames.
 template <int D, int BS>
 __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                    MixedBins MB, const int* __restrict__ bin_off,
@@ -255,6 +287,7 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
     k.tmp1_base = (float)(-1. * P_b * (1. / rhoi));
     k.cw = (float)(Cw / kern_W<D>(dxk, hi));
     k.ca = (float)(-5.0 * Cw / (hi * hi) * (-1. * P_b * (1. / rhoi)));
+    k.cwall = (float)(Cw / (-5.0 * Cw / (hi * hi)));
     k.rhoi_f = t3.x;
     k.cri = t3.y * t3.y * t3.x;
     const int pos = cell_of<D>(g, xbi);
@@ -262,16 +295,18 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
   }
   double pst[3] = {0, 0, 0}, sharp[3] = {0, 0, 0};
   float pf[3] = {0, 0, 0}, sf[3] = {0, 0, 0};  // the current cell's FP32 sums
+  const double ca_d = (double)k.ca;
   auto flush = [&]() {
 #pragma unroll
     for (int a = 0; a < D; a++) {
-      pst[a] += (double)pf[a];
+      pst[a] += (double)pf[a] * ca_d;
       sharp[a] += (double)sf[a];
       pf[a] = sf[a] = 0.f;
     }
   };
   const int r = g.radius;
   const float psf[3] = {(float)g.ps[0], (float)g.ps[1], (float)g.ps[2]};
+  const float ipsz = (float)(1.0 / g.ps[2]);
   // the target's positions in the frame of a candidate cell (t, s, m):
   // (target offset) - (cell offset) * cell size
   auto frame = [&](int t, int s, int m, float* vb, float* vu) {
@@ -335,6 +370,10 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
         }
       }
       const int kA_l = sg ? kA[1] : kA[0];
+      // the line frames: z relative to the lower corner of the middle cell
+      // of each line's staged cells
+      const int kM[2] = {(kA[0] + kB[0]) >> 1, (kA[1] + kB[1]) >> 1};
+      const int kM_l = sg ? kM[1] : kM[0];
       if (staged && (has[0] || has[1])) {
         for (int dI = di0 - r; dI <= di1 + r; dI++)
           for (int dJ = dj0 - r; dJ <= dj1 + r; dJ++) {
@@ -359,40 +398,66 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
                 const int c = cb[s_] + kA[s_] + lc;
                 s_off[s_][lc] = bin_off[c];
                 if (lc <= kB[s_] - kA[s_]) {
-                  s_box[s_][lc][0] = MB.box[2 * c];
-                  s_box[s_][lc][1] = MB.box[2 * c + 1];
+                  s_box[s_][lc][0] = MB.ubox[2 * c];
+                  s_box[s_][lc][1] = MB.ubox[2 * c + 1];
                 }
               }
             }
+            __syncthreads();  // s_off, s_box
             const int I = li + dI, J = lj + dJ;
-            const bool mine = active && (sg ? v[1] : v[0]) && abs(I - pi) <= r && abs(J - pj) <= r;
+            bool mine = active && (sg ? v[1] : v[0]) && abs(I - pi) <= r && abs(J - pj) <= r;
             const int base = sg == 0 ? 0 : n0[0];
             const int sh = (sg ? a0[1] : a0[0]) - base;  // entry index - staged index
+            float vb[3], vu[3];  // the target in the line frame
+            frame(I, J, kM_l, vb, vu);
+            // the lane's window: the entries of the candidate cells whose z
+            // range (kM frame) comes within rho of the target, rho the
+            // search radius left after the lateral distance to the union of
+            // the candidate cells' boxes; it holds every cell whose own box
+            // is within reach (the others' entries fail the list test)
+            int wx0 = 0, wx1 = 0;
+            if (mine) {
+              const int lp = pk - kA_l;
+              const float4 ulo = s_box[sg][lp][0], uhi = s_box[sg][lp][1];
+              const float gx = fmaxf(fmaxf(ulo.x - vb[0], vb[0] - uhi.x), 0.f);
+              const float gy = fmaxf(fmaxf(ulo.y - vb[1], vb[1] - uhi.y), 0.f);
+              const float rho2 = fmaf(-gx, gx, fmaf(-gy, gy, k.r2_box));
+              int m0 = max(pk - r, 0), m1 = min(pk + r, N2 - 1);
+              if (rho2 >= 0.f) {
+                const float rho = __builtin_amdgcn_sqrtf(rho2) * 1.0001f;
+                // cell m: (m - kM) psz + ulo.z <= vz + rho, (m - kM) psz + uhi.z >= vz - rho
+                m1 = min(m1, kM_l + (int)floorf((vb[2] + rho - ulo.z) * ipsz));
+                m0 = max(m0, kM_l + (int)ceilf((vb[2] - rho - uhi.z) * ipsz));
+              } else {
+                m1 = m0 - 1;
+              }
+              mine = m0 <= m1;
+              if (mine) {
+                wx0 = s_off[sg][m0 - kA_l] - sh;
+                wx1 = s_off[sg][m1 + 1 - kA_l] - sh;
+              }
+            }
             for (int q0 = 0; q0 < ntot; q0 += CAP) {
               const int qn = min(ntot - q0, CAP);
               for (int q = threadIdx.x; q < qn; q += BS) {
                 const int x = q0 + q;
-                const int f = x < n0[0] ? a0[0] + x : a0[1] + (x - n0[0]);
-                s_r1[q] = MB.r1[f];
-                s_r2[q] = MB.r2[f];
+                const bool l1 = x >= n0[0];
+                const int f = l1 ? a0[1] + (x - n0[0]) : a0[0] + x;
+                const float oz = (float)(MB.ecell[f] - (l1 ? cb[1] + kM[1] : cb[0] + kM[0])) * psf[2];
+                float4 c1 = MB.r1[f], c2 = MB.r2[f];
+                c1.z += oz;
+                c2.z += oz;
+                c1.w = fmaf(c1.x, c1.x, fmaf(c1.y, c1.y, c1.z * c1.z));
+                s_r1[q] = c1;
+                s_r2[q] = c2;
                 s_ph[q] = MB.ph[f];
               }
               __syncthreads();
               if (mine) {
-                const int m1 = min(pk + r, N2 - 1);
-                for (int m = max(pk - r, 0); m <= m1; m++) {
-                  const int lc = m - kA_l;
-                  const int x0 = max(s_off[sg][lc] - sh, q0);
-                  const int x1 = min(s_off[sg][lc + 1] - sh, q0 + qn);
-                  if (x0 >= x1) continue;
-                  float vb[3], vu[3];
-                  frame(I, J, m, vb, vu);
-                  if (!box_reach<D>(s_box[sg][lc][0], s_box[sg][lc][1], vb, k.r2_box)) continue;
-                  pst_pairs<D>(
-                      x0, x1, vb, vu, k, [&](int x) { return s_r1[x - q0]; },
-                      [&](int x) { return s_r2[x - q0]; }, [&](int x) { return s_ph[x - q0]; },
-                      [&](int x) { return x + sh; }, MB, B, W, pf, sf);
-                }
+                const int x0 = max(wx0, q0), x1 = min(wx1, q0 + qn);
+                pst_pairs<D>(
+                    x0, x1, vb, vu, k, [&](int x) { return s_r1[x - q0]; }, [&](int x) { return s_r2[x - q0]; },
+                    [&](int x) { return s_ph[x - q0]; }, [&](int x) { return x + sh; }, MB, B, W, pf, sf);
               }
               __syncthreads();
             }
@@ -521,6 +586,7 @@ int launch_pst_mixed(const Geo& g, const PtlDev& P, const WallDev& W, const Stag
     hipLaunchKernelGGL(k_bin_gather_mixed<D>, dim3((unsigned)((n_ent + 255) / 256)), dim3(256), 0, st, g,
                        P, W, S, B, MB, bin_off, bin_ent, sc, !bins_unchanged);
     hipLaunchKernelGGL(k_cell_bbox_mixed<D>, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB, bin_off);
+    if (D == 3) hipLaunchKernelGGL(k_cell_ubox_mixed, dim3((g.nc + 255) / 256), dim3(256), 0, st, g, MB);
     PGDG_HIP_TRY(hipGetLastError());
   }
   if (parts & 2) PGDG_TRY(launch_pst_mixed_kernel<D>(g, P, W, S, B, MB, n_ent, bin_off, sc, st, block, staging));

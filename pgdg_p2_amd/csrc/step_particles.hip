@@ -242,12 +242,42 @@ template <int D, int BS>
 __global__ void __launch_bounds__(BS) k_pst_binned(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                     const int* __restrict__ bin_off,
                                                     const DevScalars* __restrict__ sc) {
-  int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= sc->nent_dev) return;
+  const int64_t E0 = (int64_t)xcd_block() * BS;
+  const int64_t nent = sc->nent_dev;
+  if (E0 >= nent) return;  // the whole workgroup is past the entries
+  const int64_t nb = B.ne_cap;
+  int64_t e = E0 + threadIdx.x;
+  if constexpr (BS >= 128) {
+    // lanes dealt to wavefronts by the target's quadrant (x, y halves of
+    // its cell): a wavefront's lanes then sit at the same lateral offset
+    // from each candidate line, accept about as many candidates of each
+    // candidate cell and evaluate their pair terms together (each lane sums
+    // its own target's pairs in list order, whatever lane it runs on)
+    __shared__ int s_gcnt[5];
+    __shared__ int s_perm[BS];
+    if (threadIdx.x < 5) s_gcnt[threadIdx.x] = 0;
+    __syncthreads();
+    int key = 4;  // walls and past-the-end entries last
+    if (e < nent && B.ref[e] >= 0) {
+      key = 0;
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const double u = (B.xb[a * nb + e] - g.box_l[a]) / g.ps[a];
+        key = 2 * key + (u - floor(u) >= 0.5 ? 1 : 0);
+      }
+    }
+    const int rank = atomicAdd(&s_gcnt[key], 1);
+    __syncthreads();
+    int at = rank;
+    for (int u = 0; u < key; u++) at += s_gcnt[u];
+    s_perm[at] = threadIdx.x;
+    __syncthreads();
+    e = E0 + s_perm[threadIdx.x];
+  }
+  if (e >= nent) return;
   const int i = B.ref[e];
   if (i < 0) return;  // wall entry
   const int64_t cap = P.cap;
-  const int64_t nb = B.ne_cap;
   const double dt = sc->dt;
   const double max_v = sc->max_v;
   double xi[3], xbi[3];
@@ -292,15 +322,34 @@ __global__ void __launch_bounds__(BS) k_pst_binned(Geo g, PtlDev P, WallDev W, S
           // entry order
           const int nf = min(32, e1 - f0);
           uint32_t msk = 0;
-#pragma unroll 4
-          for (int k = 0; k < nf; k++) {
-            double sqb = 0;
+          // four candidates at a time: their loads are issued before any of
+          // them is tested, so the load latency is paid once per four
+          for (int k = 0; k < nf; k += 4) {
+            double sqb[4];
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-              double d = xbi[a] - B.xb[a * nb + f0 + k];
-              sqb += d * d;
+            for (int u = 0; u < 4; u++) {
+              const int f = min(f0 + k + u, e1 - 1);  // a repeat of the last candidate past the end
+              double d[3];
+#pragma unroll
+              for (int a = 0; a < D; a++) d[a] = xbi[a] - B.xb[a * nb + f];
+              double q = 0;
+#pragma unroll
+              for (int a = 0; a < D; a++) q += d[a] * d[a];
+              sqb[u] = q;
             }
-            if (sqb <= r2_in || (sqb <= r2_out && sqrt(sqb) <= rlim)) msk |= 1u << k;
+            bool band = false;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              if (k + u < nf && sqb[u] <= r2_in) msk |= 1u << (k + u);
+              band |= k + u < nf && sqb[u] > r2_in && sqb[u] <= r2_out;
+            }
+            // inside the rounding band the exact test takes the square root
+            // (rare: a wave-level branch)
+            if (__ballot(band) != 0 && band)
+#pragma unroll
+              for (int u = 0; u < 4; u++)
+                if (k + u < nf && sqb[u] > r2_in && sqb[u] <= r2_out && sqrt(sqb[u]) <= rlim)
+                  msk |= 1u << (k + u);
           }
           while (msk) {
             const int f = f0 + __builtin_ctz(msk);

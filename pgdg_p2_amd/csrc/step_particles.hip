@@ -351,20 +351,39 @@ __global__ void __launch_bounds__(BS) k_pst_binned(Geo g, PtlDev P, WallDev W, S
                 if (k + u < nf && sqb[u] > r2_in && sqb[u] <= r2_out && sqrt(sqb[u]) <= rlim)
                   msk |= 1u << (k + u);
           }
+          // the next accepted candidate's position, volume and phase are
+          // loaded while the current one's pair terms are evaluated
+          int fn = msk ? f0 + __builtin_ctz(msk) : -1;
+          double xn[3] = {0, 0, 0}, voln = 0;
+          int phn = 0;
+          if (fn >= 0) {
+#pragma unroll
+            for (int a = 0; a < D; a++) xn[a] = B.x[a * nb + fn];
+            voln = B.vol[fn];
+            phn = B.ph[fn];
+          }
           while (msk) {
-            const int f = f0 + __builtin_ctz(msk);
+            const int f = fn;
             msk &= msk - 1;
+            double xj[3] = {xn[0], xn[1], xn[2]};
+            const double vol_j = voln;
+            const int phj = phn;
+            fn = msk ? f0 + __builtin_ctz(msk) : -1;
+            if (fn >= 0) {
+#pragma unroll
+              for (int a = 0; a < D; a++) xn[a] = B.x[a * nb + fn];
+              voln = B.vol[fn];
+              phn = B.ph[fn];
+            }
             double dr[3];
 #pragma unroll
-            for (int a = 0; a < D; a++) dr[a] = xi[a] - B.x[a * nb + f];
+            for (int a = 0; a < D; a++) dr[a] = xi[a] - xj[a];
             double sq = 0;
 #pragma unroll
             for (int a = 0; a < D; a++) sq += dr[a] * dr[a];
             double dist = sqrt(sq);
             if (dist != 0) {
-              const int phj = B.ph[f];
               const bool wl = phj == 10;
-              double vol_j = B.vol[f];
               double aw = kern_dW<D>(dist, hi);
               double Wij = kern_W<D>(dist, hi);
               double rr = Wij / W_dp;

@@ -551,6 +551,73 @@ def iface_iterations(gpu, mlp):
     return res
 
 
+def late_window_bench(size=(256, 128, 128), before=190, steps=10, scheme=None, iface="learned"):
+    """The headline step (mixed form, the same interface mode) late in the
+    200-step problem: `before` steps run untimed from the initial state
+    (the network of the learned interface guess trained on the first
+    step's records, as in the headline run), then `steps` steps timed one
+    by one, each with its split, merge, inlet and outlet counts (the
+    headline's window, right after the warm-up steps, has none of the
+    splits and merges of the later run)."""
+    import numpy as np
+    from pgdg_p2_amd import cases
+    from pgdg_p2_amd.step import GpuSim
+    nx, ny, nz = size
+    case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, ppc_axis=2, jitter=0.25, cap_factor=1.0,
+                                     scheme=scheme)
+    case.precision = 1
+    n0 = st.n
+    gpu = GpuSim(case, st, cap=int(n0 * 2.2) + 4096)
+    t_start = time.perf_counter()
+    taken, mlp = 0, None
+    if iface != "inline":
+        gpu.iface_mode(GpuSim.IFACE_BATCH)
+        gpu.step(1)
+        taken = 1
+        if iface == "learned":
+            tr = train_iface_guess(gpu)
+            taken += tr.get("steps_taken", 0)
+            mlp = tr.get("mlp")
+    names = ("split", "merge", "inlet", "outlet")
+    ev_all = dict.fromkeys(names, 0)
+
+    def add_events(acc):
+        sc = gpu.scalars()
+        for k, v in zip(names, (sc.n_split, sc.n_merge, sc.n_inlet, sc.n_outlet)):
+            acc[k] += int(v)
+
+    done = taken
+    while done < before:  # one step at a time, for the event counts
+        gpu.step(1)
+        add_events(ev_all)
+        done += 1
+        if done % 50 == 0:
+            print(f"[late] step {done}: {gpu.count()} particles", file=sys.stderr, flush=True)
+    n_before = gpu.count()
+    ms, ev = [], dict.fromkeys(names, 0)
+    for _ in range(steps):
+        gpu.sync()
+        t0 = time.perf_counter()
+        gpu.step(1)
+        gpu.sync()
+        ms.append((time.perf_counter() - t0) * 1e3)
+        add_events(ev)
+    counts = gpu.cell_particle_counts()
+    n_after = gpu.count()
+    gpu.close()
+    if mlp is not None:
+        mlp.close()
+    nc = nx * ny * nz
+    ms_step = float(np.mean(ms))
+    return {"grid": f"{nx}x{ny}x{nz}", "interface_mgfm": iface, "timed_steps": [before + 1, before + steps],
+            "ms_per_step": ms_step, "ms_per_step_min_max": [float(min(ms)), float(max(ms))],
+            "cell_steps_per_s": nc / (ms_step * 1e-3),
+            "particles_start_end": [n0, n_before, n_after],
+            "events_in_window": ev, "events_before_window": ev_all,
+            "particles_per_cell_min_max": [int(counts.min()), int(counts.max())],
+            "wall_s": time.perf_counter() - t_start}
+
+
 def learned_step_bench(size=(128, 64, 64), x_shock=0.3, steps=10, scheme=None):
     """The learned-guess step where its interface points meet strong jumps:
     the shock-bubble channel with the shock starting inside the helium
@@ -624,6 +691,8 @@ def main():
     ap.add_argument("--no-3phase", action="store_true")
     ap.add_argument("--no-tp", action="store_true", help="skip the 112x48 triple-point leg")
     ap.add_argument("--no-horizon", action="store_true", help="skip the 200-step error horizon")
+    ap.add_argument("--no-late", action="store_true", help="skip the timed window late in the 200-step run")
+    ap.add_argument("--only-late", action="store_true")
     ap.add_argument("--3phase-warmup", dest="tp3_warmup", type=int, default=500)
     ap.add_argument("--only-3phase", action="store_true")
     ap.add_argument("--only-horizon", action="store_true")
@@ -676,6 +745,10 @@ def main():
         return
     if args.only_mgfm:
         print(json.dumps({"mgfm": mgfm_bench(mlp_save=args.mlp_save, mlp_load=args.mlp_load)}))
+        return
+    if args.only_late:
+        print(json.dumps({"late_window": late_window_bench(size=SIZES[args.size], scheme=SCHEMES[args.scheme],
+                                                           iface=args.iface)}))
         return
     if args.only_horizon:
         print(json.dumps({"error_horizon": [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
@@ -851,6 +924,9 @@ def main():
                                  if inline else None),
         "stage_graphs": "RK stages replayed as HIP graphs in the timed steps",
     }
+    if not args.no_late:
+        line["late_window"] = late_window_bench(size=SIZES[args.size], scheme=SCHEMES[args.scheme],
+                                                iface=args.iface)
     if not args.no_horizon:
         line["error_horizon"] = [error_horizon(size=sz, scheme=SCHEMES[args.scheme])
                                  for sz in ((64, 32, 32), (128, 64, 64))]

@@ -435,9 +435,12 @@ int pgdg_graphs_enable(pgdg_ctx* ctx, int on);
  * limiter kernels are chosen by timing each candidate in place (these
  * kernels overwrite their outputs, so the extra launches change nothing).
  * pgdg_tuning_read returns {shifting, RHS, limiter}; PGDG_ERR_STATE before
- * the tuning step has run.                                                */
+ * the tuning step has run.  pgdg_tuning_set fixes the three sizes (each 64,
+ * 128 or 256, else PGDG_ERR_ARG) and switches the autotuner off.  The
+ * step's results do not depend on the sizes.                              */
 int pgdg_autotune_enable(pgdg_ctx* ctx, int on);
 int pgdg_tuning_read(pgdg_ctx* ctx, int* blocks);
+int pgdg_tuning_set(pgdg_ctx* ctx, const int* blocks);
 /* Mixed form, 3-D: stage the particle-shifting candidates in LDS (default
  * on) or let every lane read its candidates from global memory.  Both give
  * the same sums bit for bit; the switch exists to compare them.           */

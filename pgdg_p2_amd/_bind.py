@@ -46,6 +46,7 @@ def declare(L: C.CDLL) -> None:
                                          C.POINTER(C.c_int32), C.c_double, C.POINTER(PointDiag)]
     L.pgdg_point_append.argtypes = [C.c_char_p, C.POINTER(Case), C.POINTER(PointDiag), C.c_int]
     L.pgdg_tuning_read.argtypes = [_CTX, C.POINTER(C.c_int)]
+    L.pgdg_tuning_set.argtypes = [_CTX, C.POINTER(C.c_int)]
     L.pgdg_profile_read.argtypes = [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
     for name in _STAGES:
         getattr(L, name).argtypes = [_CTX]

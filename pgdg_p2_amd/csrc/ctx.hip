@@ -1376,6 +1376,19 @@ int pgdg_tuning_read(pgdg_ctx* x, int* blocks) {
   return x->c.tuned ? PGDG_OK : PGDG_ERR_STATE;
 }
 
+int pgdg_tuning_set(pgdg_ctx* x, const int* blocks) {
+  if (!x || !blocks) return PGDG_ERR_ARG;
+  for (int q = 0; q < 3; q++)
+    if (blocks[q] != 64 && blocks[q] != 128 && blocks[q] != 256) return PGDG_ERR_ARG;
+  x->c.tune.pst = blocks[0];
+  x->c.tune.rhs = blocks[1];
+  x->c.tune.limiter = blocks[2];
+  x->c.autotune = false;
+  x->c.tuned = true;
+  x->c.drop_graphs();
+  return PGDG_OK;
+}
+
 int pgdg_profile_enable(pgdg_ctx* x, int on) {
   if (!x) return PGDG_ERR_ARG;
   x->c.prof.on = on != 0;

@@ -121,6 +121,20 @@ __global__ void k_cell_ubox_mixed(Geo g, MixedBins MB) {
   MB.ubox[2 * c + 1] = hi;
 }
 
+// 3-D: the frame of cell k's records in the candidate walk: z relative to
+// the lower corner of a fixed cell of k's line, so that every walk, staged
+// or not, at any workgroup size, forms the same FP32 offsets.  Lines of up
+// to kZLine cells (ZL): the middle cell of the line, one frame for the
+// whole line; longer lines: the middle cell of k's block of kZBlock cells
+// (a window that crosses a block edge switches frames there).  Offsets of
+// up to 64 cell sizes keep the rounding of a record's z near 4e-6 cell
+// sizes.
+constexpr int kZLine = 128, kZBlock = 64;
+template <bool ZL>
+__device__ __forceinline__ int zframe(int k, int n2) {
+  return ZL ? n2 >> 1 : (k & ~(kZBlock - 1)) + kZBlock / 2;
+}
+
 // Per-target constants of the FP32 pair terms.
 struct PstConst {
   float r2, r2_box, Cf, inv_h, dWc, inv_Wdp, tmp1_base, rhoi_f, cri;
@@ -153,20 +167,29 @@ __device__ __forceinline__ bool box_reach(const float4 blo, const float4 bhi, co
 // to the FP64 totals; the particle-shifting sums leave out the per-target
 // factor ca (dW/r prefactor times the pressure term), which the caller
 // applies to each line's sum.
-template <int D, typename R1, typename R2, typename RP, typename G>
-__device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const float* vu, const PstConst& k,
+template <int D, bool SW = false, typename R1, typename R2, typename RP, typename G>
+__device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb_, const float* vu_, const PstConst& k,
                                           R1 rec1, R2 rec2, RP recp, G glob, const MixedBins& MB,
-                                          const BinData& B, const WallDev& W, float* pst, float* sharp) {
+                                          const BinData& B, const WallDev& W, float* pst, float* sharp,
+                                          int xs = -1, float vbz2 = 0.f, float vuz2 = 0.f) {
   // list test |vb - cb|^2 <= r2 as |cb|^2 - 2 vb.cb <= r2 - |vb|^2 (the
-  // record carries |cb|^2)
-  float m2vb[3], thr = k.r2;
+  // record carries |cb|^2); from entry xs on the target's z positions are
+  // vbz2, vuz2 (the records from there on sit in the next block's frame)
+  float vu[3], m2vb[3], thr = k.r2;
 #pragma unroll
   for (int a = 0; a < D; a++) {
-    m2vb[a] = -2.f * vb[a];
-    thr = fmaf(-vb[a], vb[a], thr);
+    vu[a] = vu_[a];
+    m2vb[a] = -2.f * vb_[a];
+    thr = fmaf(-vb_[a], vb_[a], thr);
   }
   if (x0 >= x1) return;
   auto pair = [&](int x, const float4 c1) {
+    if (SW && x == xs) {  // (x, y of the frame are unchanged; -0.5 m2vb is vb exactly)
+      const float vbx = -0.5f * m2vb[0], vby = -0.5f * m2vb[1];
+      thr = fmaf(-vbz2, vbz2, fmaf(-vby, vby, fmaf(-vbx, vbx, k.r2)));
+      m2vb[2] = -2.f * vbz2;
+      vu[2] = vuz2;
+    }
     const float cb[3] = {c1.x, c1.y, c1.z};
     float t = c1.w;
 #pragma unroll
@@ -244,8 +267,8 @@ __device__ __forceinline__ void pst_pairs(int x0, int x1, const float* vb, const
 // (the path taken in 2-D, for workgroups spanning more lines, candidate
 // ranges wider than the LDS tables, or with staging switched off); the
 // pair offsets differ by the FP32 rounding of the frames.
-template <int D, int BS>
-__global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
+template <int D, int BS, bool ZL>
+__global__ void __launch_bounds__(BS, 4) k_pst_mixed(Geo g, PtlDev P, WallDev W, StageBufs S, BinData B,
                                                    MixedBins MB, const int* __restrict__ bin_off,
                                                    const DevScalars* __restrict__ sc, int staging) {
   const int64_t E0 = (int64_t)xcd_block() * BS;
@@ -370,10 +393,6 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
         }
       }
       const int kA_l = sg ? kA[1] : kA[0];
-      // the line frames: z relative to the lower corner of the middle cell
-      // of each line's staged cells
-      const int kM[2] = {(kA[0] + kB[0]) >> 1, (kA[1] + kB[1]) >> 1};
-      const int kM_l = sg ? kM[1] : kM[0];
       if (staged && (has[0] || has[1])) {
         for (int dI = di0 - r; dI <= di1 + r; dI++)
           for (int dJ = dj0 - r; dJ <= dj1 + r; dJ++) {
@@ -408,15 +427,18 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
             bool mine = active && (sg ? v[1] : v[0]) && abs(I - pi) <= r && abs(J - pj) <= r;
             const int base = sg == 0 ? 0 : n0[0];
             const int sh = (sg ? a0[1] : a0[0]) - base;  // entry index - staged index
-            float vb[3], vu[3];  // the target in the line frame
-            frame(I, J, kM_l, vb, vu);
             // the lane's window: the entries of the candidate cells whose z
-            // range (kM frame) comes within rho of the target, rho the
-            // search radius left after the lateral distance to the union of
-            // the candidate cells' boxes; it holds every cell whose own box
-            // is within reach (the others' entries fail the list test)
-            int wx0 = 0, wx1 = 0;
+            // range comes within rho of the target, rho the search radius
+            // left after the lateral distance to the union of the candidate
+            // cells' boxes; it holds every cell whose own box is within
+            // reach (the others' entries fail the list test).  A window
+            // that crosses the edge of a block of cells is walked in two
+            // segments, each with the target in its block's frame.
+            int wx0 = 0, wx1 = 0, wxb = 0;  // [wx0, wxb) first block, [wxb, wx1) second
+            float vb[3], vu[3], vbz2 = 0.f, vuz2 = 0.f;
             if (mine) {
+              const int kF = zframe<ZL>(pk, N2);
+              frame(I, J, kF, vb, vu);
               const int lp = pk - kA_l;
               const float4 ulo = s_box[sg][lp][0], uhi = s_box[sg][lp][1];
               const float gx = fmaxf(fmaxf(ulo.x - vb[0], vb[0] - uhi.x), 0.f);
@@ -425,9 +447,9 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
               int m0 = max(pk - r, 0), m1 = min(pk + r, N2 - 1);
               if (rho2 >= 0.f) {
                 const float rho = __builtin_amdgcn_sqrtf(rho2) * 1.0001f;
-                // cell m: (m - kM) psz + ulo.z <= vz + rho, (m - kM) psz + uhi.z >= vz - rho
-                m1 = min(m1, kM_l + (int)floorf((vb[2] + rho - ulo.z) * ipsz));
-                m0 = max(m0, kM_l + (int)ceilf((vb[2] - rho - uhi.z) * ipsz));
+                // cell m: (m - kF) psz + ulo.z <= vz + rho, (m - kF) psz + uhi.z >= vz - rho
+                m1 = min(m1, kF + (int)floorf((vb[2] + rho - ulo.z) * ipsz));
+                m0 = max(m0, kF + (int)ceilf((vb[2] - rho - uhi.z) * ipsz));
               } else {
                 m1 = m0 - 1;
               }
@@ -435,6 +457,13 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
               if (mine) {
                 wx0 = s_off[sg][m0 - kA_l] - sh;
                 wx1 = s_off[sg][m1 + 1 - kA_l] - sh;
+                const int mb = ZL ? 0 : m1 & ~(kZBlock - 1);  // the second block's first cell
+                wxb = mb > m0 ? s_off[sg][mb - kA_l] - sh : wx0;
+                float vb2[3], vu2[3];
+                frame(I, J, zframe<ZL>(m1, N2), vb2, vu2);
+                vbz2 = vb2[2];
+                vuz2 = vu2[2];
+                frame(I, J, zframe<ZL>(m0, N2), vb, vu);
               }
             }
             for (int q0 = 0; q0 < ntot; q0 += CAP) {
@@ -443,7 +472,8 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
                 const int x = q0 + q;
                 const bool l1 = x >= n0[0];
                 const int f = l1 ? a0[1] + (x - n0[0]) : a0[0] + x;
-                const float oz = (float)(MB.ecell[f] - (l1 ? cb[1] + kM[1] : cb[0] + kM[0])) * psf[2];
+                const int m = MB.ecell[f] - (l1 ? cb[1] : cb[0]);
+                const float oz = (float)(m - zframe<ZL>(m, N2)) * psf[2];
                 float4 c1 = MB.r1[f], c2 = MB.r2[f];
                 c1.z += oz;
                 c2.z += oz;
@@ -454,10 +484,16 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
               }
               __syncthreads();
               if (mine) {
-                const int x0 = max(wx0, q0), x1 = min(wx1, q0 + qn);
-                pst_pairs<D>(
-                    x0, x1, vb, vu, k, [&](int x) { return s_r1[x - q0]; }, [&](int x) { return s_r2[x - q0]; },
-                    [&](int x) { return s_ph[x - q0]; }, [&](int x) { return x + sh; }, MB, B, W, pf, sf);
+                auto sr1 = [&](int x) { return s_r1[x - q0]; };
+                auto sr2 = [&](int x) { return s_r2[x - q0]; };
+                auto sph = [&](int x) { return s_ph[x - q0]; };
+                auto sgi = [&](int x) { return x + sh; };
+                // (a chunk that starts past wxb starts in the second frame:
+                // switch at its first entry)
+                const int x0 = max(wx0, q0);
+                const int xs = wxb > wx0 ? max(wxb, x0) : -1;
+                pst_pairs<D, !ZL>(x0, min(wx1, q0 + qn), vb, vu, k, sr1, sr2, sph, sgi, MB, B, W, pf, sf, xs,
+                                  vbz2, vuz2);
               }
               __syncthreads();
             }
@@ -476,7 +512,25 @@ __global__ void __launch_bounds__(BS) k_pst_mixed(Geo g, PtlDev P, WallDev W, St
           float vb[3], vu[3];
           frame(t, s, m, vb, vu);
           if (!box_reach<D>(MB.box[2 * cc], MB.box[2 * cc + 1], vb, k.r2_box)) continue;  // empty or out of reach
-          pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, grecp, gidx, MB, B, W, pf, sf);
+          if constexpr (D == 3) {
+            // the records moved into the block frame, as the staged walk has them
+            const float oz = (float)(m - zframe<ZL>(m, g.N[2])) * psf[2];
+            frame(t, s, zframe<ZL>(m, g.N[2]), vb, vu);
+            auto br1 = [&](int x) {
+              float4 c1 = MB.r1[x];
+              c1.z += oz;
+              c1.w = fmaf(c1.x, c1.x, fmaf(c1.y, c1.y, c1.z * c1.z));
+              return c1;
+            };
+            auto br2 = [&](int x) {
+              float4 c2 = MB.r2[x];
+              c2.z += oz;
+              return c2;
+            };
+            pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, br1, br2, grecp, gidx, MB, B, W, pf, sf);
+          } else {
+            pst_pairs<D>(bin_off[cc], bin_off[cc + 1], vb, vu, k, grec1, grec2, grecp, gidx, MB, B, W, pf, sf);
+          }
         }
         flush();  // the line's FP32 sums
       }
@@ -555,22 +609,31 @@ template <int D>
 int launch_pst_mixed_kernel(const Geo& g, const PtlDev& P, const WallDev& W, const StageBufs& S,
                             const BinData& B, const MixedBins& MB, int64_t n_ent, const int* bin_off,
                             const DevScalars* sc, hipStream_t st, int block, int staging) {
+  // 3-D lines of up to kZLine cells: one record frame per line
+  const bool zl = D == 3 && g.N[2] <= kZLine;
+#define PGDG_PST_MIXED(BSZ)                                                                               \
+  do {                                                                                                 \
+    if (zl)                                                                                            \
+      hipLaunchKernelGGL((k_pst_mixed<D, BSZ, true>), dim3(grid_xcd(n_ent, BSZ)), dim3(BSZ), 0, st, g, P, W, \
+                         S, B, MB, bin_off, sc, staging);                                              \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_pst_mixed<D, BSZ, false>), dim3(grid_xcd(n_ent, BSZ)), dim3(BSZ), 0, st, g, P,  \
+                         W, S, B, MB, bin_off, sc, staging);                                           \
+  } while (0)
   switch (block) {
     case 64:
-      hipLaunchKernelGGL((k_pst_mixed<D, 64>), dim3(grid_xcd(n_ent, 64)), dim3(64), 0, st, g, P, W, S, B,
-                         MB, bin_off, sc, staging);
+      PGDG_PST_MIXED(64);
       break;
     case 128:
-      hipLaunchKernelGGL((k_pst_mixed<D, 128>), dim3(grid_xcd(n_ent, 128)), dim3(128), 0, st, g, P, W, S,
-                         B, MB, bin_off, sc, staging);
+      PGDG_PST_MIXED(128);
       break;
     case 256:
-      hipLaunchKernelGGL((k_pst_mixed<D, 256>), dim3(grid_xcd(n_ent, 256)), dim3(256), 0, st, g, P, W, S,
-                         B, MB, bin_off, sc, staging);
+      PGDG_PST_MIXED(256);
       break;
     default:
       return PGDG_ERR_ARG;
   }
+#undef PGDG_PST_MIXED
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

@@ -122,6 +122,11 @@ class GpuSim:
         rc = self.L.pgdg_tuning_read(self.h, b)
         return {"pst": b[0], "rhs": b[1], "limiter": b[2]} if rc == 0 else None
 
+    def set_tuning(self, pst: int, rhs: int, limiter: int):
+        """Fix the workgroup sizes (64, 128 or 256 each); the autotuner is off."""
+        b = (C.c_int * 3)(pst, rhs, limiter)
+        check(self.L.pgdg_tuning_set(self.h, b), "pgdg_tuning_set")
+
     def step(self, n: int = 1):
         check(self.L.pgdg_step(self.h, n), "pgdg_step")
 

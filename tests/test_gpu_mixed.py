@@ -72,13 +72,10 @@ def test_mixed_step_p2_rk3_shock_bubble_3d(cuda):
 
 
 def _staged_vs_walk(case, st, steps, tag):
-    # the staged windows (records in one frame per candidate line) against
-    # the per-lane walk cell by cell (records in their cells' frames): the
-    # same candidates in the same order, the pair offsets rounded in other
-    # frames.  After the first step every position agrees within 1e-6
-    # particle spacings (a skipped or repeated candidate moves its target by
-    # far more); over the later steps the two stay within the mixed form's
-    # own bounds against the FP64 form.
+    # the staged windows and the per-lane walk visit the same candidates in
+    # the same order, with the records in the same fixed frames (blocks of
+    # cells along each line): every particle and cell field identical
+    from pgdg_p2_amd.compare import compare_states
     from pgdg_p2_amd.step import GpuSim
     case.precision = 1
     ga, gb = GpuSim(case, st), GpuSim(case, st)
@@ -91,46 +88,37 @@ def _staged_vs_walk(case, st, steps, tag):
             gb.step(1)
             ga.save(a)
             gb.save(b)
-            assert a.n == b.n, (k, a.n, b.n)
-            d = np.abs(a.live("x") - b.live("x")).max(axis=0) / case.p_sz
-            ra, rb = _mix_rho(a), _mix_rho(b)
-            l1 = np.abs(ra - rb).sum() / np.abs(rb).sum()
-            print(f"{tag}: step {k}: dx mean {d.mean():.3e} max {d.max():.3e}, density L1 {l1:.3e}")
-            if k == 0:
-                assert d.max() <= 1e-6, (k, d.max())
-            assert d.mean() <= 1e-5 and d.max() <= DX_MAX_TOL and l1 <= 1e-6, (k, d.mean(), d.max(), l1)
+            bad = compare_states(a, b)
+            assert not bad, f"step {k}: {bad[:6]}"
     finally:
         ga.close()
         gb.close()
 
 
 @pytest.mark.gpu
-def test_mixed_pst_staged_windows_match_per_lane_walk(cuda):
+def test_mixed_pst_staged_equals_per_lane_walk(cuda):
     # 64x32x32 P2 cells, 8 particles per cell: a workgroup's targets lie on
     # one or two cell lines (candidates staged in LDS) or, where cells hold
     # fewer particles, on more lines (every lane walks its candidates in
-    # global memory)
+    # global memory); both must give the per-lane sums bit for bit
     sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
     case, st = cases.shock_bubble_3d(nx=64, ny=32, nz=32, scheme=sch)
     _staged_vs_walk(case, st, 6, "staged-vs-walk")
 
 
 @pytest.mark.gpu
-def test_mixed_pst_staged_windows_match_walk_with_splits_and_merges(cuda):
+def test_mixed_pst_staged_equals_walk_with_splits_and_merges(cuda):
     # splits raise the particle count of some cells (candidate ranges wider
     # than the LDS tables are walked in global memory), merges and outlet
-    # removal leave removed entries in the bins; the forced events happen at
-    # the start of the first step, whose particle shifting is compared (the
-    # forced state puts many split and merge tests at their thresholds, so
-    # later steps' events follow the rounding)
+    # removal leave removed entries in the bins
     from forced_events import shock_bubble_forced
     sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
     case, st = shock_bubble_forced(nx=32, ny=16, nz=16, scheme=sch, big_frac=0.2)
-    _staged_vs_walk(case, st, 1, "staged-vs-walk-forced")
+    _staged_vs_walk(case, st, 3, "staged-vs-walk-forced")
 
 
 @pytest.mark.gpu
-def test_mixed_pst_staged_windows_match_walk_27_per_cell(cuda):
+def test_mixed_pst_staged_equals_walk_27_per_cell(cuda):
     # 27 particles per cell: a workgroup's targets span several cells of a
     # line and the candidate ranges are wide
     sch = dict(order=2, rk_stages=3, ppc_min=8, ppc_max=40)
@@ -179,3 +167,46 @@ def test_mixed_step_p2_rk3_shock_cylinder_3d(cuda):
     # particles per cell, inlet copies every step)
     sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
     _compare(lambda: cases.shock_cylinder_3d(ny=20, ppc_axis=2, jitter=0.25, scheme=sch), 12, "c3d-p2-mixed")
+
+
+@pytest.mark.gpu
+def test_mixed_pst_staged_equals_walk_long_lines(cuda):
+    # lines of 160 cells: the records sit in the frames of blocks of 64
+    # cells and a window that crosses a block edge switches frames there
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    case, st = cases.shock_bubble_3d(nx=16, ny=8, nz=160, scheme=sch)
+    _staged_vs_walk(case, st, 3, "staged-vs-walk-long-lines")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", [(64, 32, 32), (16, 8, 160)], ids=["line-frames", "block-frames"])
+def test_mixed_step_independent_of_workgroup_sizes(cuda, grid):
+    # the autotuner picks the workgroup sizes by timing, so the mixed step
+    # must give the same state at every size: 64-lane particle-shifting
+    # workgroups (windows staged for about eight target cells) against
+    # 256-lane ones (32 cells, other line spans and staging chunks), RHS
+    # and limiter at 64 and 256; lines of 32 cells (one record frame per
+    # line) and of 160 cells (frames of 64-cell blocks)
+    from pgdg_p2_amd.compare import compare_states
+    from pgdg_p2_amd.step import GpuSim
+    sch = dict(order=2, rk_stages=3, ppc_min=4, ppc_max=16)
+    nx, ny, nz = grid
+    case, st = cases.shock_bubble_3d(nx=nx, ny=ny, nz=nz, scheme=sch)
+    case.precision = 1
+    ga, gb = GpuSim(case, st), GpuSim(case, st)
+    ga.set_tuning(64, 64, 64)
+    gb.set_tuning(256, 256, 256)
+    a, b = st.copy(), st.copy()
+    try:
+        for k in range(4):
+            print(f"workgroup sizes {grid}: step {k}", flush=True)
+            ga.step(1)
+            gb.step(1)
+            ga.save(a)
+            gb.save(b)
+            bad = compare_states(a, b)
+            assert not bad, f"step {k}: {bad[:6]}"
+        assert ga.tuning() == {"pst": 64, "rhs": 64, "limiter": 64}
+    finally:
+        ga.close()
+        gb.close()

@@ -248,6 +248,7 @@ __global__ void __launch_bounds__(128) k_face_general(Geo g, CellDev C, const in
           if (s1 >= g.F) break;
           const float alph_j = (float)C.alph[s1 * nc + cj];
           if (alph_j == 0) continue;
+          if (MODE == FG_COLLECT && s1 == ss) continue;  // a same-phase pair holds no MGFM record
           float cfj[U][M], lmj[U];
           load_set(cj, s1, cfj, lmj);
 #pragma unroll 1

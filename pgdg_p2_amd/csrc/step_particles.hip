@@ -50,13 +50,26 @@ __global__ void k_in_ele(Geo g, PtlDev P, StageBufs S, int64_t n) {
 // md: 0 first stage, 1 Heun second stage, 2 and 3 the SSP-RK3 stages (the
 // cell update's stage codes).  Order 2: the particle's copy carries the
 // gradient of the cell polynomial at the particle and its second-order
-// coefficients.  copy == false skips the coefficient copy and the
+// coefficients.  COPY == false skips the coefficient copy and the
 // conserved state (momentum, energy, pressure, mass): within a time step
 // only P2G (first stage) and split/merge (between steps) read them, so the
-// values written by stages before the last are never read.
-template <int D, int ORD>
-__global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S,
-                                             const DevScalars* __restrict__ sc, int md, bool copy) {
+// values written by stages before the last are never read.  Without the
+// copy a lane holds no coefficient set (the sums take each coefficient as
+// it arrives): the copy kernel k_g2p_copy (last stage) holds ~150
+// registers, k_g2p (the other stages) fits kG2pWaves wavefronts per SIMD
+// for the coefficient gathers (6: 9.0 against 10.4 ms per step at 8, the
+// same at 5).  The copy's stores are non-temporal: nothing reads them
+// before the next step, and they would evict the cell coefficients the
+// gathers share.
+constexpr int kG2pWaves = 6;
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+template <int D, int ORD, bool COPY>
+__device__ __forceinline__ void g2p_body(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S,
+                                         const DevScalars* __restrict__ sc, int md) {
+  constexpr bool copy = COPY;
   constexpr int U = D + 2, M = Basis<D, ORD>::M;
   // XCD-chunked numbering: the particles of one cell sit a lattice row or
   // plane apart in particle order, so runs of neighbouring workgroups on one
@@ -106,6 +119,16 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
   double phi[M - 1];
   basis_tail<D, ORD>(dri, g.q12, phi);
   double Ui[U];
+  if constexpr (!COPY) {
+#pragma unroll
+    for (int t = 0; t < U; t++) {
+      const double* cf = C.coef + (((size_t)ss * U + t) * M) * nc + c;
+      double acc = 0;
+#pragma unroll
+      for (int b = 0; b < M - 1; b++) acc += cf[(size_t)(1 + b) * nc] * phi[b];
+      Ui[t] = cf[0] + acc;
+    }
+  } else {
 #pragma unroll
   for (int t = 0; t < U; t++) {
     double acc = 0;
@@ -124,8 +147,8 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
       for (int a = 0; a < D; a++) cw[1 + a] = gr[a];
     }
 #pragma unroll
-    for (int m = 0; m < M; m++)
-      if (copy) P.coef[(size_t)(t * M + m) * cap + i] = cw[m];
+    for (int m = 0; m < M; m++) st_stream(&P.coef[(size_t)(t * M + m) * cap + i], cw[m]);
+  }
   }
   double rho = Ui[0], E = Ui[U - 1];
   double rv[3] = {0, 0, 0};
@@ -145,7 +168,7 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
 #pragma unroll
       for (int t = 0; t < U; t++)
 #pragma unroll
-        for (int b = 1; b < M; b++) P.coef[(size_t)(t * M + b) * cap + i] = 0;
+        for (int b = 1; b < M; b++) st_stream(&P.coef[(size_t)(t * M + b) * cap + i], 0.0);
     rho = C.rho[ss * nc + c];
 #pragma unroll
     for (int a = 0; a < 3; a++) rv[a] = C.rhov[(ss * 3 + a) * nc + c];
@@ -162,11 +185,23 @@ __global__ void __launch_bounds__(256) k_g2p(Geo g, PtlDev P, CellDev C, StageBu
   P.h[i] = root_d<D>(vol) * kCellRatio / kCutOff;
   if (copy) {  // read only by P2G and split/merge, as the copy
 #pragma unroll
-    for (int a = 0; a < 3; a++) P.rhov[a * cap + i] = rv[a];
-    P.E[i] = E;
-    P.P[i] = Pp;
-    P.mass[i] = vol * rho;
+    for (int a = 0; a < 3; a++) st_stream(&P.rhov[a * cap + i], rv[a]);
+    st_stream(&P.E[i], E);
+    st_stream(&P.P[i], Pp);
+    st_stream(&P.mass[i], vol * rho);
   }
+}
+
+template <int D, int ORD>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kG2pWaves)))
+    k_g2p(Geo g, PtlDev P, CellDev C, StageBufs S, const DevScalars* __restrict__ sc, int md) {
+  g2p_body<D, ORD, false>(g, P, C, S, sc, md);
+}
+
+template <int D, int ORD>
+__global__ void __launch_bounds__(256) k_g2p_copy(Geo g, PtlDev P, CellDev C, StageBufs S,
+                                                  const DevScalars* __restrict__ sc, int md) {
+  g2p_body<D, ORD, true>(g, P, C, S, sc, md);
 }
 
 // ---------------------------------------------------------------------------
@@ -677,10 +712,14 @@ template <int D>
 int launch_g2p(const Geo& g, const PtlDev& P, const CellDev& C, const StageBufs& S, int64_t n,
                const DevScalars* sc, int md, bool copy, hipStream_t st) {
   if (n <= 0) return PGDG_OK;
-  if (g.ord == 2)
-    hipLaunchKernelGGL((k_g2p<D, 2>), dim3(grid_xcd(n, 256)), dim3(256), 0, st, g, P, C, S, sc, md, copy);
-  else
-    hipLaunchKernelGGL((k_g2p<D, 1>), dim3(grid_xcd(n, 256)), dim3(256), 0, st, g, P, C, S, sc, md, copy);
+  const dim3 grid(grid_xcd(n, 256));
+  if (g.ord == 2) {
+    if (copy) hipLaunchKernelGGL((k_g2p_copy<D, 2>), grid, dim3(256), 0, st, g, P, C, S, sc, md);
+    else hipLaunchKernelGGL((k_g2p<D, 2>), grid, dim3(256), 0, st, g, P, C, S, sc, md);
+  } else {
+    if (copy) hipLaunchKernelGGL((k_g2p_copy<D, 1>), grid, dim3(256), 0, st, g, P, C, S, sc, md);
+    else hipLaunchKernelGGL((k_g2p<D, 1>), grid, dim3(256), 0, st, g, P, C, S, sc, md);
+  }
   PGDG_HIP_TRY(hipGetLastError());
   return PGDG_OK;
 }

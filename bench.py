@@ -850,7 +850,8 @@ def main():
     achieved = alg / (per_launch_ms * 1e-3) / 1e9 if alg > 0 else None
     order = case.order
     tb = R["tuning"] or {}
-    kname = {"pst": f"pgdg::k_pst_mixed<3, {tb.get('pst', 256)}>",
+    zl = "true" if nz <= 128 else "false"  # one record frame per cell line (lines of up to 128 cells)
+    kname = {"pst": f"pgdg::k_pst_mixed<3, {tb.get('pst', 256)}, {zl}>",
              "rhs": f"pgdg::k_rhs<3, {order}, {tb.get('rhs', 64)}, float>",
              "p2g": f"pgdg::k_p2g<3, {order}>", "g2p": f"pgdg::k_g2p<3, {order}>"}.get(dom, "")
     traffic = profiled_traffic(kname, args.size, args.scheme)
